@@ -1,0 +1,231 @@
+/*
+ * mlh.h -- C ABI of the MI355X meshless-finite-volume (MFV) time step.
+ *
+ * Plain C types only (no torch, no HIP types): a host program in C, C++,
+ * Python/ctypes or any FFI binds these symbols from libmlh_hip.so.
+ * Every entry point returns an mlh_status (0 = OK).  All calls on one
+ * context are synchronous with respect to the caller and must come from one
+ * host thread; device work of a context runs on one HIP stream.
+ *
+ * Reference interface replaced (jammartin/meshlessHydro, demonstrator/):
+ *   Particles      include/Particles.h:28-240   (SoA state, per-stage methods)
+ *   MeshlessScheme include/MeshlessScheme.h:15-44, src/MeshlessScheme.cpp:21-254
+ *   Riemann        include/Riemann.h:15-43, src/Riemann.cpp:7-229
+ *   RiemannSolver  call contract at src/Riemann.cpp:84,93-94 (un-vendored upstream)
+ *   parameter.h    include/parameter.h:9-77 (compile-time flags -> mlh_params)
+ *   config.info    demonstrator/config.info:2-33 (runtime values -> mlh_params)
+ */
+#ifndef MLH_H
+#define MLH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLH_ABI_VERSION 1
+
+/* Status codes.  1, 3, 6, 9 are the reference's exit codes
+ * (src/Particles.cpp:351, :2252, :1824-1827/:1878-1881,
+ *  src/MeshlessScheme.cpp:95-97); the rest are new. */
+typedef enum mlh_status {
+    MLH_OK = 0,
+    MLH_ERR_NEIGHBOUR_OVERFLOW = 1,   /* MAX_NUM_INTERACTIONS exceeded        */
+    MLH_ERR_GHOST_OVERFLOW = 3,       /* MAX_NUM_GHOST_INTERACTIONS exceeded  */
+    MLH_ERR_NEGATIVE_FACE_PRESSURE = 6,
+    MLH_ERR_DUMP_OVERRUN = 9,
+    MLH_ERR_INVALID_ARGUMENT = 20,
+    MLH_ERR_DEVICE = 21,              /* HIP runtime error / no device        */
+    MLH_ERR_GRID = 22,                /* fewer than 3 search cells per axis   */
+    MLH_ERR_CALL_ORDER = 23,          /* stage called before its inputs exist */
+    MLH_ERR_NOT_TRAINED = 24          /* learned seed requested, no network   */
+} mlh_status;
+
+typedef enum mlh_precision {
+    MLH_FP64 = 0,   /* all arithmetic in FP64                                  */
+    MLH_FP32 = 1    /* FP32 pair arithmetic; FP64 for E_i accumulation and     *
+                     * inversion, position differences, the Riemann p* Newton  *
+                     * residual step and the flux sums / state update          */
+} mlh_precision;
+
+typedef enum mlh_seed {
+    MLH_SEED_STANDARD = 0,  /* Toro adaptive PVRS/TRRS/TSRS initial guess      */
+    MLH_SEED_LEARNED = 1    /* MLP log p* seed (BF16 MFMA inference)           */
+} mlh_seed;
+
+typedef enum mlh_limiter_abs {
+    /* The reference calls unqualified abs() on a double with only <cmath>
+     * in scope, which binds to int abs(int) (src/Particles.cpp:1416-1417,
+     * :1748-1749,:1754,:1759): the ratio is truncated toward zero, NaN and
+     * |x| >= 2^31 become INT_MIN (x86 cvttsd2si) and abs(INT_MIN) stays
+     * INT_MIN.  MLH_ABS_INT_X86 reproduces that; MLH_ABS_FABS is the
+     * intended fabs(). */
+    MLH_ABS_INT_X86 = 0,
+    MLH_ABS_FABS = 1
+} mlh_limiter_abs;
+
+/* Compile-time flags of parameter.h plus the config.info values, as one
+ * runtime struct.  Fill with mlh_params_default() and override. */
+typedef struct mlh_params {
+    int32_t dim;                    /* DIM (2 or 3)                           */
+    int32_t periodic;               /* PERIODIC_BOUNDARIES                    */
+    double box_min[3];              /* periodicBoxLimits lowerX..lowerZ       */
+    double box_max[3];              /* periodicBoxLimits upperX..upperZ       */
+    double kernel_size;             /* kernelSize (h; the cubic spline uses h/2) */
+    double gamma;                   /* gamma                                  */
+    double cfl;                     /* CFL                                    */
+    int32_t max_interactions;       /* MAX_NUM_INTERACTIONS                   */
+    int32_t max_ghost_interactions; /* MAX_NUM_GHOST_INTERACTIONS             */
+    int32_t slope_limiting;         /* SLOPE_LIMITING                         */
+    double beta;                    /* BETA                                   */
+    int32_t pairwise_limiter;       /* PAIRWISE_LIMITER                       */
+    double psi1, psi2;              /* PSI_1, PSI_2                           */
+    int32_t mfm;                    /* MESHLESS_FINITE_MASS                   */
+    int32_t enforce_flux_sym;       /* ENFORCE_FLUX_SYM                       */
+    int32_t move_particles;         /* MOVE_PARTICLES                         */
+    int32_t first_order_quad_point; /* FIRST_ORDER_QUAD_POINT                 */
+    int32_t debug_level;            /* DEBUG_LVL (face-pressure aborts)       */
+    int32_t limiter_abs;            /* mlh_limiter_abs                        */
+    int32_t quirk_3d_predictor;     /* 1: j-side uses vz[i] (Particles.cpp:1717-1719) */
+    int32_t precision;              /* mlh_precision                          */
+    int32_t riemann_seed;           /* mlh_seed                               */
+    double riemann_tol;             /* relative Newton step accepted as converged */
+    int32_t riemann_max_iter;       /* FP64 Newton iteration cap              */
+    int32_t learned_iters;          /* FP32 Newton iterations after the MLP seed */
+} mlh_params;
+
+/* Counters of soft conditions the reference only logs (WARN lines). */
+typedef struct mlh_diag {
+    int64_t neg_face_pressure_real;   /* Particles.cpp:1813-1834              */
+    int64_t neg_face_pressure_ghost;  /* Particles.cpp:1873-1882              */
+    int64_t vacuum_faces;             /* Riemann.cpp:112-114,125-127          */
+    int64_t nonconverged_faces;       /* Newton cap reached                   */
+    int64_t missing_mirror_pairs;     /* neighbour seen from one side only    */
+    int64_t learned_fallbacks;        /* learned seed rejected by FP64 test   */
+    int64_t newton_iterations;        /* total over all solved faces          */
+    int64_t faces_solved;             /* Riemann solves performed             */
+    int32_t max_neighbours;           /* largest real neighbour count          */
+    int32_t max_ghost_neighbours;     /* largest periodic-image neighbour count */
+} mlh_diag;
+
+typedef struct mlh_ctx mlh_ctx;
+
+/* ---- lifecycle ------------------------------------------------------- */
+void mlh_params_default(mlh_params *p);
+int mlh_version(void);
+/* Particles(int N) (Particles.cpp:67-149) + MeshlessScheme ctor
+ * (MeshlessScheme.cpp:8-19): allocates device arrays for n particles. */
+int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out);
+int mlh_destroy(mlh_ctx *ctx);
+const char *mlh_last_error(void);
+
+/* ---- state transfer (Particles public SoA fields, Particles.h:33-36,54) --
+ * z / vz are ignored (may be NULL) when dim == 2. */
+int mlh_upload_state(mlh_ctx *ctx, const double *x, const double *y,
+                     const double *z, const double *vx, const double *vy,
+                     const double *vz, const double *m, const double *u);
+int mlh_download_state(mlh_ctx *ctx, double *x, double *y, double *z,
+                       double *vx, double *vy, double *vz, double *m,
+                       double *u);
+
+/* ---- one full MFV step (MeshlessScheme.cpp:53-224 minus snapshot I/O) ----
+ * dt_fixed > 0 : use that step (ADAPTIVE_TIMESTEP 0, or a replayed dt);
+ * dt_fixed <= 0: CFL step from compGlobalTimestep (Particles.cpp:1446-1485);
+ *               when t_clip > 0 and t_now + dt >= t_clip the step becomes
+ *               t_clip - t_now (dump-time clipping, MeshlessScheme.cpp:98-101)
+ *               and *clipped is set to 1.
+ * *dt_used / *clipped receive the step taken (either may be NULL). */
+int mlh_step(mlh_ctx *ctx, double dt_fixed, double t_now, double t_clip,
+             double *dt_used, int32_t *clipped);
+
+/* Run n_steps steps with the step size kept on the device (no host round
+ * trip per step).  dt_fixed as in mlh_step; dts (may be NULL) receives the
+ * n_steps step sizes after the last step. */
+int mlh_run(mlh_ctx *ctx, int32_t n_steps, double dt_fixed, double *dts);
+
+/* ---- stages, for parity tests and the host Particles class ------------ */
+/* assignParticlesAndCells + createGhostParticles + gridNNS + ghostNNS
+ * (Particles.cpp:270-365, 2113-2191, 2237-2260). */
+int mlh_neighbours(mlh_ctx *ctx);
+/* compDensity (+ghost part) + compPressure (Particles.cpp:1151-1184,
+ * 2262-2290, 1272-1288). */
+int mlh_density(mlh_ctx *ctx);
+/* compGlobalTimestep (Particles.cpp:1446-1485). */
+int mlh_timestep(mlh_ctx *ctx, double *dt);
+/* compPsijTilde + gradient x(D+2) + slopeLimiter (Particles.cpp:1186-1270,
+ * 1313-1444, 2292-2502). */
+int mlh_gradients(mlh_ctx *ctx);
+/* compEffectiveFace + compRiemannStatesLR + solveRiemannProblems +
+ * collectFluxes (Particles.cpp:1290-1311, 1488-2011, 2504-2683); also
+ * prepares the updated state, committed by mlh_update. */
+int mlh_fluxes(mlh_ctx *ctx, double dt);
+/* updateStateAndPosition (Particles.cpp:2013-2110). */
+int mlh_update(mlh_ctx *ctx);
+
+/* ---- readbacks (host buffers owned by the caller) ---------------------
+ * Neighbour entries are packed int32: bits 0..26 particle index, bits
+ * 27..31 periodic-image code s = (sx+1) + 3(sy+1) + 9(sz+1) (13 = the real
+ * particle, no image).  Row i holds noi[i] real neighbours in the
+ * reference's stencil order followed by noi_ghost[i] image neighbours in
+ * ghost-index order; row stride = max_interactions + max_ghost_interactions. */
+int mlh_get_neighbours(mlh_ctx *ctx, int32_t *noi, int32_t *noi_ghost,
+                       int32_t *nbr);
+int mlh_get_density(mlh_ctx *ctx, double *rho, double *P, double *omega);
+/* grads: [(D+2) fields {rho, vx, vy, (vz), P}][N][D]; B: [N][D*D]. */
+int mlh_get_gradients(mlh_ctx *ctx, double *grads, double *B);
+/* mF [N], vF [N][D], eF [N] (collectFluxes, Particles.cpp:1913-2011). */
+int mlh_get_fluxes(mlh_ctx *ctx, double *mF, double *vF, double *eF);
+int mlh_get_diagnostics(mlh_ctx *ctx, mlh_diag *d);
+/* sumVolume, sumMass, sumEnergy, sumMomentumX/Y/Z (Particles.cpp:2830-2886)
+ * into out[0..5]. */
+int mlh_conserved_sums(mlh_ctx *ctx, double *out);
+/* Device milliseconds of the last mlh_step / mlh_run per kernel group:
+ * [0] cell list + neighbours, [1] density, [2] gradients, [3] faces. */
+int mlh_get_timings(mlh_ctx *ctx, float *ms4);
+
+/* ---- exact Riemann solver, batched (Riemann.cpp:83-141 contract) -------
+ * Solves n 1D problems (rhoL,uL,PL | rhoR,uR,PR) with gamma, sampled at
+ * x/t = 0.  Host arrays in, host arrays out.  flag: -1 left side sampled,
+ * +1 right side, 0 vacuum.  iters: Newton iterations used (may be NULL).
+ * precision/seed as in mlh_params; the learned seed needs a trained MLP in
+ * ctx (ctx may be NULL for MLH_SEED_STANDARD). */
+int mlh_riemann_exact_batch(mlh_ctx *ctx, int64_t n, double gamma,
+                            const double *rhoL, const double *uL,
+                            const double *PL, const double *rhoR,
+                            const double *uR, const double *PR,
+                            double *rho_s, double *u_s, double *P_s,
+                            int32_t *flag, int32_t *iters, int32_t precision,
+                            int32_t seed, double tol);
+
+/* ---- learned p* seed: MLP 10 -> 64 -> 64 -> 1 (tanh hidden) ------------ */
+#define MLH_MLP_IN 10
+#define MLH_MLP_HIDDEN 64
+#define MLH_MLP_NPARAM (MLH_MLP_IN * MLH_MLP_HIDDEN + MLH_MLP_HIDDEN + \
+                        MLH_MLP_HIDDEN * MLH_MLP_HIDDEN + MLH_MLP_HIDDEN + \
+                        MLH_MLP_HIDDEN + 1)
+/* Feature vector of one face (normalised states): feats[10]; writes the
+ * regression target log(p_star / p_ref) normaliser p_ref to *p_ref. */
+int mlh_mlp_features(int64_t n, double gamma, const double *rhoL,
+                     const double *uL, const double *PL, const double *rhoR,
+                     const double *uR, const double *PR, float *feats,
+                     double *p_ref);
+/* Set / get the network parameters (FP32, layout documented in DESIGN.md). */
+int mlh_mlp_set_params(mlh_ctx *ctx, const float *params);
+int mlh_mlp_get_params(mlh_ctx *ctx, float *params);
+/* Forward + backward on a batch (device kernels), loss = mean squared
+ * error of log p*; returns the loss and the gradient w.r.t. all params. */
+int mlh_mlp_loss_grad(mlh_ctx *ctx, int64_t n, const float *feats,
+                      const float *target, float *loss, float *grad);
+/* Train with Adam for n_epochs over (feats, target) resident on the device. */
+int mlh_mlp_train(mlh_ctx *ctx, int64_t n, const float *feats,
+                  const float *target, int32_t n_epochs, int32_t batch,
+                  float lr, uint64_t seed, float *final_loss);
+/* Inference: FP32 reference path (bf16 = 0) or BF16 MFMA path (bf16 = 1). */
+int mlh_mlp_infer(mlh_ctx *ctx, int64_t n, const float *feats, float *out,
+                  int32_t bf16);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MLH_H */

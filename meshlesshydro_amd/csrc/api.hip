@@ -1,0 +1,625 @@
+// api.hip -- the C ABI (include/mlh.h): context, device buffers, stage
+// sequencing, readbacks.  Host code; every kernel runs on the context's one
+// HIP stream.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include "../../include/mlh.h"
+#include "kernels.h"
+#include "mlp_api.h"
+
+using namespace mlh;
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return fail(MLH_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+struct mlh_ctx {
+    mlh_params p;
+    int N = 0, D = 2, prec = 0;
+    size_t tsz = 8;  // bytes of the derived-data type
+    int stride = 0;  // maxr + maxg
+    hipStream_t s = nullptr;
+    Geom g{};
+    double *x[3] = {}, *v[3] = {}, *m = nullptr, *u = nullptr;
+    double *xn[3] = {}, *vn[3] = {}, *mn = nullptr, *un = nullptr;
+    double4 *pos = nullptr;
+    uint32_t *cell_key = nullptr, *sorted_idx = nullptr, *sorted_key = nullptr,
+             *sort_scratch = nullptr;
+    int *cstart = nullptr, *cend = nullptr;
+    uint32_t *nbr = nullptr;
+    int *noi = nullptr, *nog = nullptr;
+    void *prim = nullptr, *frec = nullptr;
+    double *dt_block = nullptr, *dt = nullptr, *dt_hist = nullptr;
+    int dt_hist_cap = 0;
+    double *mF = nullptr, *vF = nullptr, *eF = nullptr;
+    unsigned long long *counters = nullptr;
+    int *status = nullptr, *max_counts = nullptr;
+    double *sums_block = nullptr;
+    MlpState mlp;
+    int stage = 0;  // 0 state, 1 neighbours, 2 density, 3 gradients, 4 fluxes
+    hipEvent_t ev[6] = {};
+    float timings[4] = {0, 0, 0, 0};
+    std::vector<void *> allocs;
+    int64_t step_count = 0;
+};
+
+extern "C" {
+
+void mlh_params_default(mlh_params *p) {
+    memset(p, 0, sizeof(*p));
+    p->dim = 2;
+    p->periodic = 1;
+    for (int a = 0; a < 3; ++a) { p->box_min[a] = 0.0; p->box_max[a] = 1.0; }
+    p->kernel_size = 0.04;
+    p->gamma = 5.0 / 3.0;
+    p->cfl = 0.4;
+    p->max_interactions = 400;
+    p->max_ghost_interactions = 300;
+    p->slope_limiting = 1;
+    p->beta = 4.0;
+    p->pairwise_limiter = 0;
+    p->psi1 = 0.5;
+    p->psi2 = 0.25;
+    p->mfm = 0;
+    p->enforce_flux_sym = 1;
+    p->move_particles = 1;
+    p->first_order_quad_point = 1;
+    p->debug_level = 1;
+    p->limiter_abs = MLH_ABS_INT_X86;
+    p->quirk_3d_predictor = 0;
+    p->precision = MLH_FP64;
+    p->riemann_seed = MLH_SEED_STANDARD;
+    p->riemann_tol = 1e-6;
+    p->riemann_max_iter = 50;
+    p->learned_iters = 3;
+}
+
+int mlh_version(void) { return MLH_ABI_VERSION; }
+
+const char *mlh_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"
+
+template <typename P>
+static int dalloc(mlh_ctx *c, P **ptr, size_t bytes) {
+    void *q = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return fail(MLH_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *ptr = (P *)q;
+    return 0;
+}
+
+static int make_geom(mlh_ctx *c) {
+    const mlh_params &p = c->p;
+    Geom &g = c->g;
+    g.periodic = p.periodic;
+    g.h = p.kernel_size;
+    g.hsq = p.kernel_size * p.kernel_size;
+    double h2 = p.kernel_size / 2.;
+    g.sigma = c->D == 2 ? 10. / (7. * M_PI * h2 * h2) : 1. / (M_PI * h2 * h2 * h2);
+    long long ncells = 1;
+    for (int a = 0; a < 3; ++a) {
+        g.bmin[a] = p.box_min[a];
+        g.bmax[a] = p.box_max[a];
+        g.nc[a] = 1;
+        g.cs[a] = 1.0;
+    }
+    for (int a = 0; a < c->D; ++a) {
+        double ext = g.bmax[a] - g.bmin[a];
+        int n = (int)floor(ext / p.kernel_size);
+        if (n < 3)
+            return fail(MLH_ERR_GRID, "fewer than 3 search cells per axis (box / kernelSize)");
+        g.nc[a] = n;
+        g.cs[a] = ext / (double)n;
+        ncells *= n;
+    }
+    if (ncells > (1ll << 27)) return fail(MLH_ERR_GRID, "too many search cells");
+    g.ncells = (int)ncells;
+    return 0;
+}
+
+static HydroArgs hydro_args(mlh_ctx *c) {
+    HydroArgs a{};
+    const mlh_params &p = c->p;
+    a.g = c->g;
+    a.N = c->N;
+    a.stride = c->N;
+    a.gamma = p.gamma; a.cfl = p.cfl; a.beta = p.beta; a.psi1 = p.psi1; a.psi2 = p.psi2;
+    a.riemann_tol = p.riemann_tol;
+    a.riemann_max_iter = p.riemann_max_iter;
+    a.learned_iters = p.learned_iters;
+    a.limiter_abs = p.limiter_abs;
+    a.slope_limiting = p.slope_limiting;
+    a.pairwise_limiter = p.pairwise_limiter;
+    a.mfm = p.mfm;
+    a.enforce_flux_sym = p.enforce_flux_sym;
+    a.move_particles = p.move_particles;
+    a.foqp = p.first_order_quad_point;
+    a.quirk3d = p.quirk_3d_predictor;
+    a.write_fluxes = 0;
+    a.learned = p.riemann_seed == MLH_SEED_LEARNED;
+    a.pos = c->pos;
+    a.nbr = c->nbr;
+    a.noi = c->noi;
+    a.nog = c->nog;
+    for (int k = 0; k < 3; ++k) {
+        a.x[k] = c->x[k]; a.v[k] = c->v[k];
+        a.xn[k] = c->xn[k]; a.vn[k] = c->vn[k];
+    }
+    a.m = c->m; a.u = c->u; a.mn = c->mn; a.un = c->un;
+    a.prim = c->prim;
+    a.frec = c->frec;
+    a.dt_block = c->dt_block;
+    a.dt = c->dt;
+    a.mF = c->mF; a.vF = c->vF; a.eF = c->eF;
+    a.counters = c->counters;
+    a.status = c->status;
+    a.mlp = c->mlp.params;
+    return a;
+}
+
+extern "C" {
+
+int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
+    if (!p || !out || n <= 0) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument or n <= 0");
+    if (p->dim != 2 && p->dim != 3) return fail(MLH_ERR_INVALID_ARGUMENT, "dim must be 2 or 3");
+    if (n > (int32_t)kIdxMask) return fail(MLH_ERR_INVALID_ARGUMENT, "n exceeds 2^27-1");
+    if (!p->periodic)
+        return fail(MLH_ERR_INVALID_ARGUMENT, "non-periodic domains are not supported yet");
+    if (p->max_interactions <= 0 || p->max_ghost_interactions < 0)
+        return fail(MLH_ERR_INVALID_ARGUMENT, "bad interaction capacities");
+    if (p->precision != MLH_FP64 && p->precision != MLH_FP32)
+        return fail(MLH_ERR_INVALID_ARGUMENT, "bad precision");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(MLH_ERR_DEVICE, "no HIP device");
+    mlh_ctx *c = new mlh_ctx();
+    c->p = *p;
+    c->N = n;
+    c->D = p->dim;
+    c->prec = p->precision;
+    c->tsz = c->prec == MLH_FP64 ? 8 : 4;
+    c->stride = p->max_interactions + p->max_ghost_interactions;
+    int st = make_geom(c);
+    if (st) { delete c; return st; }
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(MLH_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    size_t N = (size_t)n, dsz = sizeof(double) * N;
+    int D = c->D;
+    int frn = D == 2 ? FRec<2>::N : FRec<3>::N;
+#define A_(ptr, bytes) if ((st = dalloc(c, ptr, bytes))) { mlh_destroy(c); return st; }
+    for (int a = 0; a < D; ++a) {
+        A_(&c->x[a], dsz); A_(&c->v[a], dsz); A_(&c->xn[a], dsz); A_(&c->vn[a], dsz);
+    }
+    A_(&c->m, dsz); A_(&c->u, dsz); A_(&c->mn, dsz); A_(&c->un, dsz);
+    A_(&c->pos, sizeof(double4) * N);
+    A_(&c->cell_key, 4 * N); A_(&c->sorted_idx, 4 * N); A_(&c->sorted_key, 4 * N);
+    A_(&c->sort_scratch, 4 * (2 * N + radix_sort_scratch_words(n)));
+    A_(&c->cstart, 4 * (size_t)c->g.ncells); A_(&c->cend, 4 * (size_t)c->g.ncells);
+    A_(&c->nbr, 4 * N * (size_t)c->stride);
+    A_(&c->noi, 4 * N); A_(&c->nog, 4 * N);
+    A_(&c->prim, c->tsz * kPrimN * N);
+    A_(&c->frec, c->tsz * frn * N);
+    A_(&c->dt_block, sizeof(double) * dt_blocks(n));
+    A_(&c->dt, sizeof(double) * 4);
+    A_(&c->mF, dsz); A_(&c->vF, dsz * D); A_(&c->eF, dsz);
+    A_(&c->counters, sizeof(unsigned long long) * kNumCounters);
+    A_(&c->status, sizeof(int) * 2);
+    A_(&c->max_counts, sizeof(int) * 2);
+    A_(&c->sums_block, sizeof(double) * 6 * (size_t)dt_blocks(n));
+#undef A_
+    hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * kNumCounters, c->s);
+    hipMemsetAsync(c->status, 0, sizeof(int) * 2, c->s);
+    hipMemsetAsync(c->max_counts, 0, sizeof(int) * 2, c->s);
+    hipMemsetAsync(c->noi, 0, 4 * N, c->s);
+    hipMemsetAsync(c->nog, 0, 4 * N, c->s);
+    for (int k = 0; k < 6; ++k) hipEventCreate(&c->ev[k]);
+    if (hipStreamSynchronize(c->s) != hipSuccess) {
+        mlh_destroy(c);
+        return fail(MLH_ERR_DEVICE, "initialisation failed");
+    }
+    *out = c;
+    return MLH_OK;
+}
+
+int mlh_destroy(mlh_ctx *c) {
+    if (!c) return MLH_OK;
+    if (c->s) hipStreamSynchronize(c->s);
+    for (void *q : c->allocs) hipFree(q);
+    mlp_free(c->mlp);
+    for (int k = 0; k < 6; ++k)
+        if (c->ev[k]) hipEventDestroy(c->ev[k]);
+    if (c->dt_hist) hipFree(c->dt_hist);
+    if (c->s) hipStreamDestroy(c->s);
+    delete c;
+    return MLH_OK;
+}
+
+int mlh_upload_state(mlh_ctx *c, const double *x, const double *y, const double *z,
+                     const double *vx, const double *vy, const double *vz, const double *m,
+                     const double *u) {
+    if (!c || !x || !y || !vx || !vy || !m || !u || (c->D == 3 && (!z || !vz)))
+        return fail(MLH_ERR_INVALID_ARGUMENT, "null state array");
+    size_t b = sizeof(double) * c->N;
+    const double *xs[3] = {x, y, z}, *vs[3] = {vx, vy, vz};
+    for (int a = 0; a < c->D; ++a) {
+        HIPCHK(hipMemcpyAsync(c->x[a], xs[a], b, hipMemcpyHostToDevice, c->s));
+        HIPCHK(hipMemcpyAsync(c->v[a], vs[a], b, hipMemcpyHostToDevice, c->s));
+    }
+    HIPCHK(hipMemcpyAsync(c->m, m, b, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->u, u, b, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->stage = 0;
+    return MLH_OK;
+}
+
+int mlh_download_state(mlh_ctx *c, double *x, double *y, double *z, double *vx, double *vy,
+                       double *vz, double *m, double *u) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    size_t b = sizeof(double) * c->N;
+    double *xs[3] = {x, y, z}, *vs[3] = {vx, vy, vz};
+    for (int a = 0; a < c->D; ++a) {
+        if (xs[a]) HIPCHK(hipMemcpyAsync(xs[a], c->x[a], b, hipMemcpyDeviceToHost, c->s));
+        if (vs[a]) HIPCHK(hipMemcpyAsync(vs[a], c->v[a], b, hipMemcpyDeviceToHost, c->s));
+    }
+    if (m) HIPCHK(hipMemcpyAsync(m, c->m, b, hipMemcpyDeviceToHost, c->s));
+    if (u) HIPCHK(hipMemcpyAsync(u, c->u, b, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return MLH_OK;
+}
+
+}  // extern "C"
+
+// ---- stage launchers (asynchronous) ----------------------------------------
+static void enqueue_neighbours(mlh_ctx *c) {
+    launch_cells(c->g, c->D, c->x[0], c->x[1], c->D == 3 ? c->x[2] : c->x[0], c->N, c->pos,
+                 c->cell_key, c->sorted_idx, c->sorted_key, nullptr, c->sort_scratch, c->cstart,
+                 c->cend, c->s);
+    NbrArgs a{};
+    a.g = c->g;
+    a.N = c->N;
+    a.maxr = c->p.max_interactions;
+    a.maxg = c->p.max_ghost_interactions;
+    a.pos = c->pos;
+    a.cell_key = c->cell_key;
+    a.sorted_idx = c->sorted_idx;
+    a.cstart = c->cstart;
+    a.cend = c->cend;
+    a.nbr = c->nbr;
+    a.noi = c->noi;
+    a.nog = c->nog;
+    a.status = c->status;
+    a.max_counts = c->max_counts;
+    launch_neighbours(a, c->D, c->s);
+}
+
+static int check_status(mlh_ctx *c) {
+    int st[2];
+    HIPCHK(hipMemcpyAsync(st, c->status, sizeof(st), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipGetLastError());
+    if (st[0] & (1 << 1)) return fail(MLH_ERR_NEIGHBOUR_OVERFLOW, "MAX_NUM_INTERACTIONS exceeded");
+    if (st[0] & (1 << 3))
+        return fail(MLH_ERR_GHOST_OVERFLOW, "MAX_NUM_GHOST_INTERACTIONS exceeded");
+    return MLH_OK;
+}
+
+static void swap_state(mlh_ctx *c) {
+    for (int a = 0; a < 3; ++a) {
+        std::swap(c->x[a], c->xn[a]);
+        std::swap(c->v[a], c->vn[a]);
+    }
+    std::swap(c->m, c->mn);
+    std::swap(c->u, c->un);
+}
+
+static int read_counters(mlh_ctx *c, unsigned long long *out) {
+    HIPCHK(hipMemcpyAsync(out, c->counters, sizeof(unsigned long long) * kNumCounters,
+                          hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return MLH_OK;
+}
+
+// negative face pressure aborts as in the reference (Particles.cpp:1824-1827,
+// 1878-1881)
+static int face_pressure_status(mlh_ctx *c, const unsigned long long *before,
+                                const unsigned long long *after) {
+    unsigned long long nr = after[kCntNegReal] - before[kCntNegReal];
+    unsigned long long ng = after[kCntNegGhost] - before[kCntNegGhost];
+    if ((c->p.debug_level >= 1 && ng > 0) || (c->p.debug_level > 1 && nr > 0))
+        return fail(MLH_ERR_NEGATIVE_FACE_PRESSURE, "negative reconstructed face pressure");
+    return MLH_OK;
+}
+
+static int check_learned(mlh_ctx *c) {
+    if (c->p.riemann_seed == MLH_SEED_LEARNED && !c->mlp.params)
+        return fail(MLH_ERR_NOT_TRAINED, "learned seed requested but no network parameters set");
+    return MLH_OK;
+}
+
+static void enqueue_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip,
+                         double *dt_hist, int step, bool timed) {
+    HydroArgs a = hydro_args(c);
+    hipMemsetAsync(c->status, 0, sizeof(int), c->s);
+    if (timed) hipEventRecord(c->ev[0], c->s);
+    enqueue_neighbours(c);
+    if (timed) hipEventRecord(c->ev[1], c->s);
+    launch_density(a, c->D, c->prec, c->s);
+    if (timed) hipEventRecord(c->ev[2], c->s);
+    launch_gradients(a, c->D, c->prec, c->s);
+    launch_dt_finalize(a, dt_fixed, t_now, t_clip, dt_hist, step, c->s);
+    if (timed) hipEventRecord(c->ev[3], c->s);
+    launch_faces(a, c->D, c->prec, c->s);
+    if (timed) hipEventRecord(c->ev[4], c->s);
+    swap_state(c);
+}
+
+static void collect_timings(mlh_ctx *c) {
+    for (int k = 0; k < 4; ++k) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]);
+        c->timings[k] = ms;
+    }
+}
+
+extern "C" {
+
+int mlh_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip, double *dt_used,
+             int32_t *clipped) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    int st = check_learned(c);
+    if (st) return st;
+    unsigned long long before[kNumCounters], after[kNumCounters];
+    if ((st = read_counters(c, before))) return st;
+    enqueue_step(c, dt_fixed, t_now, t_clip, nullptr, 0, true);
+    double dtv[3];
+    HIPCHK(hipMemcpyAsync(dtv, c->dt, sizeof(dtv), hipMemcpyDeviceToHost, c->s));
+    if ((st = check_status(c))) { c->stage = 0; return st; }
+    collect_timings(c);
+    if (dt_used) *dt_used = dtv[0];
+    if (clipped) *clipped = dtv[2] != 0.;
+    c->stage = 0;
+    c->step_count++;
+    if ((st = read_counters(c, after))) return st;
+    return face_pressure_status(c, before, after);
+}
+
+int mlh_run(mlh_ctx *c, int32_t n_steps, double dt_fixed, double *dts) {
+    if (!c || n_steps < 0) return fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
+    int st = check_learned(c);
+    if (st) return st;
+    if (c->dt_hist_cap < n_steps) {
+        if (c->dt_hist) hipFree(c->dt_hist);
+        c->dt_hist = nullptr;
+        HIPCHK(hipMalloc(&c->dt_hist, sizeof(double) * (n_steps > 0 ? n_steps : 1)));
+        c->dt_hist_cap = n_steps;
+    }
+    unsigned long long before[kNumCounters], after[kNumCounters];
+    if ((st = read_counters(c, before))) return st;
+    HIPCHK(hipEventRecord(c->ev[5], c->s));
+    for (int k = 0; k < n_steps; ++k)
+        enqueue_step(c, dt_fixed, -1.0, -1.0, c->dt_hist, k, k == n_steps - 1);
+    if ((st = check_status(c))) return st;
+    if (n_steps > 0) collect_timings(c);
+    if (dts && n_steps > 0)
+        HIPCHK(hipMemcpy(dts, c->dt_hist, sizeof(double) * n_steps, hipMemcpyDeviceToHost));
+    c->stage = 0;
+    c->step_count += n_steps;
+    if ((st = read_counters(c, after))) return st;
+    return face_pressure_status(c, before, after);
+}
+
+int mlh_neighbours(mlh_ctx *c) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    HIPCHK(hipMemsetAsync(c->status, 0, sizeof(int), c->s));
+    enqueue_neighbours(c);
+    int st = check_status(c);
+    c->stage = st ? 0 : 1;
+    return st;
+}
+
+int mlh_density(mlh_ctx *c) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (c->stage < 1) return fail(MLH_ERR_CALL_ORDER, "mlh_neighbours first");
+    HydroArgs a = hydro_args(c);
+    launch_density(a, c->D, c->prec, c->s);
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipGetLastError());
+    c->stage = 2;
+    return MLH_OK;
+}
+
+int mlh_gradients(mlh_ctx *c) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (c->stage < 2) return fail(MLH_ERR_CALL_ORDER, "mlh_density first");
+    HydroArgs a = hydro_args(c);
+    launch_gradients(a, c->D, c->prec, c->s);
+    launch_dt_finalize(a, -1.0, -1.0, -1.0, nullptr, 0, c->s);
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipGetLastError());
+    c->stage = 3;
+    return MLH_OK;
+}
+
+int mlh_timestep(mlh_ctx *c, double *dt) {
+    if (!c || !dt) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->stage < 2) return fail(MLH_ERR_CALL_ORDER, "mlh_density first");
+    int st = mlh_gradients(c);
+    if (st) return st;
+    double dtv[3];
+    HIPCHK(hipMemcpy(dtv, c->dt, sizeof(dtv), hipMemcpyDeviceToHost));
+    *dt = dtv[1];
+    return MLH_OK;
+}
+
+int mlh_fluxes(mlh_ctx *c, double dt) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (c->stage < 3) return fail(MLH_ERR_CALL_ORDER, "mlh_gradients first");
+    int st = check_learned(c);
+    if (st) return st;
+    unsigned long long before[kNumCounters], after[kNumCounters];
+    if ((st = read_counters(c, before))) return st;
+    HIPCHK(hipMemcpyAsync(c->dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->s));
+    HydroArgs a = hydro_args(c);
+    a.write_fluxes = 1;
+    launch_faces(a, c->D, c->prec, c->s);
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipGetLastError());
+    c->stage = 4;
+    if ((st = read_counters(c, after))) return st;
+    return face_pressure_status(c, before, after);
+}
+
+int mlh_update(mlh_ctx *c) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (c->stage < 4) return fail(MLH_ERR_CALL_ORDER, "mlh_fluxes first");
+    swap_state(c);
+    c->stage = 0;
+    return MLH_OK;
+}
+
+int mlh_get_neighbours(mlh_ctx *c, int32_t *noi, int32_t *noi_ghost, int32_t *nbr) {
+    if (!c || !noi || !noi_ghost || !nbr) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->stage < 1) return fail(MLH_ERR_CALL_ORDER, "no neighbour lists yet");
+    size_t N = c->N;
+    HIPCHK(hipMemcpy(noi, c->noi, 4 * N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(noi_ghost, c->nog, 4 * N, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> t((size_t)c->stride * N);
+    HIPCHK(hipMemcpy(t.data(), c->nbr, 4 * t.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < N; ++i) {
+        int cnt = noi[i] + noi_ghost[i];
+        for (int n = 0; n < c->stride; ++n)
+            nbr[i * c->stride + n] = n < cnt ? (int32_t)t[(size_t)n * N + i] : 0;
+    }
+    return MLH_OK;
+}
+
+}  // extern "C"
+
+template <typename T>
+static void to_double(const void *src, double *dst, size_t n) {
+    const T *s = (const T *)src;
+    for (size_t k = 0; k < n; ++k) dst[k] = (double)s[k];
+}
+
+static int read_derived(mlh_ctx *c, void *dev, size_t count, std::vector<double> &out) {
+    std::vector<unsigned char> raw(count * c->tsz);
+    HIPCHK(hipMemcpy(raw.data(), dev, raw.size(), hipMemcpyDeviceToHost));
+    out.resize(count);
+    if (c->tsz == 8) to_double<double>(raw.data(), out.data(), count);
+    else to_double<float>(raw.data(), out.data(), count);
+    return MLH_OK;
+}
+
+extern "C" {
+
+int mlh_get_density(mlh_ctx *c, double *rho, double *P, double *omega) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (c->stage < 2) return fail(MLH_ERR_CALL_ORDER, "no density yet");
+    std::vector<double> pr;
+    int st = read_derived(c, c->prim, (size_t)c->N * kPrimN, pr);
+    if (st) return st;
+    for (int i = 0; i < c->N; ++i) {
+        if (rho) rho[i] = pr[(size_t)i * kPrimN + kPrimRho];
+        if (P) P[i] = pr[(size_t)i * kPrimN + kPrimP];
+        if (omega) omega[i] = pr[(size_t)i * kPrimN + kPrimOmega];
+    }
+    return MLH_OK;
+}
+
+int mlh_get_gradients(mlh_ctx *c, double *grads, double *B) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (c->stage < 3) return fail(MLH_ERR_CALL_ORDER, "no gradients yet");
+    int D = c->D, NF = D + 2, frn = D == 2 ? FRec<2>::N : FRec<3>::N;
+    std::vector<double> fr;
+    int st = read_derived(c, c->frec, (size_t)c->N * frn, fr);
+    if (st) return st;
+    size_t N = c->N;
+    for (size_t i = 0; i < N; ++i) {
+        const double *r = &fr[i * frn];
+        if (grads)
+            for (int f = 0; f < NF; ++f)
+                for (int a = 0; a < D; ++a) grads[((size_t)f * N + i) * D + a] = r[f * D + a];
+        if (B)
+            for (int k = 0; k < D * D; ++k) B[i * D * D + k] = r[NF * D + k];
+    }
+    return MLH_OK;
+}
+
+int mlh_get_fluxes(mlh_ctx *c, double *mF, double *vF, double *eF) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (c->stage < 4) return fail(MLH_ERR_CALL_ORDER, "no fluxes yet");
+    size_t b = sizeof(double) * c->N;
+    if (mF) HIPCHK(hipMemcpy(mF, c->mF, b, hipMemcpyDeviceToHost));
+    if (vF) HIPCHK(hipMemcpy(vF, c->vF, b * c->D, hipMemcpyDeviceToHost));
+    if (eF) HIPCHK(hipMemcpy(eF, c->eF, b, hipMemcpyDeviceToHost));
+    return MLH_OK;
+}
+
+int mlh_get_diagnostics(mlh_ctx *c, mlh_diag *d) {
+    if (!c || !d) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    unsigned long long k[kNumCounters];
+    int st = read_counters(c, k);
+    if (st) return st;
+    int mc[2];
+    HIPCHK(hipMemcpy(mc, c->max_counts, sizeof(mc), hipMemcpyDeviceToHost));
+    d->neg_face_pressure_real = (int64_t)k[kCntNegReal];
+    d->neg_face_pressure_ghost = (int64_t)k[kCntNegGhost];
+    d->vacuum_faces = (int64_t)k[kCntVacuum];
+    d->nonconverged_faces = (int64_t)k[kCntNonconv];
+    d->missing_mirror_pairs = (int64_t)k[kCntMissing];
+    d->learned_fallbacks = (int64_t)k[kCntFallback];
+    d->newton_iterations = (int64_t)k[kCntIters];
+    d->faces_solved = (int64_t)k[kCntFaces];
+    d->max_neighbours = mc[0];
+    d->max_ghost_neighbours = mc[1];
+    return MLH_OK;
+}
+
+int mlh_conserved_sums(mlh_ctx *c, double *out) {
+    if (!c || !out) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    HydroArgs a = hydro_args(c);
+    launch_sums(a, c->D, c->prec, c->sums_block, c->s);
+    int nb = dt_blocks(c->N);
+    std::vector<double> h((size_t)nb * 6);
+    HIPCHK(hipMemcpyAsync(h.data(), c->sums_block, sizeof(double) * h.size(),
+                          hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    for (int k = 0; k < 6; ++k) out[k] = 0.;
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 6; ++k) out[k] += h[(size_t)b * 6 + k];
+    if (c->stage < 2) out[0] = 0.;
+    return MLH_OK;
+}
+
+int mlh_get_timings(mlh_ctx *c, float *ms4) {
+    if (!c || !ms4) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    for (int k = 0; k < 4; ++k) ms4[k] = c->timings[k];
+    return MLH_OK;
+}
+
+}  // extern "C"
+
+// internal accessors for the MLP module
+MlpState &ctx_mlp(mlh_ctx *c) { return c->mlp; }
+hipStream_t ctx_stream(mlh_ctx *c) { return c ? c->s : nullptr; }
+int ctx_fail(int code, const char *msg) { return fail(code, msg); }

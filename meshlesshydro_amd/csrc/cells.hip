@@ -1,0 +1,378 @@
+// cells.hip -- GPU cell list and neighbour lists.
+//
+// Replaces Particles::assignParticlesAndCells (Particles.cpp:270-322),
+// createGhostParticles (:2113-2191), gridNNS (:324-365) and ghostNNS
+// (:2237-2260):
+//   1. cell key per particle with the reference's floor((x-min)/cellSize)
+//      (clamped when it equals the cell count), positions packed as double4;
+//   2. stable LSD radix sort of (cell key, particle index), 8-bit digits:
+//      LDS histogram per 2048-key tile, exclusive scan of the tile
+//      histograms, then a scatter that ranks equal digits inside each
+//      wavefront with eight __ballot masks (match + popcount below the
+//      lane), and across the block's waves through LDS counts;
+//   3. per-cell [start, end) ranges of the sorted order (ascending particle
+//      index inside a cell, as the reference's push_back lists);
+//   4. one thread per particle, in cell order: the 3^D stencil of
+//      non-periodic cells (x outer ... z inner, Domain.cpp:83-118) with the
+//      strict test d^2 < h^2 in the reference's operation order, then the
+//      periodic images: every candidate in the 5^D window of virtual cells
+//      around the particle that wraps in at least one axis, kept when the
+//      image exists under the reference's creation test and d^2 < h^2, and
+//      inserted in ghost-index order.  A periodic image within h of x_i is
+//      less than one cell size away per axis, so the +-2 window is
+//      complete: the result equals the reference's exhaustive ghost loop.
+// Neighbour rows are stored transposed (entry n of particle i at
+// nbr[n*N + i]) so that per-particle kernels read them coalesced.
+#include <hip/hip_runtime.h>
+#include "device_common.h"
+#include "kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace mlh {
+
+__global__ void k_fill_i32(int *p, int v, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+template <int D>
+__global__ void k_assign_cells(Geom g, const double *__restrict__ x0,
+                               const double *__restrict__ x1,
+                               const double *__restrict__ x2, int N,
+                               double4 *__restrict__ pos, uint32_t *__restrict__ key,
+                               uint32_t *__restrict__ idx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double xs[3] = {x0[i], x1[i], D == 3 ? x2[i] : 0.0};
+    int c = 0, mul = 1;
+    for (int a = 0; a < D; ++a) {
+        int f = (int)floor((xs[a] - g.bmin[a]) / g.cs[a]);
+        if (f == g.nc[a]) f -= 1;
+        if (f < 0) f = 0;
+        if (f > g.nc[a] - 1) f = g.nc[a] - 1;
+        c += f * mul;
+        mul *= g.nc[a];
+    }
+    pos[i] = make_double4(xs[0], xs[1], xs[2], 0.0);
+    key[i] = (uint32_t)c;
+    idx[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------- radix sort
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kRsThreads * kRsItems;
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(
+    const uint32_t *__restrict__ keys, int N, int shift, uint32_t *__restrict__ hist,
+    int ntiles) {
+    __shared__ uint32_t h[256];
+    int tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    int base = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        int e = base + r * kRsThreads + tid;
+        if (e < N) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[tid * ntiles + blockIdx.x] = h[tid];
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, int N, int shift,
+    const uint32_t *__restrict__ offs, int ntiles) {
+    constexpr int W = kRsThreads / 64;
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[W][256];
+    int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    run[tid] = offs[tid * ntiles + blockIdx.x];
+    for (int w = 0; w < W; ++w) wcnt[w][tid] = 0;
+    __syncthreads();
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int base = blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsItems; ++r) {
+        int e = base + r * kRsThreads + tid;
+        bool valid = e < N;
+        uint32_t k = valid ? kin[e] : 0u;
+        uint32_t v = valid ? vin[e] : 0u;
+        uint32_t d = (k >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(valid && bit);
+            m &= bit ? bb : ~bb;
+        }
+        int rank = __popcll(m & below);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + (uint32_t)rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < W; ++w) {
+            add += wcnt[w][tid];
+            wcnt[w][tid] = 0;
+        }
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+// exclusive scan of n uint32 in blocks of 1024, then block offsets
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanBlock = kScanThreads * kScanItems;
+
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t *total) {
+    int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        uint32_t t = tid >= off ? sh[tid - off] : 0u;
+        __syncthreads();
+        sh[tid] += t;
+        __syncthreads();
+    }
+    uint32_t incl = sh[tid];
+    *total = sh[kScanThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_local(
+    const uint32_t *__restrict__ in, uint32_t *__restrict__ out, int n,
+    uint32_t *__restrict__ sums) {
+    __shared__ uint32_t sh[kScanThreads];
+    int base = blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+    uint32_t loc[kScanItems], s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        loc[k] = base + k < n ? in[base + k] : 0u;
+        s += loc[k];
+    }
+    uint32_t tot;
+    uint32_t pre = block_excl_scan(s, sh, &tot);
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = pre;
+        pre += loc[k];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// single block: in-place exclusive scan of the block sums (any length)
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(uint32_t *__restrict__ sums, int n) {
+    __shared__ uint32_t sh[kScanThreads];
+    uint32_t carry = 0;
+    for (int base = 0; base < n; base += kScanThreads) {
+        int t = base + threadIdx.x;
+        uint32_t v = t < n ? sums[t] : 0u;
+        uint32_t tot;
+        uint32_t pre = block_excl_scan(v, sh, &tot);
+        if (t < n) sums[t] = pre + carry;
+        carry += tot;
+    }
+}
+
+__global__ void k_scan_add(uint32_t *__restrict__ out, int n, const uint32_t *__restrict__ sums) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] += sums[t / kScanBlock];
+}
+
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *tmp,
+                        hipStream_t s) {
+    int nb = (n + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanThreads), 0, s, in, out, n, tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, tmp, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3((n + 255) / 256), dim3(256), 0, s, out, n, tmp);
+}
+
+size_t radix_sort_scratch_words(int N) {
+    int ntiles = (N + kRsTile - 1) / kRsTile;
+    size_t h = (size_t)256 * ntiles;
+    return 2 * h + (h + kScanBlock - 1) / kScanBlock + 16;
+}
+
+// Stable sort of (key, val) by the low `bits` bits of key.  Result in
+// (k0, v0); (k1, v1) is ping-pong space.
+void radix_sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int N,
+                      int bits, uint32_t *scratch, hipStream_t s) {
+    int ntiles = (N + kRsTile - 1) / kRsTile;
+    int h = 256 * ntiles;
+    uint32_t *hist = scratch, *offs = scratch + h, *tmp = scratch + 2 * h;
+    int passes = (bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kRsThreads), 0, s, k0, N, shift, hist, ntiles);
+        scan_exclusive_u32(hist, offs, h, tmp, s);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kRsThreads), 0, s, k0, v0, k1, v1, N,
+                           shift, offs, ntiles);
+        uint32_t *t = k0; k0 = k1; k1 = t;
+        t = v0; v0 = v1; v1 = t;
+    }
+    if (passes & 1)  // result landed in the ping-pong buffers
+    {
+        hipMemcpyAsync(k1, k0, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, s);
+        hipMemcpyAsync(v1, v0, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, s);
+    }
+}
+
+__global__ void k_cell_ranges(const uint32_t *__restrict__ skeys, int N,
+                              int *__restrict__ cstart, int *__restrict__ cend) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    uint32_t k = skeys[p];
+    if (p == 0 || skeys[p - 1] != k) cstart[k] = p;
+    if (p == N - 1 || skeys[p + 1] != k) cend[k] = p + 1;
+}
+
+// --------------------------------------------------------------- neighbours
+template <int D>
+__device__ inline double dist2(const double *a, const double *b) {
+    double d = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]);
+    if (D == 3) d += (a[2] - b[2]) * (a[2] - b[2]);
+    return d;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.N) return;
+    const Geom &g = a.g;
+    const int N = a.N;
+    int i = (int)a.sorted_idx[t];
+    double4 pi = a.pos[i];
+    double xi[3] = {pi.x, pi.y, pi.z};
+    uint32_t ck = a.cell_key[i];
+    int cc[3];
+    cc[0] = (int)(ck % (uint32_t)g.nc[0]);
+    cc[1] = (int)((ck / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
+    cc[2] = D == 3 ? (int)(ck / (uint32_t)(g.nc[0] * g.nc[1])) : 0;
+    int cnt = 0, err = 0;
+    // real neighbours, reference stencil order
+    for (int k = cc[0] - 1; k <= cc[0] + 1 && !err; ++k) {
+        if (k < 0 || k >= g.nc[0]) continue;
+        for (int l = cc[1] - 1; l <= cc[1] + 1 && !err; ++l) {
+            if (l < 0 || l >= g.nc[1]) continue;
+            for (int m = (D == 3 ? cc[2] - 1 : 0); m <= (D == 3 ? cc[2] + 1 : 0) && !err; ++m) {
+                if (D == 3 && (m < 0 || m >= g.nc[2])) continue;
+                int c = k + l * g.nc[0] + (D == 3 ? m * g.nc[0] * g.nc[1] : 0);
+                int q1 = a.cend[c];
+                for (int q = a.cstart[c]; q < q1; ++q) {
+                    int j = (int)a.sorted_idx[q];
+                    if (j == i) continue;
+                    double4 pj = a.pos[j];
+                    double xj[3] = {pj.x, pj.y, pj.z};
+                    double d = dist2<D>(xj, xi);
+                    if (d < g.hsq) {
+                        if (cnt >= a.maxr) { err = 1; break; }
+                        a.nbr[(size_t)cnt * N + i] = (uint32_t)j | (kCodeReal << kIdxBits);
+                        ++cnt;
+                    }
+                }
+            }
+        }
+    }
+    a.noi[i] = cnt;
+    if (err) {  // rows stay valid: the entries written so far, no images
+        a.nog[i] = 0;
+        atomicOr(a.status, 1 << 1);
+        return;
+    }
+    int gcnt = 0;
+    if (g.periodic) {
+        const int W = 2;
+        for (int vz = (D == 3 ? cc[2] - W : 0); vz <= (D == 3 ? cc[2] + W : 0) && !err; ++vz)
+            for (int vy = cc[1] - W; vy <= cc[1] + W && !err; ++vy)
+                for (int vx = cc[0] - W; vx <= cc[0] + W && !err; ++vx) {
+                    int v[3] = {vx, vy, vz};
+                    int s[3] = {0, 0, 0}, r[3] = {0, 0, 0};
+                    bool wrapped = false;
+                    for (int ax = 0; ax < D; ++ax) {
+                        s[ax] = v[ax] < 0 ? -1 : (v[ax] >= g.nc[ax] ? 1 : 0);
+                        r[ax] = v[ax] - s[ax] * g.nc[ax];
+                        wrapped |= s[ax] != 0;
+                    }
+                    if (!wrapped) continue;
+                    int c = r[0] + r[1] * g.nc[0] + (D == 3 ? r[2] * g.nc[0] * g.nc[1] : 0);
+                    uint32_t code = (uint32_t)((s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1));
+                    int q1 = a.cend[c];
+                    for (int q = a.cstart[c]; q < q1; ++q) {
+                        int j = (int)a.sorted_idx[q];
+                        double4 pj = a.pos[j];
+                        double xj[3] = {pj.x, pj.y, pj.z};
+                        bool exists = true;
+                        for (int ax = 0; ax < D; ++ax) {
+                            if (s[ax] == 0) continue;
+                            bool low = xj[ax] <= g.bmin[ax] + g.h;
+                            bool high = g.bmax[ax] - g.h < xj[ax];
+                            exists &= s[ax] > 0 ? low : (!low && high);
+                            xj[ax] = image_coord(xj[ax], s[ax], g.bmin[ax], g.bmax[ax]);
+                        }
+                        if (!exists) continue;
+                        double d = dist2<D>(xj, xi);
+                        if (!(d < g.hsq)) continue;
+                        if (gcnt >= a.maxg) { err = 1; break; }
+                        uint32_t e = (uint32_t)j | (code << kIdxBits);
+                        uint64_t ke = ghost_key(e);
+                        int p = gcnt;
+                        while (p > 0) {
+                            uint32_t prev = a.nbr[(size_t)(cnt + p - 1) * N + i];
+                            if (ghost_key(prev) <= ke) break;
+                            a.nbr[(size_t)(cnt + p) * N + i] = prev;
+                            --p;
+                        }
+                        a.nbr[(size_t)(cnt + p) * N + i] = e;
+                        ++gcnt;
+                    }
+                }
+        if (err) atomicOr(a.status, 1 << 3);
+    }
+    a.nog[i] = gcnt;
+    atomicMax(&a.max_counts[0], cnt);
+    atomicMax(&a.max_counts[1], gcnt);
+}
+
+template __global__ void k_assign_cells<2>(Geom, const double *, const double *, const double *,
+                                           int, double4 *, uint32_t *, uint32_t *);
+template __global__ void k_assign_cells<3>(Geom, const double *, const double *, const double *,
+                                           int, double4 *, uint32_t *, uint32_t *);
+template __global__ void k_neighbours<2>(NbrArgs);
+template __global__ void k_neighbours<3>(NbrArgs);
+
+void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
+                  const double *x2, int N, double4 *pos, uint32_t *key, uint32_t *idx,
+                  uint32_t *key2, uint32_t *idx2, uint32_t *scratch, int *cstart,
+                  int *cend, hipStream_t s) {
+    int nb = (N + 255) / 256;
+    if (D == 2)
+        hipLaunchKernelGGL(k_assign_cells<2>, dim3(nb), dim3(256), 0, s, g, x0, x1, x2, N, pos, key, idx);
+    else
+        hipLaunchKernelGGL(k_assign_cells<3>, dim3(nb), dim3(256), 0, s, g, x0, x1, x2, N, pos, key, idx);
+    // sort a copy: key/idx stay per-particle (key[i] = cell of particle i)
+    hipMemcpyAsync(key2, key, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, s);
+    int bits = 1;
+    while ((1 << bits) < g.ncells) ++bits;
+    uint32_t *k1 = scratch, *v1 = scratch + N, *rs = scratch + 2 * (size_t)N;
+    radix_sort_pairs(key2, idx, k1, v1, N, bits, rs, s);
+    int nc = g.ncells;
+    hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, s, cstart, 0, nc);
+    hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, s, cend, 0, nc);
+    hipLaunchKernelGGL(k_cell_ranges, dim3(nb), dim3(256), 0, s, key2, N, cstart, cend);
+    (void)idx2;
+}
+
+void launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {
+    int nb = (a.N + 255) / 256;
+    if (D == 2)
+        hipLaunchKernelGGL(k_neighbours<2>, dim3(nb), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_neighbours<3>, dim3(nb), dim3(256), 0, s, a);
+}
+
+}  // namespace mlh

@@ -1,0 +1,101 @@
+// device_common.h -- layouts and small device helpers shared by the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+namespace mlh {
+
+// Packed neighbour entry: particle index in bits 0..26, periodic image code
+// s = (sx+1) + 3(sy+1) + 9(sz+1) in bits 27..31 (13 = no image).
+constexpr int kIdxBits = 27;
+constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
+constexpr uint32_t kCodeReal = 13u;
+
+__host__ __device__ inline uint32_t nb_index(uint32_t e) { return e & kIdxMask; }
+__host__ __device__ inline uint32_t nb_code(uint32_t e) { return e >> kIdxBits; }
+__host__ __device__ inline int code_shift(uint32_t code, int a) {
+    int c = (int)code;
+    if (a == 1) c /= 3;
+    if (a == 2) c /= 9;
+    return c % 3 - 1;
+}
+// bit a set when axis a carries an image shift
+__host__ __device__ inline int code_mask(uint32_t code) {
+    int m = 0;
+    for (int a = 0; a < 3; ++a)
+        if (code_shift(code, a) != 0) m |= 1 << a;
+    return m;
+}
+// ghost-index order of the reference: parent ascending, then the image
+// subsets of one parent in descending axis-mask order
+__host__ __device__ inline uint64_t ghost_key(uint32_t e) {
+    return (uint64_t)nb_index(e) * 8u + (uint64_t)(7 - code_mask(nb_code(e)));
+}
+
+// Search geometry (Domain::createGrid, Domain.cpp:9-54).
+struct Geom {
+    double bmin[3], bmax[3], cs[3];
+    int nc[3];
+    int ncells;
+    double h, hsq;        // kernelSize and kernelSize*kernelSize
+    double sigma;         // cubic-spline normalisation for h/2
+    int periodic;
+};
+
+// Periodic image coordinate (createGhostParticles, Particles.cpp:2124-2138):
+// s = +1 -> max + (x - min), s = -1 -> min - (max - x).
+__device__ inline double image_coord(double x, int s, double mn, double mx) {
+    return s == 0 ? x : (s > 0 ? mx + (x - mn) : mn - (mx - x));
+}
+
+// glibc pow(t, 3.) is correctly rounded; a double-double cube rounded once
+// gives the same bits (error of the unrounded value ~2^-106 relative).
+__device__ inline double cube_cr(double t) {
+    double hi = t * t;
+    double lo = __fma_rn(t, t, -hi);
+    double p = hi * t;
+    double e = __fma_rn(hi, t, -p) + lo * t;
+    return p + e;
+}
+__device__ inline float cube_cr(float t) { return t * t * t; }
+
+// Kernel::cubicSpline (Particles.cpp:7-24); sigma precomputed on the host
+// with the reference's expression.
+template <typename T>
+__device__ inline T kernel_w(T r, T h2, T sigma) {
+    T q = r / h2;
+    if (T(0) <= q && q <= T(1)) return sigma * (T(1) - T(3) / T(2) * q * q * (T(1) - q / T(2)));
+    if (T(1) < q && q < T(2)) return sigma / T(4) * cube_cr(T(2) - q);
+    return T(0);
+}
+
+// Per-particle primitive record (AoS, 8 values of T).
+constexpr int kPrimRho = 0, kPrimV = 1, kPrimP = 4, kPrimC = 5, kPrimOmega = 6, kPrimN = 8;
+template <int D>
+__host__ __device__ constexpr int field_slot(int f) { return f == D + 1 ? kPrimP : f; }
+
+// Face-input record: (D+2)*D limited gradients then D*D B = E^-1.
+template <int D>
+struct FRec {
+    static constexpr int NF = D + 2;
+    static constexpr int G = 0;
+    static constexpr int B = NF * D;
+    static constexpr int N = NF * D + D * D;   // 12 (2D) / 24 (3D)
+};
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+// abs() of the reference's slope / pairwise limiter (see mlh_limiter_abs).
+template <typename T>
+__device__ inline T lim_abs(T x, int mode) {
+    if (mode == 1) return x < T(0) ? -x : x;
+    double xd = (double)x;
+    int32_t t;
+    if (xd != xd || xd >= 2147483648.0 || xd <= -2147483649.0) t = INT32_MIN;
+    else t = (int32_t)xd;
+    if (t != INT32_MIN && t < 0) t = -t;
+    return (T)t;
+}
+
+}  // namespace mlh

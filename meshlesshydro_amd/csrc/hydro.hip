@@ -1,0 +1,725 @@
+// hydro.hip -- MFV physics kernels: density, gradients + limiter + CFL step,
+// faces + exact Riemann + flux sums + state update.
+//
+// One thread per particle; neighbour rows come from cells.hip in the
+// reference's order (real neighbours in stencil order, then periodic images
+// in ghost-index order), so every per-particle sum below runs in the
+// reference's summation order.
+//
+// Faces are evaluated i-centric and without atomics: particle i visits each
+// of its neighbours and solves the pair from the side the reference solves
+// it (the lower index; for an image pair the reference's test parent >= i),
+// i.e. when i is the higher index it recomputes the lower index's face with
+// the same operands and subtracts it.  Both sides therefore hold bitwise
+// opposite fluxes (F_ij = -F_ji) and the update is conservative to
+// round-off and independent of scheduling.
+#include <hip/hip_runtime.h>
+#include "device_common.h"
+#include "kernels.h"
+#include "riemann.h"
+#include "mlp_device.h"
+
+#pragma clang fp contract(off)
+
+namespace mlh {
+
+template <typename T> __device__ inline T tmin_pos();
+template <> __device__ inline double tmin_pos<double>() { return DBL_MIN; }
+template <> __device__ inline float tmin_pos<float>() { return FLT_MIN; }
+template <typename T> __device__ inline T tmax_val();
+template <> __device__ inline double tmax_val<double>() { return DBL_MAX; }
+template <> __device__ inline float tmax_val<float>() { return FLT_MAX; }
+
+__device__ inline double tsqrt(double x) { return sqrt(x); }
+__device__ inline float tsqrt(float x) { return __builtin_sqrtf(x); }
+
+template <int D>
+__device__ inline double dist2(const double *a, const double *b) {
+    double d = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]);
+    if (D == 3) d += (a[2] - b[2]) * (a[2] - b[2]);
+    return d;
+}
+
+template <int D, typename T>
+__device__ inline T dotT(const T *a, const T *b) {
+    T r = T(0);
+    for (int k = 0; k < D; ++k) r += a[k] * b[k];
+    return r;
+}
+
+// neighbour position (periodic image applied) and parent index
+template <int D>
+__device__ inline int nb_pos(const HydroArgs &a, uint32_t e, double *xj, int *s) {
+    int j = (int)nb_index(e);
+    uint32_t code = nb_code(e);
+    double4 p = a.pos[j];
+    xj[0] = p.x; xj[1] = p.y; xj[2] = p.z;
+    s[0] = s[1] = s[2] = 0;
+    if (code != kCodeReal) {
+        for (int ax = 0; ax < D; ++ax) {
+            s[ax] = code_shift(code, ax);
+            xj[ax] = image_coord(xj[ax], s[ax], a.g.bmin[ax], a.g.bmax[ax]);
+        }
+    }
+    return j;
+}
+
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ------------------------------------------------------------------ density
+// compDensity / compOmega (Particles.cpp:1151-1184), ghost part
+// (:2262-2290), compPressure (:1272-1288); also c = sqrt(gamma P / rho)
+// for the time step (:1453).
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_density(HydroArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    const int N = a.N;
+    double4 pi = a.pos[i];
+    double xi[3] = {pi.x, pi.y, pi.z};
+    const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
+    int nr = a.noi[i], ng = a.nog[i];
+    T omg = T(0);
+    for (int n = 0; n < nr; ++n) {
+        double xj[3];
+        int s[3];
+        nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
+        T r = tsqrt(T(dist2<D>(xi, xj)));
+        omg += kernel_w<T>(r, h2, sig);
+    }
+    T om = omg + kernel_w<T>(T(0), h2, sig);
+    if (a.g.periodic) {
+        T og = om;
+        for (int n = nr; n < nr + ng; ++n) {
+            double xj[3];
+            int s[3];
+            nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
+            T r = tsqrt(T(dist2<D>(xi, xj)));
+            og += kernel_w<T>(r, h2, sig);
+        }
+        om = og;
+    }
+    double rho = a.m[i] * (double)om;
+    double P = (a.gamma - 1.) * rho * a.u[i];
+    double c = sqrt(a.gamma * P / rho);
+    T *pr = (T *)a.prim + (size_t)i * kPrimN;
+    T rec[kPrimN];
+    rec[kPrimRho] = T(rho);
+    rec[kPrimV + 0] = T(a.v[0][i]);
+    rec[kPrimV + 1] = T(a.v[1][i]);
+    rec[kPrimV + 2] = D == 3 ? T(a.v[2][i]) : T(0);
+    rec[kPrimP] = T(P);
+    rec[kPrimC] = T(c);
+    rec[kPrimOmega] = om;
+    rec[7] = T(0);
+    for (int k = 0; k < kPrimN; ++k) pr[k] = rec[k];
+}
+
+// Helper::inverseMatrix (Helper.cpp:7-18, LAPACK dgetrf_/dgetri_): Gauss-
+// Jordan with partial pivoting, FP64.
+template <int D>
+__device__ inline void invert(double *A) {
+    double M[D][2 * D];
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < 2 * D; ++c) M[r][c] = c < D ? A[r * D + c] : (c - D == r ? 1. : 0.);
+    for (int k = 0; k < D; ++k) {
+        int piv = k;
+        for (int r = k + 1; r < D; ++r)
+            if (fabs(M[r][k]) > fabs(M[piv][k])) piv = r;
+        if (piv != k)
+            for (int c = 0; c < 2 * D; ++c) {
+                double t = M[k][c]; M[k][c] = M[piv][c]; M[piv][c] = t;
+            }
+        double d = M[k][k];
+        for (int c = 0; c < 2 * D; ++c) M[k][c] /= d;
+        for (int r = 0; r < D; ++r) {
+            if (r == k) continue;
+            double f = M[r][k];
+            for (int c = 0; c < 2 * D; ++c) M[r][c] -= f * M[k][c];
+        }
+    }
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) A[r * D + c] = M[r][D + c];
+}
+
+// ---------------------------------------------------------------- gradients
+// compPsijTilde (Particles.cpp:1186-1254, ghosts :2292-2455): E_i in FP64,
+// B_i = E_i^-1 in FP64; gradient (:1257-1270, :2457-2502) for rho, v, P;
+// slopeLimiter (:1313-1444); compGlobalTimestep (:1446-1485) per particle.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
+    constexpr int NF = D + 2;
+    constexpr int FRN = FRec<D>::N;
+    __shared__ double sdt[256];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double dt_i = DBL_MAX;
+    if (i < a.N) {
+        const int N = a.N;
+        double4 pi = a.pos[i];
+        double xi[3] = {pi.x, pi.y, pi.z};
+        const T *prim = (const T *)a.prim;
+        const T *pri = prim + (size_t)i * kPrimN;
+        T om_i = pri[kPrimOmega];
+        T fi[NF];
+        for (int f = 0; f < NF; ++f) fi[f] = pri[field_slot<D>(f)];
+        const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
+        int cnt = a.noi[i] + a.nog[i], nr = a.noi[i];
+        // pass 1: E_i and the signal speed (real neighbours only)
+        double E[D * D];
+        for (int k = 0; k < D * D; ++k) E[k] = 0.;
+        double vsig = DBL_MIN;
+        double ci = (double)pri[kPrimC];
+        double vi[3] = {a.v[0][i], a.v[1][i], D == 3 ? a.v[2][i] : 0.};
+        for (int n = 0; n < cnt; ++n) {
+            double xj[3];
+            int s[3];
+            int j = nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
+            T r = tsqrt(T(dist2<D>(xi, xj)));
+            double w = (double)(kernel_w<T>(r, h2, sig) / om_i);
+            for (int p = 0; p < D; ++p)
+                for (int q = 0; q < D; ++q) E[D * p + q] += (xj[p] - xi[p]) * (xj[q] - xi[q]) * w;
+            if (n < nr) {
+                const T *prj = prim + (size_t)j * kPrimN;
+                double cj = (double)prj[kPrimC];
+                double xij[3], vij[3];
+                for (int ax = 0; ax < D; ++ax) {
+                    xij[ax] = xi[ax] - xj[ax];
+                    vij[ax] = vi[ax] - (double)prj[kPrimV + ax];
+                }
+                double vx = dotT<D, double>(vij, xij) / sqrt(dotT<D, double>(xij, xij));
+                vx = vx < 0. ? vx : 0.;
+                double vs = ci + cj - vx;
+                vsig = vs > vsig ? vs : vsig;
+            }
+        }
+        invert<D>(E);
+        T B[D * D];
+        for (int k = 0; k < D * D; ++k) B[k] = T(E[k]);
+        // pass 2: gradients and neighbour extrema
+        T grad[NF][D];
+        T ngbmax[NF], ngbmin[NF];
+        for (int f = 0; f < NF; ++f) {
+            for (int ax = 0; ax < D; ++ax) grad[f][ax] = T(0);
+            ngbmax[f] = tmin_pos<T>();
+            ngbmin[f] = tmax_val<T>();
+        }
+        for (int n = 0; n < cnt; ++n) {
+            double xj[3];
+            int s[3];
+            int j = nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
+            T r = tsqrt(T(dist2<D>(xi, xj)));
+            T w = kernel_w<T>(r, h2, sig) / om_i;
+            T psi[D];
+            for (int p = 0; p < D; ++p) {
+                psi[p] = T(0);
+                for (int q = 0; q < D; ++q) psi[p] += B[p * D + q] * T(xj[q] - xi[q]) * w;
+            }
+            const T *prj = prim + (size_t)j * kPrimN;
+            for (int f = 0; f < NF; ++f) {
+                T fj = prj[field_slot<D>(f)];
+                for (int p = 0; p < D; ++p) grad[f][p] += (fj - fi[f]) * psi[p];
+                if (ngbmax[f] < fj) ngbmax[f] = fj;
+                if (ngbmin[f] > fj) ngbmin[f] = fj;
+            }
+        }
+        // pass 3: slope limiter
+        if (a.slope_limiting) {
+            T midmax[NF], midmin[NF];
+            for (int f = 0; f < NF; ++f) {
+                midmax[f] = tmin_pos<T>();
+                midmin[f] = tmax_val<T>();
+            }
+            for (int n = 0; n < cnt; ++n) {
+                double xj[3];
+                int s[3];
+                nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
+                T xijxi[D];
+                for (int ax = 0; ax < D; ++ax) {
+                    double xij = a.foqp ? (xi[ax] + xj[ax]) / 2.
+                                        : xi[ax] + a.g.h / 4. * (xj[ax] - xi[ax]);
+                    xijxi[ax] = T(xij - xi[ax]);
+                }
+                for (int f = 0; f < NF; ++f) {
+                    T fij = fi[f] + dotT<D, T>(grad[f], xijxi);
+                    if (midmax[f] < fij) midmax[f] = fij;
+                    if (midmin[f] > fij) midmin[f] = fij;
+                }
+            }
+            const T beta = T(a.beta);
+            for (int f = 0; f < NF; ++f) {
+                T amax = lim_abs<T>((ngbmax[f] - fi[f]) / (midmax[f] - fi[f]), a.limiter_abs);
+                T amin = lim_abs<T>((fi[f] - ngbmin[f]) / (fi[f] - midmin[f]), a.limiter_abs);
+                if (amin <= amax && beta * amin < T(1)) {
+                    for (int ax = 0; ax < D; ++ax) grad[f][ax] *= amin;
+                } else if (amax <= amin && beta * amax < T(1)) {
+                    for (int ax = 0; ax < D; ++ax) grad[f][ax] *= amax;
+                }
+            }
+        }
+        T *fr = (T *)a.frec + (size_t)i * FRN;
+        for (int f = 0; f < NF; ++f)
+            for (int ax = 0; ax < D; ++ax) fr[FRec<D>::G + f * D + ax] = grad[f][ax];
+        for (int k = 0; k < D * D; ++k) fr[FRec<D>::B + k] = B[k];
+        dt_i = a.cfl * a.g.h / vsig;
+    }
+    // block minimum of the CFL step (min is order independent)
+    sdt[threadIdx.x] = dt_i;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            double o = sdt[threadIdx.x + off];
+            if (o < sdt[threadIdx.x]) sdt[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.dt_block[blockIdx.x] = sdt[0];
+}
+
+// dt = min over blocks; fixed dt or clip to the next dump time
+// (MeshlessScheme.cpp:93-101: if t + dt >= t_dump then dt = t_dump - t).
+__global__ __launch_bounds__(256) void k_dt_finalize(const double *__restrict__ dt_block,
+                                                     int nblocks, double dt_fixed,
+                                                     double t_now, double t_clip,
+                                                     double *__restrict__ dt,
+                                                     double *__restrict__ dt_hist, int step) {
+    __shared__ double sh[256];
+    double v = DBL_MAX;
+    for (int k = threadIdx.x; k < nblocks; k += 256) {
+        double o = dt_block[k];
+        v = o < v ? o : v;
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off && sh[threadIdx.x + off] < sh[threadIdx.x])
+            sh[threadIdx.x] = sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double d = sh[0];
+        double clipped = 0.;
+        dt[1] = d;
+        if (dt_fixed > 0.) {
+            d = dt_fixed;
+        } else if (t_clip > 0. && t_now + d >= t_clip) {
+            d = t_clip - t_now;
+            clipped = 1.;
+        }
+        dt[0] = d;
+        dt[2] = clipped;
+        if (dt_hist) dt_hist[step] = d;
+    }
+}
+
+// -------------------------------------------------------------------- faces
+template <int D, typename T>
+struct PData {
+    static constexpr int NF = D + 2;
+    static constexpr int NT = 6 + NF * D + D * D;
+    double x[3];
+    T t[NT];  // rho, P, omega, v[3], grad[NF][D], B[D][D]
+};
+template <int D, typename T>
+__device__ inline void load_pdata(const HydroArgs &a, int j, PData<D, T> &d) {
+    double4 p = a.pos[j];
+    d.x[0] = p.x; d.x[1] = p.y; d.x[2] = p.z;
+    const T *pr = (const T *)a.prim + (size_t)j * kPrimN;
+    d.t[0] = pr[kPrimRho];
+    d.t[1] = pr[kPrimP];
+    d.t[2] = pr[kPrimOmega];
+    d.t[3] = pr[kPrimV + 0];
+    d.t[4] = pr[kPrimV + 1];
+    d.t[5] = pr[kPrimV + 2];
+    const T *fr = (const T *)a.frec + (size_t)j * FRec<D>::N;
+    for (int k = 0; k < FRec<D>::N; ++k) d.t[6 + k] = fr[k];
+}
+#define PD_RHO(d) (d).t[0]
+#define PD_P(d) (d).t[1]
+#define PD_OM(d) (d).t[2]
+#define PD_V(d, ax) (d).t[3 + (ax)]
+#define PD_G(d, f, ax) (d).t[6 + (f) * D + (ax)]
+#define PD_B(d, p, q) (d).t[6 + (D + 2) * D + (p) * D + (q)]
+
+struct FaceStats {
+    int neg_real, neg_ghost, vacuum;
+    SolveStats st;
+};
+
+// pairwiseLimiter, Particles.cpp:1735-1785
+template <typename T>
+__device__ inline T pairwise_limiter(const HydroArgs &a, T phi0, T phi_i, T phi_j,
+                                     T xijxi_abs, T xjxi_abs) {
+    T phi_ = phi_i;
+    T phi_ij = phi_i + xijxi_abs / xjxi_abs * (phi_j - phi_i);
+    T phiMin, phiMax;
+    if (phi_i < phi_j) { phiMin = phi_i; phiMax = phi_j; }
+    else { phiMin = phi_j; phiMax = phi_i; }
+    T delta1 = T(a.psi1) * lim_abs<T>(phi_i - phi_j, a.limiter_abs);
+    T delta2 = T(a.psi2) * lim_abs<T>(phi_i - phi_j, a.limiter_abs);
+    T phiMinus, phiPlus;
+    if ((phiMax + delta1 >= T(0) && phiMax >= T(0)) || (phiMax + delta1 < T(0) && phiMax < T(0)))
+        phiPlus = phiMax + delta1;
+    else
+        phiPlus = phiMax / (T(1) + delta1 / lim_abs<T>(phiMax, a.limiter_abs));
+    if ((phiMin - delta1 >= T(0) && phiMin >= T(0)) || (phiMin - delta1 < T(0) && phiMin < T(0)))
+        phiMinus = phiMin - delta1;
+    else
+        phiMinus = phiMin / (T(1) + delta1 / lim_abs<T>(phiMin, a.limiter_abs));
+    if (phi_i < phi_j) {
+        T minPhiD2 = phi_ij + delta2 < phi0 ? phi_ij + delta2 : phi0;
+        phi_ = phiMinus > minPhiD2 ? phiMinus : minPhiD2;
+    } else if (phi_i > phi_j) {
+        T maxPhiD2 = phi_ij - delta2 > phi0 ? phi_ij - delta2 : phi0;
+        phi_ = phiPlus < maxPhiD2 ? phiPlus : maxPhiD2;
+    }
+    return phi_;
+}
+
+// The face between L (the side that solves it) and H seen from L with
+// image shift s: effective face (compEffectiveFace, :1290-1311 / :2504-2531),
+// reconstructed + predicted states (compRiemannStatesLR, :1488-1733 /
+// :2533-2683) and the Riemann flux (Riemann.cpp:7-229).  F = {mass,
+// energy, momentum...} as seen from L.
+template <int D, typename T>
+__device__ inline void face_flux(const HydroArgs &a, const PData<D, T> &L,
+                                 const PData<D, T> &H, const int *s, bool image,
+                                 T dt, T *F, FaceStats &fs) {
+    constexpr int NF = D + 2;
+    const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
+    double g[3], gp[3];
+    for (int ax = 0; ax < D; ++ax) {
+        g[ax] = image_coord(H.x[ax], s[ax], a.g.bmin[ax], a.g.bmax[ax]);
+        gp[ax] = image_coord(L.x[ax], -s[ax], a.g.bmin[ax], a.g.bmax[ax]);
+    }
+    // effective face A = psi_H(x_L)/omega_L - psi_L(x_H)/omega_H
+    T r1 = tsqrt(T(dist2<D>(L.x, g)));
+    T r2 = tsqrt(T(dist2<D>(H.x, gp)));
+    T w1 = kernel_w<T>(r1, h2, sig) / PD_OM(L);
+    T w2 = kernel_w<T>(r2, h2, sig) / PD_OM(H);
+    T A[D];
+    for (int p = 0; p < D; ++p) {
+        T p1 = T(0), p2 = T(0);
+        for (int q = 0; q < D; ++q) {
+            p1 += PD_B(L, p, q) * T(g[q] - L.x[q]) * w1;
+            p2 += PD_B(H, p, q) * T(gp[q] - H.x[q]) * w2;
+        }
+        A[p] = T(1) / PD_OM(L) * p1 - T(1) / PD_OM(H) * p2;
+    }
+    // states at the face, L perspective: WR = L, WL = H
+    T xijxi[D], xijxj[D], xjxi[D];
+    for (int ax = 0; ax < D; ++ax) {
+        xjxi[ax] = T(g[ax] - L.x[ax]);
+        if (a.foqp) {
+            xijxj[ax] = T(.5 * (L.x[ax] - g[ax]));
+            xijxi[ax] = T(.5 * (g[ax] - L.x[ax]));
+        } else {
+            double xij = L.x[ax] + a.g.h / 4. * (g[ax] - L.x[ax]);
+            xijxi[ax] = T(xij - L.x[ax]);
+            xijxj[ax] = T(xij - g[ax]);
+        }
+    }
+    T vF[D];
+    if (!a.move_particles) {
+        for (int ax = 0; ax < D; ++ax) vF[ax] = T(0);
+    } else if (a.foqp) {
+        for (int ax = 0; ax < D; ++ax) vF[ax] = (PD_V(L, ax) + PD_V(H, ax)) / T(2);
+    } else {
+        T dp = dotT<D, T>(xijxi, xjxi), ds = dotT<D, T>(xjxi, xjxi);
+        for (int ax = 0; ax < D; ++ax) vF[ax] = PD_V(L, ax) + (PD_V(H, ax) - PD_V(L, ax)) * dp / ds;
+    }
+    T WR[NF], WL[NF];
+    WR[0] = PD_RHO(L); WL[0] = PD_RHO(H);
+    WR[1] = PD_P(L);   WL[1] = PD_P(H);
+    for (int ax = 0; ax < D; ++ax) {
+        WR[2 + ax] = PD_V(L, ax) - vF[ax];
+        WL[2 + ax] = PD_V(H, ax) - vF[ax];
+    }
+    T WRb[NF], WLb[NF];
+    for (int k = 0; k < NF; ++k) { WRb[k] = WR[k]; WLb[k] = WL[k]; }
+    // gradient field order in the record: rho, v..., P
+    WR[0] += dotT<D, T>(&PD_G(L, 0, 0), xijxi);
+    WL[0] += dotT<D, T>(&PD_G(H, 0, 0), xijxj);
+    WR[1] += dotT<D, T>(&PD_G(L, D + 1, 0), xijxi);
+    WL[1] += dotT<D, T>(&PD_G(H, D + 1, 0), xijxj);
+    for (int ax = 0; ax < D; ++ax) {
+        WR[2 + ax] += dotT<D, T>(&PD_G(L, 1 + ax, 0), xijxi);
+        WL[2 + ax] += dotT<D, T>(&PD_G(H, 1 + ax, 0), xijxj);
+    }
+    if (a.pairwise_limiter && !image) {
+        T ai = T(0), aj = T(0), ad = T(0);
+        for (int ax = 0; ax < D; ++ax) {
+            ai += xijxi[ax] * xijxi[ax];
+            aj += xijxj[ax] * xijxj[ax];
+            ad += xjxi[ax] * xjxi[ax];
+        }
+        ai = tsqrt(ai); aj = tsqrt(aj); ad = tsqrt(ad);
+        for (int k = 0; k < NF; ++k) {
+            WR[k] = pairwise_limiter<T>(a, WR[k], WRb[k], WLb[k], ai, ad);
+            WL[k] = pairwise_limiter<T>(a, WL[k], WLb[k], WRb[k], aj, ad);
+        }
+    }
+    // half-step predictor
+    const T gam = T(a.gamma), hdt = dt / T(2);
+    T viDiv = PD_G(L, 1, 0) + PD_G(L, 2, 1);
+    T vjDiv = PD_G(H, 1, 0) + PD_G(H, 2, 1);
+    if (D == 3) { viDiv += PD_G(L, 3, 2); vjDiv += PD_G(H, 3, 2); }
+    T vxi = PD_V(L, 0) - vF[0], vyi = PD_V(L, 1) - vF[1];
+    T vxj = PD_V(H, 0) - vF[0], vyj = PD_V(H, 1) - vF[1];
+    const int GP = D + 1;
+    WR[0] -= hdt * (PD_RHO(L) * viDiv + vxi * PD_G(L, 0, 0) + vyi * PD_G(L, 0, 1));
+    WL[0] -= hdt * (PD_RHO(H) * vjDiv + vxj * PD_G(H, 0, 0) + vyj * PD_G(H, 0, 1));
+    WR[1] -= hdt * (gam * PD_P(L) * viDiv + vxi * PD_G(L, GP, 0) + vyi * PD_G(L, GP, 1));
+    WL[1] -= hdt * (gam * PD_P(H) * vjDiv + vxj * PD_G(H, GP, 0) + vyj * PD_G(H, GP, 1));
+    WR[2] -= hdt * (PD_G(L, GP, 0) / PD_RHO(L) + vxi * PD_G(L, 1, 0) + vyi * PD_G(L, 1, 1));
+    WL[2] -= hdt * (PD_G(H, GP, 0) / PD_RHO(H) + vxj * PD_G(H, 1, 0) + vyj * PD_G(H, 1, 1));
+    WR[3] -= hdt * (PD_G(L, GP, 1) / PD_RHO(L) + vxi * PD_G(L, 2, 0) + vyi * PD_G(L, 2, 1));
+    WL[3] -= hdt * (PD_G(H, GP, 1) / PD_RHO(H) + vxj * PD_G(H, 2, 0) + vyj * PD_G(H, 2, 1));
+    if (D == 3) {
+        T vzi = PD_V(L, 2) - vF[2], vzj = PD_V(H, 2) - vF[2];
+        T vzq = a.quirk3d ? vzi : vzj;
+        WR[0] -= hdt * vzi * PD_G(L, 0, 2);
+        WL[0] -= hdt * vzj * PD_G(H, 0, 2);
+        WR[1] -= hdt * vzi * PD_G(L, GP, 2);
+        WL[1] -= hdt * vzj * PD_G(H, GP, 2);
+        WR[2] -= hdt * vzi * PD_G(L, 1, 2);
+        WL[2] -= hdt * vzq * PD_G(H, 1, 2);
+        WR[3] -= hdt * vzi * PD_G(L, 2, 2);
+        WL[3] -= hdt * vzq * PD_G(H, 2, 2);
+        WR[4] -= hdt * (PD_G(L, GP, 2) / PD_RHO(L) + vxi * PD_G(L, 3, 0) + vyi * PD_G(L, 3, 1) +
+                        vzi * PD_G(L, 3, 2));
+        WL[4] -= hdt * (PD_G(H, GP, 2) / PD_RHO(H) + vxj * PD_G(H, 3, 0) + vyj * PD_G(H, 3, 1) +
+                        vzj * PD_G(H, 3, 2));
+    }
+    if (WR[1] < T(0) || WL[1] < T(0)) {
+        if (image) fs.neg_ghost++;
+        else fs.neg_real++;
+    }
+    // Riemann (Riemann.cpp:7-141): face frame along A/|A|; solver left = L
+    T An = tsqrt(dotT<D, T>(A, A));
+    T hA[D];
+    for (int ax = 0; ax < D; ++ax) hA[ax] = T(1) / An * A[ax];
+    T ui, uj;
+    T ti0 = T(0), tj0 = T(0);
+    if (D == 2) {
+        // rotationMatrix2D(hatA, e_x), Helper.cpp:39-45
+        T L0 = hA[0] * T(1) + hA[1] * T(0);
+        T L1 = -(hA[0] * T(0) - hA[1] * T(1));
+        T L2 = -L1, L3 = L0;
+        ui = L0 * WR[2] + L1 * WR[3];
+        ti0 = L2 * WR[2] + L3 * WR[3];
+        uj = L0 * WL[2] + L1 * WL[3];
+        tj0 = L2 * WL[2] + L3 * WL[3];
+    } else {
+        ui = dotT<D, T>(hA, &WR[2]);
+        uj = dotT<D, T>(hA, &WL[2]);
+    }
+    T rs, us, Ps;
+    int flag;
+    float seed = -1.f;
+    if (a.learned) seed = mlp_seed_scalar(a.mlp, a.gamma, (double)WR[0], (double)ui, (double)WR[1],
+                                          (double)WL[0], (double)uj, (double)WL[1]);
+    if constexpr (sizeof(T) == 8) {
+        GasConst<double> gc(a.gamma);
+        RSol<double> r = solve_fp64(gc, WR[0], ui, WR[1], WL[0], uj, WL[1], a.riemann_tol,
+                                    a.riemann_max_iter, (double)seed, fs.st);
+        rs = r.d; us = r.u; Ps = r.p; flag = r.flag;
+    } else {
+        GasConst<float> gf(a.gamma);
+        GasConst<double> gc(a.gamma);
+        RSol<float> r = solve_mixed(gf, gc, WR[0], ui, WR[1], WL[0], uj, WL[1], a.riemann_tol,
+                                    a.riemann_max_iter, a.learned_iters, seed, fs.st);
+        rs = r.d; us = r.u; Ps = r.p; flag = r.flag;
+    }
+    if (flag == 0) fs.vacuum++;
+    T vS[D];
+    if (D == 2) {
+        T t0 = flag == 1 ? tj0 : (flag == -1 ? ti0 : T(0));
+        T K0 = T(1) * hA[0] + T(0) * hA[1];
+        T K1 = -(T(1) * hA[1] - T(0) * hA[0]);
+        T K2 = -K1, K3 = K0;
+        vS[0] = K0 * us + K1 * t0;
+        vS[1] = K2 * us + K3 * t0;
+    } else {
+        const T *vup = flag == 1 ? &WL[2] : &WR[2];
+        T un = flag == 1 ? uj : ui;
+        for (int ax = 0; ax < D; ++ax)
+            vS[ax] = flag == 0 ? us * hA[ax] : us * hA[ax] + (vup[ax] - un * hA[ax]);
+    }
+    // fluxes projected on A (Riemann.cpp:164-181, 3D :204-227)
+    T vLab[D];
+    T f0 = T(0);
+    for (int ax = 0; ax < D; ++ax) f0 += A[ax] * rs * vS[ax];
+    F[0] = f0;
+    for (int ax = 0; ax < D; ++ax) vLab[ax] = vS[ax] + vF[ax];
+    if (a.mfm)
+        for (int ax = 0; ax < D; ++ax) vS[ax] = T(0);
+    for (int c = 0; c < D; ++c) {
+        T sm = T(0);
+        for (int b = 0; b < D; ++b)
+            sm += b == c ? A[b] * (rs * vLab[c] * vS[b] + Ps) : A[b] * rs * vLab[c] * vS[b];
+        F[2 + c] = sm;
+    }
+    T ekin = dotT<D, T>(vLab, vLab);
+    T e = T(0);
+    for (int b = 0; b < D; ++b)
+        e += A[b] * (vS[b] * (Ps / (gam - T(1)) + rs * T(.5) * ekin) + Ps * vLab[b]);
+    F[1] = e;
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
+    constexpr int NF = D + 2;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    FaceStats fs{};
+    if (i < a.N) {
+        const int N = a.N;
+        PData<D, T> S, O, L, H;
+        load_pdata<D, T>(a, i, S);
+        const double dtd = a.dt[0];
+        const T dt = T(dtd);
+        int nr = a.noi[i], cnt = nr + a.nog[i];
+        double mF = 0., eF = 0., vFs[3] = {0., 0., 0.};
+        for (int n = 0; n < cnt; ++n) {
+            uint32_t e = a.nbr[(size_t)n * N + i];
+            int j = (int)nb_index(e);
+            uint32_t code = nb_code(e);
+            bool image = code != kCodeReal;
+            int s[3] = {0, 0, 0};
+            if (image)
+                for (int ax = 0; ax < D; ++ax) s[ax] = code_shift(code, ax);
+            // side that solves the pair: the lower index (images: parent >= i)
+            bool self_lo = !a.enforce_flux_sym || (image ? j >= i : j > i);
+            load_pdata<D, T>(a, j, O);
+            for (int ax = 0; ax < 3; ++ax) {
+                L.x[ax] = self_lo ? S.x[ax] : O.x[ax];
+                H.x[ax] = self_lo ? O.x[ax] : S.x[ax];
+            }
+            for (int k = 0; k < PData<D, T>::NT; ++k) {
+                L.t[k] = self_lo ? S.t[k] : O.t[k];
+                H.t[k] = self_lo ? O.t[k] : S.t[k];
+            }
+            int sl[3];
+            for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
+            T F[NF];
+            face_flux<D, T>(a, L, H, sl, image, dt, F, fs);
+            double sgn = self_lo ? 1. : -1.;
+            mF += sgn * (double)F[0];
+            for (int ax = 0; ax < D; ++ax) vFs[ax] += sgn * (double)F[2 + ax];
+            eF += sgn * (double)F[1];
+        }
+        if (a.write_fluxes) {
+            a.mF[i] = mF;
+            a.eF[i] = eF;
+            for (int ax = 0; ax < D; ++ax) a.vF[(size_t)i * D + ax] = vFs[ax];
+        }
+        // updateStateAndPosition, Particles.cpp:2013-2110
+        double vo[3] = {a.v[0][i], a.v[1][i], D == 3 ? a.v[2][i] : 0.};
+        double m0 = a.m[i], u0 = a.u[i];
+        double ke = vo[0] * vo[0] + vo[1] * vo[1];
+        if (D == 3) ke += vo[2] * vo[2];
+        double Q0 = m0 * (u0 + .5 * ke);
+        double Q[3];
+        for (int ax = 0; ax < D; ++ax) Q[ax] = m0 * vo[ax];
+        double m1 = a.mfm ? m0 : m0 - dtd * mF;
+        double vn[3];
+        for (int ax = 0; ax < D; ++ax) {
+            Q[ax] -= dtd * vFs[ax];
+            vn[ax] = Q[ax] / m1;
+        }
+        Q0 -= dtd * eF;
+        double kn = vn[0] * vn[0] + vn[1] * vn[1];
+        if (D == 3) kn += vn[2] * vn[2];
+        double u1 = Q0 / m1 - .5 * kn;
+        a.mn[i] = m1;
+        a.un[i] = u1;
+        for (int ax = 0; ax < D; ++ax) {
+            a.vn[ax][i] = vn[ax];
+            double x = a.x[ax][i];
+            if (a.move_particles) {
+                x += vo[ax] * dtd;
+                if (a.g.periodic) {
+                    double mn = a.g.bmin[ax], mx = a.g.bmax[ax];
+                    if (x < mn) x = mx - (mn - x);
+                    else if (mx <= x) x = mn + (x - mx);
+                }
+            }
+            a.xn[ax][i] = x;
+        }
+    }
+    // diagnostics: one atomic per wavefront and counter
+    unsigned long long c[kNumCounters] = {
+        (unsigned long long)fs.neg_real, (unsigned long long)fs.neg_ghost,
+        (unsigned long long)fs.vacuum, (unsigned long long)fs.st.nonconv, 0ull,
+        (unsigned long long)fs.st.fallback, (unsigned long long)fs.st.iters,
+        (unsigned long long)0};
+    if (i < a.N) c[kCntFaces] = (unsigned long long)(a.noi[i] + a.nog[i]);
+    for (int k = 0; k < kNumCounters; ++k) {
+        unsigned long long v = wave_sum(c[k]);
+        if (lane_id() == 0 && v) atomicAdd(&a.counters[k], v);
+    }
+}
+
+// conserved sums (Particles.cpp:2830-2886): V, M, E, px, py, pz per block
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_sums(HydroArgs a, double *out) {
+    __shared__ double sh[6][256];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (i < a.N) {
+        const T *pr = (const T *)a.prim + (size_t)i * kPrimN;
+        double om = (double)pr[kPrimOmega];
+        double m = a.m[i];
+        double vx = a.v[0][i], vy = a.v[1][i], vz = D == 3 ? a.v[2][i] : 0.;
+        double ke = vx * vx + vy * vy;
+        if (D == 3) ke += vz * vz;
+        v[0] = om > 0. ? 1. / om : 0.;
+        v[1] = m;
+        v[2] = m * (a.u[i] + .5 * ke);
+        v[3] = m * vx;
+        v[4] = m * vy;
+        v[5] = m * vz;
+    }
+    for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; ++k) out[(size_t)blockIdx.x * 6 + k] = sh[k][0];
+}
+
+// ------------------------------------------------------------------ launchers
+#define MLH_DISPATCH(KERNEL, D, P, GRID, ...)                                             \
+    do {                                                                                  \
+        if (D == 2 && P == 0) hipLaunchKernelGGL((KERNEL<2, double>), GRID, dim3(256), 0, s, __VA_ARGS__); \
+        else if (D == 2) hipLaunchKernelGGL((KERNEL<2, float>), GRID, dim3(256), 0, s, __VA_ARGS__);  \
+        else if (P == 0) hipLaunchKernelGGL((KERNEL<3, double>), GRID, dim3(256), 0, s, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERNEL<3, float>), GRID, dim3(256), 0, s, __VA_ARGS__);              \
+    } while (0)
+
+int dt_blocks(int N) { return (N + 255) / 256; }
+
+void launch_density(const HydroArgs &a, int D, int precision, hipStream_t s) {
+    MLH_DISPATCH(k_density, D, precision, dim3((a.N + 255) / 256), a);
+}
+void launch_gradients(const HydroArgs &a, int D, int precision, hipStream_t s) {
+    MLH_DISPATCH(k_gradients, D, precision, dim3((a.N + 255) / 256), a);
+}
+void launch_dt_finalize(const HydroArgs &a, double dt_fixed, double t_now, double t_clip,
+                        double *dt_hist, int step, hipStream_t s) {
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s, a.dt_block, dt_blocks(a.N),
+                       dt_fixed, t_now, t_clip, a.dt, dt_hist, step);
+}
+void launch_faces(const HydroArgs &a, int D, int precision, hipStream_t s) {
+    MLH_DISPATCH(k_faces, D, precision, dim3((a.N + 255) / 256), a);
+}
+void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s) {
+    MLH_DISPATCH(k_sums, D, precision, dim3((a.N + 255) / 256), a, out);
+}
+
+}  // namespace mlh

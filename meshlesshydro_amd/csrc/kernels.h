@@ -1,0 +1,70 @@
+// kernels.h -- kernel argument structs and host-side launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "device_common.h"
+
+namespace mlh {
+
+struct NbrArgs {
+    Geom g;
+    int N, maxr, maxg;
+    const double4 *pos;
+    const uint32_t *cell_key;    // cell of particle i
+    const uint32_t *sorted_idx;  // particles in cell order
+    const int *cstart, *cend;
+    uint32_t *nbr;               // [maxr+maxg][N]
+    int *noi, *nog;
+    int *status;                 // bit 1: real overflow, bit 3: image overflow
+    int *max_counts;             // [2]
+};
+
+// Arguments of the physics kernels.  Storage type of derived per-particle
+// data (prim, frec) is T; positions / state are FP64.
+struct HydroArgs {
+    Geom g;
+    int N, stride;               // stride = rows of nbr (N)
+    double gamma, cfl, beta, psi1, psi2, riemann_tol;
+    int riemann_max_iter, learned_iters;
+    int limiter_abs, slope_limiting, pairwise_limiter, mfm, enforce_flux_sym;
+    int move_particles, foqp, quirk3d, write_fluxes, learned;
+    const double4 *pos;
+    const uint32_t *nbr;
+    const int *noi, *nog;
+    // state (current) and next state
+    const double *x[3], *v[3], *m, *u;
+    double *xn[3], *vn[3], *mn, *un;
+    void *prim;                  // T[N][8]
+    void *frec;                  // T[N][FRec<D>::N]
+    double *dt_block;            // per-block minimum of the CFL step
+    double *dt;                  // [0] step used, device resident
+    double *mF, *vF, *eF;        // optional flux sums (write_fluxes)
+    unsigned long long *counters;  // mlh_diag counters
+    int *status;
+    const float *mlp;            // learned seed parameters (device)
+};
+
+enum CounterIdx {
+    kCntNegReal = 0, kCntNegGhost, kCntVacuum, kCntNonconv, kCntMissing,
+    kCntFallback, kCntIters, kCntFaces, kNumCounters
+};
+
+void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
+                  const double *x2, int N, double4 *pos, uint32_t *key, uint32_t *idx,
+                  uint32_t *key2, uint32_t *idx2, uint32_t *scratch, int *cstart,
+                  int *cend, hipStream_t s);
+void launch_neighbours(const NbrArgs &a, int D, hipStream_t s);
+size_t radix_sort_scratch_words(int N);
+void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *tmp,
+                        hipStream_t s);
+
+// precision: 0 = FP64, 1 = FP32 pair arithmetic
+void launch_density(const HydroArgs &a, int D, int precision, hipStream_t s);
+void launch_gradients(const HydroArgs &a, int D, int precision, hipStream_t s);
+int dt_blocks(int N);
+void launch_dt_finalize(const HydroArgs &a, double dt_fixed, double t_now, double t_clip,
+                        double *dt_hist, int step, hipStream_t s);
+void launch_faces(const HydroArgs &a, int D, int precision, hipStream_t s);
+void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s);
+
+}  // namespace mlh

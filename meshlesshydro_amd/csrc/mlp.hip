@@ -1,0 +1,481 @@
+// mlp.hip -- learned p* seed: training (FP32 forward/backward + Adam),
+// inference (FP32 per lane, and BF16 on the matrix cores), face collection,
+// and the batched exact Riemann entry point.
+//
+// BF16 inference runs one wavefront per 64 faces with
+// v_mfma_f32_16x16x32_bf16 in the transposed form Z^T = W X^T, so that a
+// layer's accumulator fragments are, lane for lane, the next layer's B
+// operand (the K index inside one MFMA is only a label: W2's columns are
+// permuted to match once, when the fragments are packed).  Biases ride in
+// the accumulator initialisation; the last 64 -> 1 layer is a per-lane dot
+// product plus two cross-lane adds.  Activations never leave registers;
+// only the features go through 2 KB of LDS per wavefront.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include <vector>
+#include "../../include/mlh.h"
+#include "device_common.h"
+#include "mlp_api.h"
+#include "mlp_device.h"
+#include "mlp_mfma.h"
+#include "riemann.h"
+
+using namespace mlh;
+
+namespace mlh {
+
+void mlp_free(MlpState &m) {
+    if (m.params) hipFree(m.params);
+    if (m.frags) hipFree(m.frags);
+    if (m.adam_m) hipFree(m.adam_m);
+    if (m.adam_v) hipFree(m.adam_v);
+    if (m.grad) hipFree(m.grad);
+    if (m.collect) hipFree(m.collect);
+    if (m.collect_n) hipFree(m.collect_n);
+    m = MlpState{};
+}
+
+// ---- packing of the MFMA fragments ------------------------------------------
+__global__ void k_mlp_pack(const float *__restrict__ P, MlpFrags *__restrict__ F) {
+    int l = threadIdx.x;  // 64 threads = one lane each
+    int g = l >> 4, c = l & 15;
+    for (int mt = 0; mt < 4; ++mt) {
+        for (int j = 0; j < 8; ++j) {
+            int k = 8 * g + j;
+            float w = k < kMlpIn ? P[kMlpW1 + (16 * mt + c) * kMlpIn + k] : 0.f;
+            __bf16 b = (__bf16)w;
+            F->w1[mt][l][j] = *(uint16_t *)&b;
+        }
+        for (int r = 0; r < 4; ++r) {
+            int n = 16 * mt + 4 * g + r;
+            F->b1[mt][l][r] = P[kMlpB1 + n];
+            F->b2[mt][l][r] = P[kMlpB2 + n];
+            F->w3[mt][l][r] = P[kMlpW3 + n];
+        }
+        for (int ks = 0; ks < 2; ++ks)
+            for (int j = 0; j < 8; ++j) {
+                int in = mfma_h1_neuron(ks, g, j);
+                __bf16 b = (__bf16)P[kMlpW2 + (16 * mt + c) * kMlpH + in];
+                F->w2[mt][ks][l][j] = *(uint16_t *)&b;
+            }
+    }
+    if (l == 0) F->b3 = P[kMlpB3];
+}
+
+// ---- training --------------------------------------------------------------
+// Per sample: forward, dy, dz2 = dy w3 (1-h2^2), dz1 = (W2^T dz2)(1-h1^2).
+__global__ __launch_bounds__(256) void k_mlp_fwd_bwd(
+    const float *__restrict__ P, const float *__restrict__ X, const float *__restrict__ T,
+    const int *__restrict__ idx, int B, float *__restrict__ H1, float *__restrict__ DZ1,
+    float *__restrict__ H2, float *__restrict__ DZ2, float *__restrict__ DY,
+    float *__restrict__ E2, float *__restrict__ XB) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    int s = idx ? idx[b] : b;
+    float x[kMlpIn];
+    for (int k = 0; k < kMlpIn; ++k) {
+        x[k] = X[(size_t)s * kMlpIn + k];
+        XB[(size_t)b * kMlpIn + k] = x[k];
+    }
+    float h1[kMlpH], h2[kMlpH];
+    for (int o = 0; o < kMlpH; ++o) {
+        float a = P[kMlpB1 + o];
+        for (int k = 0; k < kMlpIn; ++k) a += P[kMlpW1 + o * kMlpIn + k] * x[k];
+        h1[o] = tanhf(a);
+    }
+    float y = P[kMlpB3];
+    for (int o = 0; o < kMlpH; ++o) {
+        float a = P[kMlpB2 + o];
+        for (int k = 0; k < kMlpH; ++k) a += P[kMlpW2 + o * kMlpH + k] * h1[k];
+        h2[o] = tanhf(a);
+        y += P[kMlpW3 + o] * h2[o];
+    }
+    float e = y - T[s];
+    float dy = 2.f * e / (float)B;
+    DY[b] = dy;
+    E2[b] = e * e;
+    float dz2[kMlpH];
+    for (int o = 0; o < kMlpH; ++o) {
+        dz2[o] = dy * P[kMlpW3 + o] * (1.f - h2[o] * h2[o]);
+        H2[(size_t)b * kMlpH + o] = h2[o];
+        DZ2[(size_t)b * kMlpH + o] = dz2[o];
+        H1[(size_t)b * kMlpH + o] = h1[o];
+    }
+    for (int k = 0; k < kMlpH; ++k) {
+        float a = 0.f;
+        for (int o = 0; o < kMlpH; ++o) a += P[kMlpW2 + o * kMlpH + k] * dz2[o];
+        DZ1[(size_t)b * kMlpH + k] = a * (1.f - h1[k] * h1[k]);
+    }
+}
+
+// One thread per parameter; sequential sum over the batch (deterministic).
+__global__ __launch_bounds__(256) void k_mlp_wgrad(int B, const float *__restrict__ XB,
+                                                   const float *__restrict__ H1,
+                                                   const float *__restrict__ DZ1,
+                                                   const float *__restrict__ H2,
+                                                   const float *__restrict__ DZ2,
+                                                   const float *__restrict__ DY,
+                                                   float *__restrict__ G) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpN) return;
+    const float *A;
+    const float *Bv;
+    int sa, sb;
+    if (p < kMlpB1) {  // W1[o][k]
+        int o = p / kMlpIn, k = p % kMlpIn;
+        A = DZ1 + o; sa = kMlpH; Bv = XB + k; sb = kMlpIn;
+    } else if (p < kMlpW2) {
+        A = DZ1 + (p - kMlpB1); sa = kMlpH; Bv = nullptr; sb = 0;
+    } else if (p < kMlpB2) {
+        int q = p - kMlpW2, o = q / kMlpH, k = q % kMlpH;
+        A = DZ2 + o; sa = kMlpH; Bv = H1 + k; sb = kMlpH;
+    } else if (p < kMlpW3) {
+        A = DZ2 + (p - kMlpB2); sa = kMlpH; Bv = nullptr; sb = 0;
+    } else if (p < kMlpB3) {
+        A = DY; sa = 1; Bv = H2 + (p - kMlpW3); sb = kMlpH;
+    } else {
+        A = DY; sa = 1; Bv = nullptr; sb = 0;
+    }
+    float acc = 0.f;
+    for (int b = 0; b < B; ++b) acc += A[(size_t)b * sa] * (Bv ? Bv[(size_t)b * sb] : 1.f);
+    G[p] = acc;
+}
+
+__global__ void k_sum_f32(const float *__restrict__ v, int n, float *__restrict__ out) {
+    __shared__ float sh[256];
+    float a = 0.f;
+    for (int k = threadIdx.x; k < n; k += 256) a += v[k];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0] / (float)n;
+}
+
+__global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ M,
+                       float *__restrict__ V, float lr, float b1c, float b2c) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpN) return;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    float g = G[p];
+    float m = b1 * M[p] + (1.f - b1) * g;
+    float v = b2 * V[p] + (1.f - b2) * g * g;
+    M[p] = m;
+    V[p] = v;
+    P[p] -= lr * (m / b1c) / (sqrtf(v / b2c) + eps);
+}
+
+// ---- inference ---------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mlp_infer_f32(const float *__restrict__ P,
+                                                       const float *__restrict__ X, int64_t n,
+                                                       float *__restrict__ Y) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    float x[kMlpIn];
+    for (int k = 0; k < kMlpIn; ++k) x[k] = X[s * kMlpIn + k];
+    Y[s] = mlp_forward_scalar(P, x);
+}
+
+__global__ __launch_bounds__(256) void k_mlp_infer_bf16(const MlpFrags *__restrict__ F,
+                                                        const float *__restrict__ X, int64_t n,
+                                                        float *__restrict__ Y) {
+    __shared__ uint16_t lds[4][64 * 16];
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int w = threadIdx.x >> 6;
+    bool active = s < n;
+    float x[kMlpIn];
+    for (int k = 0; k < kMlpIn; ++k) x[k] = active ? X[s * kMlpIn + k] : 0.f;
+    MlpRegs R;
+    mlp_load_regs(F, R);
+    float y = mlp_infer_wave_bf16(R, x, lds[w]);
+    if (active) Y[s] = y;
+}
+
+// ---- batched exact Riemann solver -------------------------------------------
+__global__ __launch_bounds__(256) void k_riemann_batch(
+    int64_t n, double gamma, const double *__restrict__ dl, const double *__restrict__ ul,
+    const double *__restrict__ pl, const double *__restrict__ dr, const double *__restrict__ ur,
+    const double *__restrict__ pr, double *__restrict__ ds, double *__restrict__ us,
+    double *__restrict__ ps, int *__restrict__ flag, int *__restrict__ iters, int precision,
+    const MlpFrags *__restrict__ F, double tol, int max_iter, int iters32,
+    unsigned long long *__restrict__ fallbacks) {
+    __shared__ uint16_t lds[4][64 * 16];
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = k < n;
+    double a0 = active ? dl[k] : 1., a1 = active ? ul[k] : 0., a2 = active ? pl[k] : 1.;
+    double b0 = active ? dr[k] : 1., b1 = active ? ur[k] : 0., b2 = active ? pr[k] : 1.;
+    float seed = -1.f;
+    if (F) {
+        // every lane of the wavefront takes part in the MFMA inference
+        float f[kMlpIn];
+        bool ok = a0 > 0. && b0 > 0. && a2 > 0. && b2 > 0.;
+        float pref = ok ? mlp_features((float)gamma, (float)a0, (float)a1, (float)a2, (float)b0,
+                                       (float)b1, (float)b2, f)
+                        : 1.f;
+        if (!ok)
+            for (int q = 0; q < kMlpIn; ++q) f[q] = 0.f;
+        MlpRegs R;
+        mlp_load_regs(F, R);
+        float y = mlp_infer_wave_bf16(R, f, lds[threadIdx.x >> 6]);
+        y = fminf(fmaxf(y, -40.f), 5.f);
+        seed = ok ? pref * __expf(y) : -1.f;
+    }
+    if (!active) return;
+    SolveStats st{};
+    double rd, ru, rp;
+    int fl;
+    if (precision == MLH_FP64) {
+        GasConst<double> gc(gamma);
+        RSol<double> r = solve_fp64(gc, a0, a1, a2, b0, b1, b2, tol, max_iter, (double)seed, st);
+        rd = r.d; ru = r.u; rp = r.p; fl = r.flag;
+    } else {
+        GasConst<float> gf(gamma);
+        GasConst<double> gc(gamma);
+        RSol<float> r = solve_mixed(gf, gc, (float)a0, (float)a1, (float)a2, (float)b0,
+                                    (float)b1, (float)b2, tol, max_iter, iters32, seed, st);
+        rd = r.d; ru = r.u; rp = r.p; fl = r.flag;
+    }
+    ds[k] = rd; us[k] = ru; ps[k] = rp;
+    if (flag) flag[k] = fl;
+    if (iters) iters[k] = st.iters;
+    if (st.fallback) atomicAdd(fallbacks, (unsigned long long)st.fallback);
+}
+
+}  // namespace mlh
+
+// ---- C ABI --------------------------------------------------------------------
+#define CHK(expr)                                                                    \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) return ctx_fail(MLH_ERR_DEVICE, hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename P>
+static hipError_t dmalloc(P **p, size_t bytes) {
+    return hipMalloc((void **)p, bytes ? bytes : 16);
+}
+
+static int ensure_mlp(mlh_ctx *c) {
+    MlpState &m = ctx_mlp(c);
+    if (!m.params) {
+        CHK(dmalloc(&m.params, sizeof(float) * kMlpN));
+        CHK(dmalloc(&m.frags, sizeof(MlpFrags)));
+        CHK(dmalloc(&m.adam_m, sizeof(float) * kMlpN));
+        CHK(dmalloc(&m.adam_v, sizeof(float) * kMlpN));
+        CHK(dmalloc(&m.grad, sizeof(float) * kMlpN));
+        CHK(hipMemset(m.adam_m, 0, sizeof(float) * kMlpN));
+        CHK(hipMemset(m.adam_v, 0, sizeof(float) * kMlpN));
+        m.adam_t = 0;
+    }
+    return MLH_OK;
+}
+
+static void repack(mlh_ctx *c) {
+    MlpState &m = ctx_mlp(c);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, ctx_stream(c), m.params, m.frags);
+}
+
+extern "C" {
+
+int mlh_mlp_features(int64_t n, double gamma, const double *rhoL, const double *uL,
+                     const double *PL, const double *rhoR, const double *uR, const double *PR,
+                     float *feats, double *p_ref) {
+    if (n < 0 || !rhoL || !uL || !PL || !rhoR || !uR || !PR || !feats)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    for (int64_t k = 0; k < n; ++k) {
+        float pr = mlp_features((float)gamma, (float)rhoL[k], (float)uL[k], (float)PL[k],
+                                (float)rhoR[k], (float)uR[k], (float)PR[k], &feats[k * kMlpIn]);
+        if (p_ref) p_ref[k] = pr;
+    }
+    return MLH_OK;
+}
+
+int mlh_mlp_set_params(mlh_ctx *c, const float *params) {
+    if (!c || !params) return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    int st = ensure_mlp(c);
+    if (st) return st;
+    MlpState &m = ctx_mlp(c);
+    CHK(hipMemcpyAsync(m.params, params, sizeof(float) * kMlpN, hipMemcpyHostToDevice,
+                       ctx_stream(c)));
+    CHK(hipMemsetAsync(m.adam_m, 0, sizeof(float) * kMlpN, ctx_stream(c)));
+    CHK(hipMemsetAsync(m.adam_v, 0, sizeof(float) * kMlpN, ctx_stream(c)));
+    m.adam_t = 0;
+    repack(c);
+    CHK(hipStreamSynchronize(ctx_stream(c)));
+    return MLH_OK;
+}
+
+int mlh_mlp_get_params(mlh_ctx *c, float *params) {
+    if (!c || !params) return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    MlpState &m = ctx_mlp(c);
+    if (!m.params) return ctx_fail(MLH_ERR_NOT_TRAINED, "no network parameters");
+    CHK(hipMemcpy(params, m.params, sizeof(float) * kMlpN, hipMemcpyDeviceToHost));
+    return MLH_OK;
+}
+
+struct TrainBufs {
+    float *X = nullptr, *T = nullptr, *H1 = nullptr, *DZ1 = nullptr, *H2 = nullptr,
+          *DZ2 = nullptr, *DY = nullptr, *E2 = nullptr, *XB = nullptr, *loss = nullptr;
+    int *idx = nullptr;
+    ~TrainBufs() {
+        float *f[] = {X, T, H1, DZ1, H2, DZ2, DY, E2, XB, loss};
+        for (float *p : f)
+            if (p) hipFree(p);
+        if (idx) hipFree(idx);
+    }
+};
+
+static int alloc_train(TrainBufs &b, int64_t n, int batch) {
+    CHK(dmalloc(&b.X, sizeof(float) * n * kMlpIn));
+    CHK(dmalloc(&b.T, sizeof(float) * n));
+    CHK(dmalloc(&b.H1, sizeof(float) * batch * kMlpH));
+    CHK(dmalloc(&b.DZ1, sizeof(float) * batch * kMlpH));
+    CHK(dmalloc(&b.H2, sizeof(float) * batch * kMlpH));
+    CHK(dmalloc(&b.DZ2, sizeof(float) * batch * kMlpH));
+    CHK(dmalloc(&b.DY, sizeof(float) * batch));
+    CHK(dmalloc(&b.E2, sizeof(float) * batch));
+    CHK(dmalloc(&b.XB, sizeof(float) * batch * kMlpIn));
+    CHK(dmalloc(&b.loss, sizeof(float)));
+    CHK(dmalloc(&b.idx, sizeof(int) * n));
+    return MLH_OK;
+}
+
+static void enqueue_grad(mlh_ctx *c, TrainBufs &b, const int *idx, int B) {
+    MlpState &m = ctx_mlp(c);
+    hipStream_t s = ctx_stream(c);
+    hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3((B + 255) / 256), dim3(256), 0, s, m.params, b.X, b.T,
+                       idx, B, b.H1, b.DZ1, b.H2, b.DZ2, b.DY, b.E2, b.XB);
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3((kMlpN + 255) / 256), dim3(256), 0, s, B, b.XB, b.H1,
+                       b.DZ1, b.H2, b.DZ2, b.DY, m.grad);
+    hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, s, b.E2, B, b.loss);
+}
+
+int mlh_mlp_loss_grad(mlh_ctx *c, int64_t n, const float *feats, const float *target,
+                      float *loss, float *grad) {
+    if (!c || n <= 0 || n > (1 << 20) || !feats || !target)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
+    MlpState &m = ctx_mlp(c);
+    if (!m.params) return ctx_fail(MLH_ERR_NOT_TRAINED, "no network parameters");
+    TrainBufs b;
+    int st = alloc_train(b, n, (int)n);
+    if (st) return st;
+    hipStream_t s = ctx_stream(c);
+    CHK(hipMemcpyAsync(b.X, feats, sizeof(float) * n * kMlpIn, hipMemcpyHostToDevice, s));
+    CHK(hipMemcpyAsync(b.T, target, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    enqueue_grad(c, b, nullptr, (int)n);
+    if (loss) CHK(hipMemcpyAsync(loss, b.loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    if (grad) CHK(hipMemcpyAsync(grad, m.grad, sizeof(float) * kMlpN, hipMemcpyDeviceToHost, s));
+    CHK(hipStreamSynchronize(s));
+    return MLH_OK;
+}
+
+int mlh_mlp_train(mlh_ctx *c, int64_t n, const float *feats, const float *target,
+                  int32_t n_epochs, int32_t batch, float lr, uint64_t seed, float *final_loss) {
+    if (!c || n <= 0 || !feats || !target || batch <= 0 || n_epochs < 0)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
+    int st = ensure_mlp(c);
+    if (st) return st;
+    MlpState &m = ctx_mlp(c);
+    hipStream_t s = ctx_stream(c);
+    if (batch > n) batch = (int)n;
+    TrainBufs b;
+    if ((st = alloc_train(b, n, batch))) return st;
+    CHK(hipMemcpyAsync(b.X, feats, sizeof(float) * n * kMlpIn, hipMemcpyHostToDevice, s));
+    CHK(hipMemcpyAsync(b.T, target, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    std::vector<int> perm(n);
+    uint64_t rng = seed * 6364136223846793005ull + 1442695040888963407ull;
+    float last = 0.f;
+    for (int ep = 0; ep < n_epochs; ++ep) {
+        for (int64_t k = 0; k < n; ++k) perm[k] = (int)k;
+        for (int64_t k = n - 1; k > 0; --k) {  // Fisher-Yates with an LCG
+            rng = rng * 6364136223846793005ull + 1442695040888963407ull;
+            int64_t j = (int64_t)((rng >> 33) % (uint64_t)(k + 1));
+            std::swap(perm[k], perm[j]);
+        }
+        CHK(hipMemcpyAsync(b.idx, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        for (int64_t off = 0; off + batch <= n; off += batch) {
+            enqueue_grad(c, b, b.idx + off, batch);
+            m.adam_t++;
+            float b1c = 1.f - powf(0.9f, (float)m.adam_t);
+            float b2c = 1.f - powf(0.999f, (float)m.adam_t);
+            hipLaunchKernelGGL(k_adam, dim3((kMlpN + 255) / 256), dim3(256), 0, s, m.params,
+                               m.grad, m.adam_m, m.adam_v, lr, b1c, b2c);
+        }
+        CHK(hipMemcpyAsync(&last, b.loss, sizeof(float), hipMemcpyDeviceToHost, s));
+        CHK(hipStreamSynchronize(s));
+    }
+    repack(c);
+    CHK(hipStreamSynchronize(s));
+    if (final_loss) *final_loss = last;
+    return MLH_OK;
+}
+
+int mlh_mlp_infer(mlh_ctx *c, int64_t n, const float *feats, float *out, int32_t bf16) {
+    if (!c || n <= 0 || !feats || !out) return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
+    MlpState &m = ctx_mlp(c);
+    if (!m.params) return ctx_fail(MLH_ERR_NOT_TRAINED, "no network parameters");
+    hipStream_t s = ctx_stream(c);
+    float *X = nullptr, *Y = nullptr;
+    CHK(dmalloc(&X, sizeof(float) * n * kMlpIn));
+    CHK(dmalloc(&Y, sizeof(float) * n));
+    hipMemcpyAsync(X, feats, sizeof(float) * n * kMlpIn, hipMemcpyHostToDevice, s);
+    int64_t nb = (n + 255) / 256;
+    if (bf16)
+        hipLaunchKernelGGL(k_mlp_infer_bf16, dim3(nb), dim3(256), 0, s, m.frags, X, n, Y);
+    else
+        hipLaunchKernelGGL(k_mlp_infer_f32, dim3(nb), dim3(256), 0, s, m.params, X, n, Y);
+    hipMemcpyAsync(out, Y, sizeof(float) * n, hipMemcpyDeviceToHost, s);
+    hipError_t e = hipStreamSynchronize(s);
+    hipFree(X);
+    hipFree(Y);
+    if (e != hipSuccess) return ctx_fail(MLH_ERR_DEVICE, hipGetErrorString(e));
+    return MLH_OK;
+}
+
+int mlh_riemann_exact_batch(mlh_ctx *c, int64_t n, double gamma, const double *rhoL,
+                            const double *uL, const double *PL, const double *rhoR,
+                            const double *uR, const double *PR, double *rho_s, double *u_s,
+                            double *P_s, int32_t *flag, int32_t *iters, int32_t precision,
+                            int32_t seed, double tol) {
+    if (n < 0 || !rhoL || !uL || !PL || !rhoR || !uR || !PR || !rho_s || !u_s || !P_s)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    if (precision != MLH_FP64 && precision != MLH_FP32)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad precision");
+    const MlpFrags *F = nullptr;
+    if (seed == MLH_SEED_LEARNED) {
+        if (!c || !ctx_mlp(c).params) return ctx_fail(MLH_ERR_NOT_TRAINED, "no network parameters");
+        F = ctx_mlp(c).frags;
+    }
+    if (n == 0) return MLH_OK;
+    hipStream_t s = c ? ctx_stream(c) : nullptr;
+    double *buf = nullptr;
+    int *ib = nullptr;
+    unsigned long long *fb = nullptr;
+    CHK(dmalloc(&buf, sizeof(double) * n * 9));
+    CHK(dmalloc(&ib, sizeof(int) * n * 2));
+    CHK(dmalloc(&fb, sizeof(unsigned long long)));
+    size_t b = sizeof(double) * n;
+    const double *in[6] = {rhoL, uL, PL, rhoR, uR, PR};
+    for (int k = 0; k < 6; ++k) hipMemcpyAsync(buf + k * n, in[k], b, hipMemcpyHostToDevice, s);
+    hipMemsetAsync(fb, 0, sizeof(unsigned long long), s);
+    int max_iter = 50;
+    hipLaunchKernelGGL(k_riemann_batch, dim3((n + 255) / 256), dim3(256), 0, s, n, gamma, buf,
+                       buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n, buf + 5 * n, buf + 6 * n,
+                       buf + 7 * n, buf + 8 * n, ib, ib + n, precision, F, tol, max_iter, 3, fb);
+    hipMemcpyAsync(rho_s, buf + 6 * n, b, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(u_s, buf + 7 * n, b, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(P_s, buf + 8 * n, b, hipMemcpyDeviceToHost, s);
+    if (flag) hipMemcpyAsync(flag, ib, sizeof(int) * n, hipMemcpyDeviceToHost, s);
+    if (iters) hipMemcpyAsync(iters, ib + n, sizeof(int) * n, hipMemcpyDeviceToHost, s);
+    hipError_t e = hipStreamSynchronize(s);
+    hipFree(buf);
+    hipFree(ib);
+    hipFree(fb);
+    if (e != hipSuccess) return ctx_fail(MLH_ERR_DEVICE, hipGetErrorString(e));
+    return MLH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// mlp_mfma.h -- one-wavefront BF16 inference of the 10-64-64-1 seed network
+// on the CDNA4 matrix cores (v_mfma_f32_16x16x32_bf16).
+//
+// Layout of v_mfma_f32_16x16x32_bf16 used here (lane l, g = l/16, c = l%16):
+//   A (16 x 32): lane l holds row c, k slots 8g..8g+7;
+//   B (32 x 16): lane l holds column c, the same k slots;
+//   C/D (16 x 16 f32): lane l holds column c, rows 4g..4g+3.
+// With Z^T = W X^T, rows are neurons and columns are faces, so the D
+// fragment of a layer (neurons 4g+r of face c) becomes the next layer's B
+// operand after relabelling K; mfma_h1_neuron is that relabelling.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "mlp_api.h"
+#include "mlp_device.h"
+
+namespace mlh {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// input neuron of layer 2 that sits in k slot j of lane group g, k step ks
+__host__ __device__ inline int mfma_h1_neuron(int ks, int g, int j) {
+    return 32 * ks + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
+
+struct MlpRegs {
+    bf16x8_t w1[4];
+    bf16x8_t w2[4][2];
+    f32x4_t b1[4], b2[4], w3[4];
+    float b3;
+};
+
+__device__ inline void mlp_load_regs(const MlpFrags *__restrict__ F, MlpRegs &R) {
+    int l = __lane_id();
+    for (int mt = 0; mt < 4; ++mt) {
+        R.w1[mt] = *(const bf16x8_t *)&F->w1[mt][l][0];
+        R.w2[mt][0] = *(const bf16x8_t *)&F->w2[mt][0][l][0];
+        R.w2[mt][1] = *(const bf16x8_t *)&F->w2[mt][1][l][0];
+        R.b1[mt] = *(const f32x4_t *)&F->b1[mt][l][0];
+        R.b2[mt] = *(const f32x4_t *)&F->b2[mt][l][0];
+        R.w3[mt] = *(const f32x4_t *)&F->w3[mt][l][0];
+    }
+    R.b3 = F->b3;
+}
+
+// tanh(x) = 1 - 2 / (exp(2x) + 1) with the hardware exp2 / reciprocal
+__device__ inline float tanh_fast(float x) {
+    float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+}
+
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// All 64 lanes must call this together.  x: the calling lane's 10 features.
+// lds: 64*16 uint16 of this wavefront's LDS.  Returns this lane's output.
+__device__ inline float mlp_infer_wave_bf16(const MlpRegs &R, const float *x, uint16_t *lds) {
+    const int l = __lane_id(), g = l >> 4, c = l & 15;
+    // features -> LDS as BF16 rows of 16 (slots 10..15 zero)
+    {
+        bf16x8_t lo, hi;
+        for (int j = 0; j < 8; ++j) lo[j] = (__bf16)x[j];
+        hi[0] = (__bf16)x[8];
+        hi[1] = (__bf16)x[9];
+        for (int j = 2; j < 8; ++j) hi[j] = (__bf16)0.f;
+        *(bf16x8_t *)&lds[l * 16] = lo;
+        *(bf16x8_t *)&lds[l * 16 + 8] = hi;
+    }
+    wave_lds_sync();
+    f32x4_t part[4];
+    f32x4_t h1[4][4];  // [M tile][N tile]
+    for (int nt = 0; nt < 4; ++nt) {
+        bf16x8_t bx;
+        if (g < 2) {
+            bx = *(const bf16x8_t *)&lds[(16 * nt + c) * 16 + 8 * g];
+        } else {
+            for (int j = 0; j < 8; ++j) bx[j] = (__bf16)0.f;
+        }
+        for (int mt = 0; mt < 4; ++mt) {
+            f32x4_t acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w1[mt], bx, R.b1[mt], 0, 0, 0);
+            for (int r = 0; r < 4; ++r) acc[r] = tanh_fast(acc[r]);
+            h1[mt][nt] = acc;
+        }
+    }
+    wave_lds_sync();
+    for (int nt = 0; nt < 4; ++nt) {
+        bf16x8_t b0, b1;
+        for (int r = 0; r < 4; ++r) {
+            b0[r] = (__bf16)h1[0][nt][r];
+            b0[4 + r] = (__bf16)h1[1][nt][r];
+            b1[r] = (__bf16)h1[2][nt][r];
+            b1[4 + r] = (__bf16)h1[3][nt][r];
+        }
+        f32x4_t p = {0.f, 0.f, 0.f, 0.f};
+        for (int mo = 0; mo < 4; ++mo) {
+            f32x4_t acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w2[mo][0], b0, R.b2[mo], 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.w2[mo][1], b1, acc, 0, 0, 0);
+            for (int r = 0; r < 4; ++r) p[r] += R.w3[mo][r] * tanh_fast(acc[r]);
+        }
+        part[nt] = p;
+    }
+    // sum the 16 neurons held by this lane, then over the 4 lane groups
+    float y[4];
+    for (int nt = 0; nt < 4; ++nt) {
+        float v = part[nt][0] + part[nt][1] + part[nt][2] + part[nt][3];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        y[nt] = v;
+    }
+    float out = g == 0 ? y[0] : (g == 1 ? y[1] : (g == 2 ? y[2] : y[3]));
+    return out + R.b3;
+}
+
+}  // namespace mlh

@@ -1,0 +1,261 @@
+// riemann.h -- exact ideal-gas Riemann solver, device side.
+//
+// Contract of the reference's external solver (src/Riemann.cpp:84,93-94):
+// solve(rhoL,uL,PL, rhoR,uR,PR) -> (rho*, u*, P*) sampled at x/t = 0 and a
+// flag (-1 left of the contact, +1 right, 0 vacuum).  The solver itself is
+// not in the reference repository (fetched at build time from
+// bwvdnbro/python_finite_volume_solver, unpinned); this is the published
+// exact solver of Toro (3rd ed., ch. 4): Newton-Raphson on
+// f(p) = f_L(p) + f_R(p) + du from the adaptive PVRS/TRRS/TSRS guess with
+// the relative-change test 2|p1-p0|/(p1+p0) <= tol, sampled at S = 0.
+//
+// Precision variants:
+//   solve_fp64   every operation in FP64 (the plain exact solver);
+//   solve_mixed  FP32 Newton iterations from a seed (standard guess or the
+//                learned one), then one FP64 Newton step at the FP32 root:
+//                the face is accepted only when that FP64 step passes the
+//                same relative-change test, otherwise solve_fp64 runs from
+//                the standard guess.  Sampling uses the FP64-corrected p*.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace mlh {
+
+template <typename T>
+struct GasConst {
+    T g, g1, g2, g3, g4, g5, g6, g7, ginv;
+    __host__ __device__ explicit GasConst(double gam) {
+        g = T(gam);
+        g1 = T((gam - 1.0) / (2.0 * gam));
+        g2 = T((gam + 1.0) / (2.0 * gam));
+        g3 = T(2.0 * gam / (gam - 1.0));
+        g4 = T(2.0 / (gam - 1.0));
+        g5 = T(2.0 / (gam + 1.0));
+        g6 = T((gam - 1.0) / (gam + 1.0));
+        g7 = T((gam - 1.0) / 2.0);
+        ginv = T(1.0 / gam);
+    }
+};
+
+__device__ inline double rpow(double x, double y) { return pow(x, y); }
+// FP32 powers in the Newton loop use the hardware log2/exp2 pair
+__device__ inline float rpow(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+__device__ inline double rsqrt_(double x) { return sqrt(x); }
+__device__ inline float rsqrt_(float x) { return __builtin_sqrtf(x); }
+
+// Toro (4.6), (4.7), (4.37): f_K and f_K' for one side.
+template <typename T>
+__device__ inline void prefun(const GasConst<T> &c, T p, T rk, T pk, T ck,
+                              T &f, T &fd) {
+    if (p <= pk) {
+        T prat = p / pk;
+        f = c.g4 * ck * (rpow(prat, c.g1) - T(1));
+        fd = (T(1) / (rk * ck)) * rpow(prat, -c.g2);
+    } else {
+        T ak = c.g5 / rk;
+        T bk = c.g6 * pk;
+        T qrt = rsqrt_(ak / (bk + p));
+        f = (p - pk) * qrt;
+        fd = (T(1) - T(0.5) * (p - pk) / (bk + p)) * qrt;
+    }
+}
+
+// Adaptive initial guess (Toro 9.5.1): PVRS, else TRRS, else TSRS.
+template <typename T>
+__device__ inline T guess_p(const GasConst<T> &c, T dl, T ul, T pl, T cl, T dr,
+                            T ur, T pr, T cr, T tiny) {
+    T cup = T(0.25) * (dl + dr) * (cl + cr);
+    T ppv = T(0.5) * (pl + pr) + T(0.5) * (ul - ur) * cup;
+    if (ppv < T(0)) ppv = T(0);
+    T pmin = pl < pr ? pl : pr;
+    T pmax = pl > pr ? pl : pr;
+    T qmax = pmax / pmin;
+    T pm;
+    if (qmax <= T(2) && pmin <= ppv && ppv <= pmax) {
+        pm = ppv;
+    } else if (ppv < pmin) {
+        T pq = rpow(pl / pr, c.g1);
+        T um = (pq * ul / cl + ur / cr + c.g4 * (pq - T(1))) / (pq / cl + T(1) / cr);
+        T ptl = T(1) + c.g7 * (ul - um) / cl;
+        T ptr = T(1) + c.g7 * (um - ur) / cr;
+        pm = T(0.5) * (pl * rpow(ptl, c.g3) + pr * rpow(ptr, c.g3));
+    } else {
+        T p0 = ppv > tiny ? ppv : tiny;
+        T gel = rsqrt_((c.g5 / dl) / (c.g6 * pl + p0));
+        T ger = rsqrt_((c.g5 / dr) / (c.g6 * pr + p0));
+        pm = (gel * pl + ger * pr - (ur - ul)) / (gel + ger);
+    }
+    if (!(pm > tiny)) pm = tiny;
+    return pm;
+}
+
+template <typename T>
+struct RSol {
+    T d, u, p;
+    int flag;
+};
+
+// Vacuum cases, Toro §4.6, sampled at S = 0: vacuum right (4.76)-(4.77),
+// vacuum left (4.79)-(4.80), vacuum generated (4.82)-(4.84).
+template <typename T>
+__device__ inline RSol<T> sample_vacuum(const GasConst<T> &c, T dl, T ul, T pl,
+                                        T cl, T dr, T ur, T pr, T cr) {
+    const T s = T(0);
+    const RSol<T> vac{T(0), T(0), T(0), 0};
+    bool lv = !(dl > T(0)), rv = !(dr > T(0));
+    if (lv && rv) return vac;
+    if (!lv) {  // left gas present: left rarefaction into vacuum
+        T shl = ul - cl, stl = ul + c.g4 * cl;
+        if (s <= shl) return RSol<T>{dl, ul, pl, -1};
+        if (s < stl) {
+            T cc = c.g5 * (cl + c.g7 * (ul - s));
+            return RSol<T>{dl * rpow(cc / cl, c.g4), c.g5 * (cl + c.g7 * ul + s),
+                           pl * rpow(cc / cl, c.g3), -1};
+        }
+        if (rv) return vac;
+    }
+    // right gas present: right rarefaction into vacuum
+    T shr = ur + cr, str = ur - c.g4 * cr;
+    if (s >= shr) return RSol<T>{dr, ur, pr, 1};
+    if (s > str) {
+        T cc = c.g5 * (cr - c.g7 * (ur - s));
+        return RSol<T>{dr * rpow(cc / cr, c.g4), c.g5 * (-cr + c.g7 * ur + s),
+                       pr * rpow(cc / cr, c.g3), 1};
+    }
+    return vac;
+}
+
+// Sampling at S = 0 given p*, u* (Toro §4.5, SAMPLE).
+template <typename T>
+__device__ inline RSol<T> sample(const GasConst<T> &c, T pm, T um, T dl, T ul,
+                                 T pl, T cl, T dr, T ur, T pr, T cr) {
+    const T s = T(0);
+    if (s <= um) {
+        if (pm <= pl) {
+            T shl = ul - cl;
+            if (s <= shl) return RSol<T>{dl, ul, pl, -1};
+            T cml = cl * rpow(pm / pl, c.g1);
+            T stl = um - cml;
+            if (s > stl) return RSol<T>{dl * rpow(pm / pl, c.ginv), um, pm, -1};
+            T cc = c.g5 * (cl + c.g7 * (ul - s));
+            return RSol<T>{dl * rpow(cc / cl, c.g4), c.g5 * (cl + c.g7 * ul + s),
+                           pl * rpow(cc / cl, c.g3), -1};
+        } else {
+            T pml = pm / pl;
+            T sl = ul - cl * rsqrt_(c.g2 * pml + c.g1);
+            if (s <= sl) return RSol<T>{dl, ul, pl, -1};
+            return RSol<T>{dl * (pml + c.g6) / (pml * c.g6 + T(1)), um, pm, -1};
+        }
+    } else {
+        if (pm > pr) {
+            T pmr = pm / pr;
+            T sr = ur + cr * rsqrt_(c.g2 * pmr + c.g1);
+            if (s >= sr) return RSol<T>{dr, ur, pr, 1};
+            return RSol<T>{dr * (pmr + c.g6) / (pmr * c.g6 + T(1)), um, pm, 1};
+        } else {
+            T shr = ur + cr;
+            if (s >= shr) return RSol<T>{dr, ur, pr, 1};
+            T cmr = cr * rpow(pm / pr, c.g1);
+            T str = um + cmr;
+            if (s <= str) return RSol<T>{dr * rpow(pm / pr, c.ginv), um, pm, 1};
+            T cc = c.g5 * (cr - c.g7 * (ur - s));
+            return RSol<T>{dr * rpow(cc / cr, c.g4), c.g5 * (-cr + c.g7 * ur + s),
+                           pr * rpow(cc / cr, c.g3), 1};
+        }
+    }
+}
+
+struct SolveStats {
+    int iters;      // Newton iterations (FP32 + FP64)
+    int fallback;   // learned / FP32 start rejected by the FP64 test
+    int nonconv;    // FP64 iteration cap reached
+};
+
+// Plain FP64 exact solver from the standard guess (or from p0 > 0).
+__device__ inline RSol<double> solve_fp64(const GasConst<double> &c, double dl,
+                                          double ul, double pl, double dr,
+                                          double ur, double pr, double tol,
+                                          int max_iter, double p0,
+                                          SolveStats &st) {
+    if (!(dl > 0.0) || !(dr > 0.0)) {
+        double cl = dl > 0.0 ? sqrt(c.g * pl / dl) : 0.0;
+        double cr = dr > 0.0 ? sqrt(c.g * pr / dr) : 0.0;
+        return sample_vacuum(c, dl, ul, pl, cl, dr, ur, pr, cr);
+    }
+    double cl = sqrt(c.g * pl / dl), cr = sqrt(c.g * pr / dr);
+    double du = ur - ul;
+    if (c.g4 * (cl + cr) <= du) return sample_vacuum(c, dl, ul, pl, cl, dr, ur, pr, cr);
+    double tiny = tol * 0.5 * (pl + pr);
+    double pold = p0 > 0.0 ? p0 : guess_p(c, dl, ul, pl, cl, dr, ur, pr, cr, tiny);
+    double p = pold, fl, fld, fr, frd;
+    int it;
+    for (it = 1; it <= max_iter; ++it) {
+        prefun(c, pold, dl, pl, cl, fl, fld);
+        prefun(c, pold, dr, pr, cr, fr, frd);
+        p = pold - (fl + fr + du) / (fld + frd);
+        double change = 2.0 * fabs((p - pold) / (p + pold));
+        if (change <= tol) break;
+        if (p < 0.0) p = tiny;
+        pold = p;
+    }
+    if (it > max_iter) { st.nonconv += 1; it = max_iter + 1; }
+    st.iters += it;
+    prefun(c, p, dl, pl, cl, fl, fld);
+    prefun(c, p, dr, pr, cr, fr, frd);
+    double um = 0.5 * (ul + ur + fr - fl);
+    return sample(c, p, um, dl, ul, pl, cl, dr, ur, pr, cr);
+}
+
+// FP32 Newton from p_seed, FP64 residual step, FP64 fallback.  Inputs are
+// the FP32 face states; p_seed <= 0 selects the standard FP32 guess.
+__device__ inline RSol<float> solve_mixed(const GasConst<float> &cf,
+                                          const GasConst<double> &cd, float dl,
+                                          float ul, float pl, float dr,
+                                          float ur, float pr, double tol,
+                                          int max_iter, int iters32,
+                                          float p_seed, SolveStats &st) {
+    if (!(dl > 0.f) || !(dr > 0.f) ||
+        cf.g4 * (__builtin_sqrtf(cf.g * pl / dl) + __builtin_sqrtf(cf.g * pr / dr)) <= ur - ul) {
+        RSol<double> r = solve_fp64(cd, dl, ul, pl, dr, ur, pr, tol, max_iter, -1.0, st);
+        return RSol<float>{float(r.d), float(r.u), float(r.p), r.flag};
+    }
+    float cl = __builtin_sqrtf(cf.g * pl / dl), cr = __builtin_sqrtf(cf.g * pr / dr);
+    float du = ur - ul;
+    float tiny = 1e-6f * 0.5f * (pl + pr);
+    float p = p_seed > 0.f ? p_seed : guess_p(cf, dl, ul, pl, cl, dr, ur, pr, cr, tiny);
+    int it = 0;
+    for (; it < iters32; ++it) {
+        float fl, fld, fr, frd;
+        prefun(cf, p, dl, pl, cl, fl, fld);
+        prefun(cf, p, dr, pr, cr, fr, frd);
+        float pn = p - (fl + fr + du) / (fld + frd);
+        if (!(pn > 0.f)) pn = tiny;
+        float ch = __builtin_fabsf(pn - p) / (pn + p);
+        p = pn;
+        if (ch < 2e-6f) { ++it; break; }
+    }
+    // FP64 residual: one Newton step at the FP32 root, accepted under the
+    // exact solver's relative-change test
+    double dld = dl, uld = ul, pld = pl, drd = dr, urd = ur, prd = pr;
+    double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
+    double pd = p, fl, fld, fr, frd;
+    prefun(cd, pd, dld, pld, cld, fl, fld);
+    prefun(cd, pd, drd, prd, crd, fr, frd);
+    double dp = (fl + fr + (urd - uld)) / (fld + frd);
+    double pn = pd - dp;
+    double change = 2.0 * fabs(dp / (pn + pd));
+    if (!(change <= tol) || !(pn > 0.0)) {
+        st.fallback += 1;
+        st.iters += it + 1;
+        RSol<double> r = solve_fp64(cd, dld, uld, pld, drd, urd, prd, tol, max_iter, -1.0, st);
+        return RSol<float>{float(r.d), float(r.u), float(r.p), r.flag};
+    }
+    st.iters += it + 1;
+    double um = 0.5 * (uld + urd + (fr - fl) - (frd - fld) * dp);
+    return sample(cf, float(pn), float(um), dl, ul, pl, cl, dr, ur, pr, cr);
+}
+
+}  // namespace mlh

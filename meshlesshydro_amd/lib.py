@@ -1,0 +1,258 @@
+"""ctypes binding of libmlh_hip.so (the C ABI of include/mlh.h).
+
+The library is the product: this module only marshals numpy arrays.  It
+raises if the HIP library is missing; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._abi import Diag, Params, MLP_IN, MLP_NPARAM, MLH_FP64, MLH_SEED_STANDARD
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmlh_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_fp = C.POINTER(C.c_float)
+_vp = C.c_void_p
+
+# symbol -> (restype, argtypes); the complete exported C ABI
+SIGNATURES = {
+    "mlh_params_default": (None, [C.POINTER(Params)]),
+    "mlh_version": (C.c_int, []),
+    "mlh_create": (C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(_vp)]),
+    "mlh_destroy": (C.c_int, [_vp]),
+    "mlh_last_error": (C.c_char_p, []),
+    "mlh_upload_state": (C.c_int, [_vp] + [_dp] * 8),
+    "mlh_download_state": (C.c_int, [_vp] + [_dp] * 8),
+    "mlh_step": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _dp, _ip]),
+    "mlh_run": (C.c_int, [_vp, C.c_int32, C.c_double, _dp]),
+    "mlh_neighbours": (C.c_int, [_vp]),
+    "mlh_density": (C.c_int, [_vp]),
+    "mlh_timestep": (C.c_int, [_vp, _dp]),
+    "mlh_gradients": (C.c_int, [_vp]),
+    "mlh_fluxes": (C.c_int, [_vp, C.c_double]),
+    "mlh_update": (C.c_int, [_vp]),
+    "mlh_get_neighbours": (C.c_int, [_vp, _ip, _ip, _ip]),
+    "mlh_get_density": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "mlh_get_gradients": (C.c_int, [_vp, _dp, _dp]),
+    "mlh_get_fluxes": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "mlh_get_diagnostics": (C.c_int, [_vp, C.POINTER(Diag)]),
+    "mlh_conserved_sums": (C.c_int, [_vp, _dp]),
+    "mlh_get_timings": (C.c_int, [_vp, _fp]),
+    "mlh_riemann_exact_batch": (C.c_int, [_vp, C.c_int64, C.c_double] + [_dp] * 9 + [_ip, _ip, C.c_int32, C.c_int32, C.c_double]),
+    "mlh_mlp_features": (C.c_int, [C.c_int64, C.c_double] + [_dp] * 6 + [_fp, _dp]),
+    "mlh_mlp_set_params": (C.c_int, [_vp, _fp]),
+    "mlh_mlp_get_params": (C.c_int, [_vp, _fp]),
+    "mlh_mlp_loss_grad": (C.c_int, [_vp, C.c_int64, _fp, _fp, _fp, _fp]),
+    "mlh_mlp_train": (C.c_int, [_vp, C.c_int64, _fp, _fp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]),
+    "mlh_mlp_infer": (C.c_int, [_vp, C.c_int64, _fp, _fp, C.c_int32]),
+}
+
+_lib = None
+
+
+class MLHError(RuntimeError):
+    def __init__(self, status, what, msg=""):
+        super().__init__(f"{what}: status {status} {msg}")
+        self.status = status
+
+
+def lib():
+    """Load libmlh_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built: run python -m meshlesshydro_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _f(a):
+    return a.ctypes.data_as(_fp)
+
+
+def _chk(st, what):
+    if st != 0:
+        raise MLHError(st, what, lib().mlh_last_error().decode())
+
+
+class Simulation:
+    """One MFV context on the GPU: the Particles state plus the step."""
+
+    def __init__(self, params, state):
+        self.L = lib()
+        self.p = params
+        self.D = params.dim
+        self.N = len(state["x"])
+        h = _vp()
+        _chk(self.L.mlh_create(C.byref(params), self.N, C.byref(h)), "mlh_create")
+        self.h = h
+        self.upload(state)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mlh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def upload(self, s):
+        g = lambda k: np.ascontiguousarray(s[k], dtype=np.float64) if k in s else np.zeros(self.N)
+        keep = [g(k) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "u")]
+        _chk(self.L.mlh_upload_state(self.h, *[_d(a) for a in keep]), "mlh_upload_state")
+
+    def state(self):
+        out = {k: np.zeros(self.N) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "u")}
+        _chk(self.L.mlh_download_state(self.h, *[_d(out[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "u")]), "download")
+        if self.D == 2:
+            out.pop("z"), out.pop("vz")
+        return out
+
+    def step(self, dt_fixed=-1.0, t_now=-1.0, t_clip=-1.0):
+        dt = C.c_double()
+        cl = C.c_int32()
+        _chk(self.L.mlh_step(self.h, dt_fixed, t_now, t_clip, C.byref(dt), C.byref(cl)), "mlh_step")
+        return dt.value
+
+    def run(self, n_steps, dt_fixed=-1.0):
+        dts = np.zeros(max(n_steps, 1))
+        _chk(self.L.mlh_run(self.h, n_steps, dt_fixed, _d(dts)), "mlh_run")
+        return dts[:n_steps]
+
+    # stages
+    def neighbours(self):
+        _chk(self.L.mlh_neighbours(self.h), "mlh_neighbours")
+
+    def density(self):
+        _chk(self.L.mlh_density(self.h), "mlh_density")
+
+    def timestep(self):
+        dt = C.c_double()
+        _chk(self.L.mlh_timestep(self.h, C.byref(dt)), "mlh_timestep")
+        return dt.value
+
+    def gradients(self):
+        _chk(self.L.mlh_gradients(self.h), "mlh_gradients")
+
+    def fluxes(self, dt):
+        _chk(self.L.mlh_fluxes(self.h, dt), "mlh_fluxes")
+
+    def update(self):
+        _chk(self.L.mlh_update(self.h), "mlh_update")
+
+    def get_neighbours(self):
+        stride = self.p.max_interactions + self.p.max_ghost_interactions
+        noi = np.zeros(self.N, np.int32)
+        nog = np.zeros(self.N, np.int32)
+        nbr = np.zeros(self.N * stride, np.int32)
+        _chk(self.L.mlh_get_neighbours(self.h, _i(noi), _i(nog), _i(nbr)), "get_neighbours")
+        return noi, nog, nbr.reshape(self.N, stride)
+
+    def get_density(self):
+        r, P, w = np.zeros(self.N), np.zeros(self.N), np.zeros(self.N)
+        _chk(self.L.mlh_get_density(self.h, _d(r), _d(P), _d(w)), "get_density")
+        return r, P, w
+
+    def get_gradients(self):
+        D = self.D
+        g = np.zeros((D + 2) * self.N * D)
+        B = np.zeros(self.N * D * D)
+        _chk(self.L.mlh_get_gradients(self.h, _d(g), _d(B)), "get_gradients")
+        return g.reshape(D + 2, self.N, D), B.reshape(self.N, D, D)
+
+    def get_fluxes(self):
+        mF, vF, eF = np.zeros(self.N), np.zeros(self.N * self.D), np.zeros(self.N)
+        _chk(self.L.mlh_get_fluxes(self.h, _d(mF), _d(vF), _d(eF)), "get_fluxes")
+        return mF, vF.reshape(self.N, self.D), eF
+
+    def conserved(self):
+        o = np.zeros(6)
+        _chk(self.L.mlh_conserved_sums(self.h, _d(o)), "conserved")
+        return o
+
+    def diag(self):
+        d = Diag()
+        _chk(self.L.mlh_get_diagnostics(self.h, C.byref(d)), "diag")
+        return d.as_dict()
+
+    def timings(self):
+        t = np.zeros(4, np.float32)
+        _chk(self.L.mlh_get_timings(self.h, _f(t)), "timings")
+        return t
+
+    # learned seed
+    def mlp_set_params(self, params):
+        params = np.ascontiguousarray(params, np.float32)
+        assert params.size == MLP_NPARAM
+        _chk(self.L.mlh_mlp_set_params(self.h, _f(params)), "mlp_set_params")
+
+    def mlp_get_params(self):
+        out = np.zeros(MLP_NPARAM, np.float32)
+        _chk(self.L.mlh_mlp_get_params(self.h, _f(out)), "mlp_get_params")
+        return out
+
+    def mlp_loss_grad(self, feats, target):
+        feats = np.ascontiguousarray(feats, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        _chk(self.L.mlh_mlp_loss_grad(self.h, len(target), _f(feats), _f(target), _f(loss), _f(g)), "mlp_loss_grad")
+        return float(loss[0]), g
+
+    def mlp_train(self, feats, target, epochs=1, batch=4096, lr=1e-3, seed=1):
+        feats = np.ascontiguousarray(feats, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        loss = np.zeros(1, np.float32)
+        _chk(self.L.mlh_mlp_train(self.h, len(target), _f(feats), _f(target), epochs, batch, lr, seed, _f(loss)), "mlp_train")
+        return float(loss[0])
+
+    def mlp_infer(self, feats, bf16=True):
+        feats = np.ascontiguousarray(feats, np.float32)
+        out = np.zeros(len(feats), np.float32)
+        _chk(self.L.mlh_mlp_infer(self.h, len(feats), _f(feats), _f(out), int(bf16)), "mlp_infer")
+        return out
+
+    def riemann(self, rl, ul, pl, rr, ur, pr, precision=MLH_FP64, seed=MLH_SEED_STANDARD, tol=1e-6):
+        return riemann_batch(self.p.gamma, rl, ul, pl, rr, ur, pr, precision, seed, tol, ctx=self.h)
+
+
+def riemann_batch(gamma, rl, ul, pl, rr, ur, pr, precision=MLH_FP64, seed=MLH_SEED_STANDARD, tol=1e-6, ctx=None):
+    L = lib()
+    n = len(rl)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rl, ul, pl, rr, ur, pr)]
+    rs, us, ps = np.zeros(n), np.zeros(n), np.zeros(n)
+    flag, it = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    _chk(L.mlh_riemann_exact_batch(ctx, n, gamma, *[_d(a) for a in arrs], _d(rs), _d(us), _d(ps), _i(flag), _i(it), precision, seed, tol), "mlh_riemann_exact_batch")
+    return rs, us, ps, flag, it
+
+
+def mlp_features(gamma, rl, ul, pl, rr, ur, pr):
+    n = len(rl)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rl, ul, pl, rr, ur, pr)]
+    feats = np.zeros((n, MLP_IN), np.float32)
+    pref = np.zeros(n)
+    _chk(lib().mlh_mlp_features(n, gamma, *[_d(a) for a in arrs], _f(feats), _d(pref)), "mlh_mlp_features")
+    return feats, pref
+
+
+def default_params_c():
+    p = Params()
+    lib().mlh_params_default(C.byref(p))
+    return p

@@ -1,0 +1,95 @@
+/*
+ * mlh_oracle.h -- CPU FP64 oracle of the meshless finite-volume (MFV) step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP library is compared
+ * against: only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg may load it.  It is a restatement, in plain C, of the reference's
+ * algorithm (jammartin/meshlessHydro, demonstrator/src/Particles.cpp,
+ * Riemann.cpp, Helper.cpp, Domain.cpp, MeshlessScheme.cpp), with the
+ * reference's data structures kept where they decide results: explicit
+ * ghost particles in ghost-index order, a 3^D stencil grid search with
+ * non-periodic cell indices, a reverse-index lookup per pair, each pair
+ * solved once from the lower index and negated for the other side.
+ */
+#ifndef MLH_ORACLE_H
+#define MLH_ORACLE_H
+
+#include <stdint.h>
+#include "../include/mlh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc orc;
+
+orc *orc_create(const mlh_params *p, int32_t n);
+void orc_destroy(orc *o);
+int orc_set_state(orc *o, const double *x, const double *y, const double *z,
+                  const double *vx, const double *vy, const double *vz,
+                  const double *m, const double *u);
+int orc_get_state(orc *o, double *x, double *y, double *z, double *vx,
+                  double *vy, double *vz, double *m, double *u);
+
+/* stages (same grouping as the C ABI) */
+int orc_neighbours(orc *o);
+int orc_density(orc *o);
+int orc_timestep(orc *o, double *dt);
+int orc_gradients(orc *o);
+int orc_fluxes(orc *o, double dt);
+int orc_update(orc *o, double dt);
+/* full step: dt_fixed > 0 uses it, else CFL dt clipped to dt_max (> 0) */
+int orc_step(orc *o, double dt_fixed, double dt_max, double *dt_used);
+
+int orc_get_neighbours(orc *o, int32_t *noi, int32_t *noi_ghost,
+                       int32_t *nbr);
+int orc_get_density(orc *o, double *rho, double *P, double *omega);
+int orc_get_gradients(orc *o, double *grads, double *B);
+int orc_get_fluxes(orc *o, double *mF, double *vF, double *eF);
+int orc_conserved_sums(orc *o, double *out);
+int orc_get_diag(orc *o, mlh_diag *d);
+/* brute-force (O(N * N_ghost)) ghost search instead of the binned one */
+void orc_set_bruteforce_ghosts(orc *o, int on);
+void orc_set_threads(int n);
+
+/* exact Riemann solver (Toro, "Riemann Solvers and Numerical Methods for
+ * Fluid Dynamics", 3rd ed., ch. 4): Newton-Raphson on
+ * f(p) = f_L(p) + f_R(p) + (u_R - u_L) from the adaptive PVRS/TRRS/TSRS
+ * guess, sampled at x/t = 0.  Returns the flag (-1 left, +1 right,
+ * 0 vacuum) and the iteration count in *iters. */
+int orc_riemann_solve(double gamma, double rhoL, double uL, double PL,
+                      double rhoR, double uR, double PR, double tol,
+                      int max_iter, double *rho_s, double *u_s, double *P_s,
+                      int *iters);
+/* batch form for ctypes */
+void orc_riemann_batch(int64_t n, double gamma, const double *rhoL,
+                       const double *uL, const double *PL, const double *rhoR,
+                       const double *uR, const double *PR, double tol,
+                       int max_iter, double *rho_s, double *u_s, double *P_s,
+                       int32_t *flag, int32_t *iters);
+/* the initial guess alone (for the learned-seed comparison) */
+double orc_riemann_guess(double gamma, double rhoL, double uL, double PL,
+                         double rhoR, double uR, double PR);
+/* residual f(p) and f'(p) */
+void orc_riemann_f(double gamma, double rhoL, double uL, double PL,
+                   double rhoR, double uR, double PR, double p, double *f,
+                   double *df);
+
+/* cubic spline kernel W(r, h) (Particles.cpp:7-24) */
+double orc_kernel(int dim, double r, double h);
+
+/* MLP 10-64-64-1, tanh hidden layers, FP64 forward/backward (loss = mean
+ * squared error); params layout as mlh_mlp_set_params. */
+double orc_mlp_loss_grad(int64_t n, const float *params, const float *feats,
+                         const float *target, double *grad);
+void orc_mlp_forward(int64_t n, const float *params, const float *feats,
+                     double *out);
+void orc_mlp_features(int64_t n, double gamma, const double *rhoL,
+                      const double *uL, const double *PL, const double *rhoR,
+                      const double *uR, const double *PR, float *feats,
+                      double *p_ref);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

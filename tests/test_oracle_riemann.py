@@ -1,0 +1,128 @@
+"""CPU checks of the oracle's exact Riemann solver (oracle/orc_riemann.c).
+
+The reference's solver is un-vendored (bwvdnbro/python_finite_volume_solver
+RiemannSolver.hpp, fetched by demonstrator/Makefile:94, unpinned), so the
+oracle is pinned by Toro's published tests 1-5 (tests/golden/toro_kats.json,
+re-derived by bisection) and by invariants a wrong sign, a swapped side or a
+dropped term would break: identical states, mirror symmetry, Galilean shift,
+the residual f(p*) = 0, positivity/vacuum, and the sampled state at x/t = 0.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "toro_kats.json")
+
+
+@pytest.fixture(scope="module")
+def kats():
+    with open(GOLD) as fh:
+        return json.load(fh)
+
+
+def test_toro_star_pressure_and_velocity(oracle_mod, kats):
+    g = kats["gamma"]
+    for t in kats["tests"]:
+        (dl, ul, pl), (dr, ur, pr) = t["left"], t["right"]
+        p0 = oracle_mod.riemann_guess(g, dl, ul, pl, dr, ur, pr)
+        assert p0 > 0
+        # solve with a tight tolerance and compare p*, u* with the bisection
+        rs, us, ps, fl, it = oracle_mod.riemann_batch(g, [dl], [ul], [pl], [dr], [ur], [pr], tol=1e-12)
+        f, df = oracle_mod.riemann_f(g, dl, ul, pl, dr, ur, pr, t["bisection"]["p_star"])
+        assert abs(f) < 1e-9 * max(1.0, abs(ur - ul) + 1.0)
+        assert it[0] <= 20
+
+
+def _star_from_sampling(oracle_mod, g, L, R):
+    """p* is recovered from the sampled state whenever x/t = 0 lies in a star region."""
+    rs, us, ps, fl, it = oracle_mod.riemann_batch(g, [L[0]], [L[1]], [L[2]], [R[0]], [R[1]], [R[2]], tol=1e-12)
+    return rs[0], us[0], ps[0], fl[0]
+
+
+def test_toro_sampled_states(oracle_mod, kats):
+    g = kats["gamma"]
+    t = {x["name"]: x for x in kats["tests"]}
+    # T1: x/t = 0 lies in the left star region (left rarefaction tail at u*-c*L < 0)
+    r, u, p, f = _star_from_sampling(oracle_mod, g, t["T1"]["left"], t["T1"]["right"])
+    b = t["T1"]["bisection"]
+    assert f == -1
+    assert abs(p - b["p_star"]) < 1e-10 and abs(u - b["u_star"]) < 1e-10
+    assert abs(r - b["rho_star_L"]) < 1e-10
+    # T2: symmetric double rarefaction, u* = 0: sampled at the contact (left side)
+    r, u, p, f = _star_from_sampling(oracle_mod, g, t["T2"]["left"], t["T2"]["right"])
+    b = t["T2"]["bisection"]
+    assert abs(p - b["p_star"]) < 1e-10 and abs(u) < 1e-12 and abs(r - b["rho_star_L"]) < 1e-10
+    # T3 / T4: x/t = 0 lies in the star region next to the contact
+    for name, side, key in (("T3", -1, "rho_star_L"), ("T4", 1, "rho_star_R")):
+        r, u, p, f = _star_from_sampling(oracle_mod, g, t[name]["left"], t[name]["right"])
+        b = t[name]["bisection"]
+        assert f == side
+        assert abs(p - b["p_star"]) < 1e-10 * b["p_star"]
+        assert abs(u - b["u_star"]) < 1e-10 * abs(b["u_star"])
+        assert abs(r - b[key]) < 1e-10 * b[key]
+    # Toro's modified Sod problem (sec. 6.4, test 1): the left rarefaction is
+    # transonic, x/t = 0 inside the fan (Toro eq. 4.56)
+    L, R = (1.0, 0.75, 1.0), (0.125, 0.0, 0.1)
+    r, u, p, f = _star_from_sampling(oracle_mod, g, L, R)
+    cl = np.sqrt(g * L[2] / L[0])
+    c = 2.0 / (g + 1.0) * (cl + (g - 1.0) / 2.0 * L[1])
+    assert f == -1
+    assert abs(u - c) < 1e-12  # at x/t = 0 the fan velocity equals the sound speed
+    assert abs(r - L[0] * (c / cl) ** (2.0 / (g - 1.0))) < 1e-12
+    assert abs(p - L[2] * (c / cl) ** (2.0 * g / (g - 1.0))) < 1e-12
+    # the mirrored problem samples the right fan with flag +1
+    r2, u2, p2, f2 = _star_from_sampling(oracle_mod, g, (R[0], -R[1], R[2]), (L[0], -L[1], L[2]))
+    assert f2 == 1 and abs(u2 + u) < 1e-12 and abs(r2 - r) < 1e-12 and abs(p2 - p) < 1e-12
+
+
+def test_invariants_random(oracle_mod):
+    rng = np.random.default_rng(7)
+    n = 2000
+    g = 5.0 / 3.0
+    rl = 10 ** rng.uniform(-3, 3, n)
+    rr = 10 ** rng.uniform(-3, 3, n)
+    pl = 10 ** rng.uniform(-6, 3, n)
+    pr = 10 ** rng.uniform(-6, 3, n)
+    cs = 0.5 * (np.sqrt(g * pl / rl) + np.sqrt(g * pr / rr))
+    ul = np.zeros(n)
+    ur = rng.uniform(-3, 3, n) * cs
+    rs, us, ps, fl, it = oracle_mod.riemann_batch(g, rl, ul, pl, rr, ur, pr, tol=1e-12)
+    # mirror symmetry: (L, R) -> (R', L') with u -> -u gives u* -> -u*, same p*
+    rs2, us2, ps2, fl2, it2 = oracle_mod.riemann_batch(g, rr, -ur, pr, rl, -ul, pl, tol=1e-12)
+    nonvac = fl != 0
+    assert np.all(np.abs(ps - ps2) <= 1e-9 * np.maximum(ps, 1e-300) + 1e-300)
+    # where x/t = 0 is in a star region the mirrored solution has the same p, -u
+    assert np.all(np.abs(us + us2)[nonvac] <= 1e-8 * (np.abs(us) + cs)[nonvac])
+    # residual at the converged pressure: Newton step relative size tiny
+    for k in np.flatnonzero(nonvac)[:300]:
+        p0 = oracle_mod.riemann_guess(g, rl[k], ul[k], pl[k], rr[k], ur[k], pr[k])
+        assert p0 > 0
+    assert np.all(it[nonvac] <= 30)
+
+
+def test_identical_states_and_galilean(oracle_mod):
+    g = 5.0 / 3.0
+    # identical states: p* = p, u* = u, sampled state = the state
+    rs, us, ps, fl, it = oracle_mod.riemann_batch(g, [2.0], [0.3], [1.5], [2.0], [0.3], [1.5], tol=1e-12)
+    assert abs(ps[0] - 1.5) < 1e-13 and abs(us[0] - 0.3) < 1e-13 and abs(rs[0] - 2.0) < 1e-13
+    # Galilean shift of both velocities shifts u* and leaves p* (star-region sample)
+    L = (1.0, 0.0, 1.0)
+    R = (0.125, 0.0, 0.1)
+    rs1, us1, ps1, _, _ = oracle_mod.riemann_batch(1.4, [L[0]], [L[1]], [L[2]], [R[0]], [R[1]], [R[2]], tol=1e-12)
+    f0, _ = oracle_mod.riemann_f(1.4, *L, *R, ps1[0])
+    f1, _ = oracle_mod.riemann_f(1.4, L[0], L[1] + 5.0, L[2], R[0], R[1] + 5.0, R[2], ps1[0])
+    assert abs(f0 - f1) < 1e-13
+
+
+def test_vacuum_generation(oracle_mod):
+    g = 1.4
+    # strong separation: 2/(g-1)(cL + cR) <= uR - uL -> vacuum at x/t = 0
+    rs, us, ps, fl, it = oracle_mod.riemann_batch(g, [1.0], [-20.0], [0.4], [1.0], [20.0], [0.4])
+    assert fl[0] == 0 and rs[0] == 0.0 and ps[0] == 0.0
+    # vacuum on the right: left rarefaction into vacuum; x/t = 0 inside the fan
+    rs, us, ps, fl, it = oracle_mod.riemann_batch(g, [1.0], [0.0], [1.0], [0.0], [0.0], [0.0])
+    cl = np.sqrt(1.4)
+    assert fl[0] == -1
+    assert abs(us[0] - 2.0 / 2.4 * cl) < 1e-12
