@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the MFV time step on one MI355X.
+
+Workload (default): the 3D point-energy blast, 128^3 particles on a
+cell-centred lattice in the periodic unit cube, rho = 1, P = 1e-6, E = 1 on
+the 32 central particles, gamma = 5/3, h = 2.4 dx (56 neighbours).
+
+Metric: MFV particle-steps/s (N x steps / wall time of the step loop, from
+the cell list through the state update; no snapshot I/O), FP32 pair
+arithmetic + learned p* seed against plain FP64, at fixed L1 error:
+  1. FP64 plain runs W + K steps with its own CFL steps (dt recorded);
+  2. the FP32 run replays that dt sequence from the same initial state and
+     its density after W + K steps is compared with the FP64 one
+     (L1_rel(rho) = sum|rho - rho64| / sum|rho64|), so the error measures
+     arithmetic, not time-step drift;
+  3. the timed region is the last K steps of each run, inputs resident in
+     HBM, bracketed by device synchronisation.
+`value` is the FP32 (learned seed when trained, else standard seed) rate.
+
+    python bench.py [--steps K] [--warmup W] [--n 128] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def face_kernel_bytes(D, tsz, k_mean):
+    """Algorithmic HBM bytes per particle of the faces+update kernel: own
+    position (double4), primitive record (8 values), face record (grads +
+    B), FP64 state read (x, v, m, u), neighbour counts and row, next state
+    written.  Neighbour records are gathered through the caches and not
+    counted."""
+    nf = D + 2
+    frn = nf * D + D * D
+    own = 32 + 8 * tsz + frn * tsz + 8 * (2 * D + 2) + 8
+    row = 4 * k_mean
+    write = 8 * (2 * D + 2)
+    return own + row + write
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=128, help="particles per side")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=0, help="side of the CPU sample (default: --n)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-learned", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this MFV step has no multi-GPU path "
+              "(sharding is described in DESIGN.md); use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from meshlesshydro_amd import ics, lib
+    from meshlesshydro_amd import learned as L
+    from meshlesshydro_amd._abi import default_params, MLH_FP32, MLH_FP64, MLH_SEED_LEARNED
+
+    n = args.n
+    N = n ** 3
+    W, K = args.warmup, args.steps
+    state = ics.blast_3d(n)
+    base = dict(dim=3, kernel_size=ics.blast_kernel_size(n), max_interactions=96,
+                max_ghost_interactions=96, debug_level=0)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)
+
+    # 1. FP64 plain, own CFL steps; the warmup steps also record the exact
+    #    solver's faces (every 8th particle) as training data for the seed
+    p64 = default_params(precision=MLH_FP64, **base)
+    sim = lib.Simulation(p64, state)
+    log(f"[bench] FP64 plain: {W} warmup + {K} timed steps, N={N}")
+    if not args.no_learned:
+        sim.set_face_collection(1 << 21, 8)
+    dts_w = sim.run(W)
+    faces = sim.collected_faces() if not args.no_learned else None
+    sim.set_face_collection(0, 0)
+    t0 = time.perf_counter()
+    dts_k = sim.run(K)
+    t64 = time.perf_counter() - t0
+    tim64 = sim.timings()
+    d64 = sim.diag()
+    ref = sim.state()
+    dts = np.concatenate([dts_w, dts_k])
+    rate64 = N * K / t64
+    mlp_params, train_info = None, None
+    if not args.no_learned:
+        log(f"[bench] training the p* seed on {len(faces)} FP64 faces of the warmup steps")
+        mlp_params, train_info = L.train_seed_from_faces(sim, faces, p64.gamma, n_epochs=4, log=log)
+        sub = faces[np.random.default_rng(9).choice(len(faces), min(len(faces), 200_000), replace=False)]
+        train_info["newton_iters_heldout_fp32"] = L.newton_stats(sim, p64.gamma, sub)
+    sim.close()
+
+    # 2. FP32 runs replaying the FP64 dt sequence
+    def fp32_run(seed_params=None):
+        p = default_params(precision=MLH_FP32, **base)
+        if seed_params is not None:
+            p.riemann_seed = MLH_SEED_LEARNED
+        s = lib.Simulation(p, state)
+        if seed_params is not None:
+            s.mlp_set_params(seed_params)
+        s.run(W, dts_in=dts[:W])
+        t0 = time.perf_counter()
+        s.run(K, dts_in=dts[W:])
+        t = time.perf_counter() - t0
+        out = (N * K / t, s.timings(), s.diag(), s.state(), s.conserved())
+        s.close()
+        return out
+
+    log("[bench] FP32 standard seed, replayed dt")
+    rate32, tim32, d32, st32, cons32 = fp32_run()
+    learned = None
+    if mlp_params is not None:
+        log("[bench] FP32 learned seed, replayed dt")
+        learned = fp32_run(mlp_params) + (train_info,)
+
+    # density error: rho = m * omega, omega from the FP64 neighbour sets of
+    # each final state (one FP64 density pass)
+    def final_rho(st):
+        p = default_params(precision=MLH_FP64, **base)
+        s = lib.Simulation(p, st)
+        s.neighbours()
+        s.density()
+        r = s.get_density()[0]
+        s.close()
+        return r
+
+    rho64 = final_rho(ref)
+
+    def rho_err(st):
+        r = final_rho(st)
+        return float(np.sum(np.abs(r - rho64)) / np.sum(np.abs(rho64))), float(np.max(np.abs(r - rho64)) / np.max(np.abs(rho64)))
+
+    l1_32, linf_32 = rho_err(st32)
+    head = dict(rate=rate32, tim=tim32, diag=d32, l1=l1_32, linf=linf_32, seed="standard")
+    if learned is not None:
+        l1_l, linf_l = rho_err(learned[3])
+        head = dict(rate=learned[0], tim=learned[1], diag=learned[2], l1=l1_l, linf=linf_l,
+                    seed="learned", train=learned[5])
+
+    k_mean = (d64["faces_solved"] / max(1, (W + K) * N))
+    tface = float(head["tim"][3]) * 1e-3
+    bytes_face = face_kernel_bytes(3, 4, k_mean) * N
+    achieved = bytes_face / tface / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        cn = args.cpu_n or n
+        cst = ics.blast_3d(cn)
+        cp = default_params(dim=3, kernel_size=ics.blast_kernel_size(cn), max_interactions=96,
+                            max_ghost_interactions=96, debug_level=0)
+        log(f"[bench] CPU oracle sample: 1 step of the {cn}^3 blast, {args.cpu_threads} threads")
+        o = O.Oracle(cp, cst, threads=args.cpu_threads)
+        t0 = time.perf_counter()
+        o.step()
+        tc = time.perf_counter() - t0
+        cpu = {"value": cn ** 3 / tc, "unit": "particle-steps/s", "cores": args.cpu_threads,
+               "kind": "port",
+               "sample": f"1 FP64 step of the {cn}^3 blast (same workload) by oracle/ (C, OpenMP), {tc:.1f} s"}
+        del o
+
+    out = {
+        "metric": "MFV particle-steps/s, FP32+learned p* vs FP64 plain, at fixed L1 error",
+        "value": head["rate"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * N / head["rate"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (FP64 E_i inverse, Newton residual, flux sums, state)",
+        "data": "synthetic",
+        "config": {"workload": f"3D point blast {n}^3 periodic (C4)", "particles": N,
+                   "neighbours_mean": round(k_mean, 2), "seed": head["seed"],
+                   "parallelism": "single GPU"},
+        "fp64_plain": {"value": rate64, "ms_per_step": 1e3 * t64 / K,
+                       "kernel_ms": [round(float(x), 4) for x in tim64]},
+        "speedup_vs_fp64": head["rate"] / rate64,
+        "fp32_standard": {"value": rate32, "L1_rel_rho": l1_32, "Linf_rel_rho": linf_32,
+                          "kernel_ms": [round(float(x), 4) for x in tim32],
+                          "newton_iters_per_face": d32["newton_iterations"] / max(1, d32["faces_solved"])},
+        "L1_rel_rho": head["l1"],
+        "Linf_rel_rho": head["linf"],
+        "kernel_ms": [round(float(x), 4) for x in head["tim"]],
+        "newton_iters_per_face": head["diag"]["newton_iterations"] / max(1, head["diag"]["faces_solved"]),
+        "fallback_rate": head["diag"]["learned_fallbacks"] / max(1, head["diag"]["faces_solved"]),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_faces (faces + Riemann + flux + update)"},
+        "cpu_baseline": cpu,
+    }
+    if "train" in head:
+        out["learned_seed"] = head["train"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -140,9 +140,11 @@ int mlh_step(mlh_ctx *ctx, double dt_fixed, double t_now, double t_clip,
              double *dt_used, int32_t *clipped);
 
 /* Run n_steps steps with the step size kept on the device (no host round
- * trip per step).  dt_fixed as in mlh_step; dts (may be NULL) receives the
- * n_steps step sizes after the last step. */
-int mlh_run(mlh_ctx *ctx, int32_t n_steps, double dt_fixed, double *dts);
+ * trip per step).  Step k uses dts_in[k] when dts_in is not NULL (a replayed
+ * dt sequence), else dt_fixed as in mlh_step.  dts_out (may be NULL)
+ * receives the n_steps step sizes after the last step. */
+int mlh_run(mlh_ctx *ctx, int32_t n_steps, double dt_fixed,
+            const double *dts_in, double *dts_out);
 
 /* ---- stages, for parity tests and the host Particles class ------------ */
 /* assignParticlesAndCells + createGhostParticles + gridNNS + ghostNNS
@@ -180,8 +182,9 @@ int mlh_get_diagnostics(mlh_ctx *ctx, mlh_diag *d);
 /* sumVolume, sumMass, sumEnergy, sumMomentumX/Y/Z (Particles.cpp:2830-2886)
  * into out[0..5]. */
 int mlh_conserved_sums(mlh_ctx *ctx, double *out);
-/* Device milliseconds of the last mlh_step / mlh_run per kernel group:
- * [0] cell list + neighbours, [1] density, [2] gradients, [3] faces. */
+/* Device milliseconds per step of the last mlh_step / mlh_run, averaged
+ * over its steps, per kernel group: [0] cell list + neighbours,
+ * [1] density, [2] gradients + time step, [3] faces + update. */
 int mlh_get_timings(mlh_ctx *ctx, float *ms4);
 
 /* ---- exact Riemann solver, batched (Riemann.cpp:83-141 contract) -------
@@ -197,6 +200,14 @@ int mlh_riemann_exact_batch(mlh_ctx *ctx, int64_t n, double gamma,
                             double *rho_s, double *u_s, double *P_s,
                             int32_t *flag, int32_t *iters, int32_t precision,
                             int32_t seed, double tol);
+
+/* Converged star pressure p* of the plain FP64 exact solver for n face
+ * problems (0 where vacuum forms): the training target of the learned seed. */
+int mlh_riemann_pstar_batch(int64_t n, double gamma, const double *rhoL,
+                            const double *uL, const double *PL,
+                            const double *rhoR, const double *uR,
+                            const double *PR, double tol, double *p_star,
+                            int32_t *iters);
 
 /* ---- learned p* seed: MLP 10 -> 64 -> 64 -> 1 (tanh hidden) ------------ */
 #define MLH_MLP_IN 10
@@ -224,6 +235,14 @@ int mlh_mlp_train(mlh_ctx *ctx, int64_t n, const float *feats,
 /* Inference: FP32 reference path (bf16 = 0) or BF16 MFMA path (bf16 = 1). */
 int mlh_mlp_infer(mlh_ctx *ctx, int64_t n, const float *feats, float *out,
                   int32_t bf16);
+/* Record the 1D face problems (rhoL,uL,PL,rhoR,uR,PR along the face normal,
+ * as handed to the exact solver) of every particle i with
+ * i % particle_stride == 0 during the following steps, up to cap faces
+ * (particle_stride <= 0 switches recording off).  Training data for the
+ * learned seed, including online retraining during a run. */
+int mlh_set_face_collection(mlh_ctx *ctx, int64_t cap, int32_t particle_stride);
+/* Copy the recorded faces (n x 6 doubles) out and restart the recording. */
+int mlh_get_collected_faces(mlh_ctx *ctx, double *states6, int64_t *n_out);
 
 #ifdef __cplusplus
 }

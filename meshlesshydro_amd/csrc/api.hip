@@ -51,7 +51,7 @@ struct mlh_ctx {
     double *sums_block = nullptr;
     MlpState mlp;
     int stage = 0;  // 0 state, 1 neighbours, 2 density, 3 gradients, 4 fluxes
-    hipEvent_t ev[6] = {};
+    std::vector<hipEvent_t> evpool;
     float timings[4] = {0, 0, 0, 0};
     std::vector<void *> allocs;
     int64_t step_count = 0;
@@ -171,7 +171,11 @@ static HydroArgs hydro_args(mlh_ctx *c) {
     a.mF = c->mF; a.vF = c->vF; a.eF = c->eF;
     a.counters = c->counters;
     a.status = c->status;
-    a.mlp = c->mlp.params;
+    a.mlp_frags = c->mlp.frags;
+    a.collect = c->mlp.collect_stride > 0 ? c->mlp.collect : nullptr;
+    a.collect_n = c->mlp.collect_n;
+    a.collect_cap = c->mlp.collect_cap;
+    a.collect_stride = c->mlp.collect_stride > 0 ? c->mlp.collect_stride : 1;
     return a;
 }
 
@@ -232,7 +236,6 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     hipMemsetAsync(c->max_counts, 0, sizeof(int) * 2, c->s);
     hipMemsetAsync(c->noi, 0, 4 * N, c->s);
     hipMemsetAsync(c->nog, 0, 4 * N, c->s);
-    for (int k = 0; k < 6; ++k) hipEventCreate(&c->ev[k]);
     if (hipStreamSynchronize(c->s) != hipSuccess) {
         mlh_destroy(c);
         return fail(MLH_ERR_DEVICE, "initialisation failed");
@@ -246,8 +249,7 @@ int mlh_destroy(mlh_ctx *c) {
     if (c->s) hipStreamSynchronize(c->s);
     for (void *q : c->allocs) hipFree(q);
     mlp_free(c->mlp);
-    for (int k = 0; k < 6; ++k)
-        if (c->ev[k]) hipEventDestroy(c->ev[k]);
+    for (hipEvent_t e : c->evpool) hipEventDestroy(e);
     if (c->dt_hist) hipFree(c->dt_hist);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
@@ -351,34 +353,47 @@ static int face_pressure_status(mlh_ctx *c, const unsigned long long *before,
 }
 
 static int check_learned(mlh_ctx *c) {
-    if (c->p.riemann_seed == MLH_SEED_LEARNED && !c->mlp.params)
+    if (c->p.riemann_seed == MLH_SEED_LEARNED && !c->mlp.frags)
         return fail(MLH_ERR_NOT_TRAINED, "learned seed requested but no network parameters set");
     return MLH_OK;
 }
 
+static int ensure_events(mlh_ctx *c, int nsteps) {
+    size_t need = (size_t)5 * (nsteps > 0 ? nsteps : 1);
+    while (c->evpool.size() < need) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        c->evpool.push_back(e);
+    }
+    return MLH_OK;
+}
+
 static void enqueue_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip,
-                         double *dt_hist, int step, bool timed) {
+                         const double *dt_in, double *dt_hist, int step, hipEvent_t *ev) {
     HydroArgs a = hydro_args(c);
     hipMemsetAsync(c->status, 0, sizeof(int), c->s);
-    if (timed) hipEventRecord(c->ev[0], c->s);
+    if (ev) hipEventRecord(ev[0], c->s);
     enqueue_neighbours(c);
-    if (timed) hipEventRecord(c->ev[1], c->s);
+    if (ev) hipEventRecord(ev[1], c->s);
     launch_density(a, c->D, c->prec, c->s);
-    if (timed) hipEventRecord(c->ev[2], c->s);
+    if (ev) hipEventRecord(ev[2], c->s);
     launch_gradients(a, c->D, c->prec, c->s);
-    launch_dt_finalize(a, dt_fixed, t_now, t_clip, dt_hist, step, c->s);
-    if (timed) hipEventRecord(c->ev[3], c->s);
+    launch_dt_finalize(a, dt_fixed, t_now, t_clip, dt_in, dt_hist, step, c->s);
+    if (ev) hipEventRecord(ev[3], c->s);
     launch_faces(a, c->D, c->prec, c->s);
-    if (timed) hipEventRecord(c->ev[4], c->s);
+    if (ev) hipEventRecord(ev[4], c->s);
     swap_state(c);
 }
 
-static void collect_timings(mlh_ctx *c) {
-    for (int k = 0; k < 4; ++k) {
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]);
-        c->timings[k] = ms;
-    }
+// average device time per step of each kernel group over nsteps steps
+static void collect_timings(mlh_ctx *c, int nsteps) {
+    for (int k = 0; k < 4; ++k) c->timings[k] = 0.f;
+    for (int st = 0; st < nsteps; ++st)
+        for (int k = 0; k < 4; ++k) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, c->evpool[5 * st + k], c->evpool[5 * st + k + 1]);
+            c->timings[k] += ms / (float)nsteps;
+        }
 }
 
 extern "C" {
@@ -390,11 +405,12 @@ int mlh_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip, double *d
     if (st) return st;
     unsigned long long before[kNumCounters], after[kNumCounters];
     if ((st = read_counters(c, before))) return st;
-    enqueue_step(c, dt_fixed, t_now, t_clip, nullptr, 0, true);
+    if ((st = ensure_events(c, 1))) return st;
+    enqueue_step(c, dt_fixed, t_now, t_clip, nullptr, nullptr, 0, c->evpool.data());
     double dtv[3];
     HIPCHK(hipMemcpyAsync(dtv, c->dt, sizeof(dtv), hipMemcpyDeviceToHost, c->s));
     if ((st = check_status(c))) { c->stage = 0; return st; }
-    collect_timings(c);
+    collect_timings(c, 1);
     if (dt_used) *dt_used = dtv[0];
     if (clipped) *clipped = dtv[2] != 0.;
     c->stage = 0;
@@ -403,25 +419,31 @@ int mlh_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip, double *d
     return face_pressure_status(c, before, after);
 }
 
-int mlh_run(mlh_ctx *c, int32_t n_steps, double dt_fixed, double *dts) {
+int mlh_run(mlh_ctx *c, int32_t n_steps, double dt_fixed, const double *dts_in,
+            double *dts_out) {
     if (!c || n_steps < 0) return fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
     int st = check_learned(c);
     if (st) return st;
     if (c->dt_hist_cap < n_steps) {
         if (c->dt_hist) hipFree(c->dt_hist);
         c->dt_hist = nullptr;
-        HIPCHK(hipMalloc(&c->dt_hist, sizeof(double) * (n_steps > 0 ? n_steps : 1)));
+        HIPCHK(hipMalloc(&c->dt_hist, sizeof(double) * 2 * (size_t)(n_steps > 0 ? n_steps : 1)));
         c->dt_hist_cap = n_steps;
     }
+    double *dev_in = nullptr;
+    if (dts_in && n_steps > 0) {
+        dev_in = c->dt_hist + c->dt_hist_cap;
+        HIPCHK(hipMemcpyAsync(dev_in, dts_in, sizeof(double) * n_steps, hipMemcpyHostToDevice, c->s));
+    }
+    if ((st = ensure_events(c, n_steps))) return st;
     unsigned long long before[kNumCounters], after[kNumCounters];
     if ((st = read_counters(c, before))) return st;
-    HIPCHK(hipEventRecord(c->ev[5], c->s));
     for (int k = 0; k < n_steps; ++k)
-        enqueue_step(c, dt_fixed, -1.0, -1.0, c->dt_hist, k, k == n_steps - 1);
+        enqueue_step(c, dt_fixed, -1.0, -1.0, dev_in, c->dt_hist, k, &c->evpool[5 * k]);
     if ((st = check_status(c))) return st;
-    if (n_steps > 0) collect_timings(c);
-    if (dts && n_steps > 0)
-        HIPCHK(hipMemcpy(dts, c->dt_hist, sizeof(double) * n_steps, hipMemcpyDeviceToHost));
+    if (n_steps > 0) collect_timings(c, n_steps);
+    if (dts_out && n_steps > 0)
+        HIPCHK(hipMemcpy(dts_out, c->dt_hist, sizeof(double) * n_steps, hipMemcpyDeviceToHost));
     c->stage = 0;
     c->step_count += n_steps;
     if ((st = read_counters(c, after))) return st;
@@ -453,7 +475,7 @@ int mlh_gradients(mlh_ctx *c) {
     if (c->stage < 2) return fail(MLH_ERR_CALL_ORDER, "mlh_density first");
     HydroArgs a = hydro_args(c);
     launch_gradients(a, c->D, c->prec, c->s);
-    launch_dt_finalize(a, -1.0, -1.0, -1.0, nullptr, 0, c->s);
+    launch_dt_finalize(a, -1.0, -1.0, -1.0, nullptr, nullptr, 0, c->s);
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipGetLastError());
     c->stage = 3;
@@ -608,6 +630,37 @@ int mlh_conserved_sums(mlh_ctx *c, double *out) {
     for (int b = 0; b < nb; ++b)
         for (int k = 0; k < 6; ++k) out[k] += h[(size_t)b * 6 + k];
     if (c->stage < 2) out[0] = 0.;
+    return MLH_OK;
+}
+
+int mlh_set_face_collection(mlh_ctx *c, int64_t cap, int32_t particle_stride) {
+    if (!c || cap < 0) return fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
+    MlpState &m = c->mlp;
+    if (m.collect) { hipFree(m.collect); m.collect = nullptr; }
+    if (!m.collect_n) HIPCHK(hipMalloc(&m.collect_n, sizeof(unsigned long long)));
+    HIPCHK(hipMemset(m.collect_n, 0, sizeof(unsigned long long)));
+    m.collect_cap = 0;
+    m.collect_stride = 0;
+    if (particle_stride <= 0 || cap == 0) return MLH_OK;
+    HIPCHK(hipMalloc(&m.collect, sizeof(double) * 6 * (size_t)cap));
+    m.collect_cap = cap;
+    m.collect_stride = particle_stride;
+    return MLH_OK;
+}
+
+int mlh_get_collected_faces(mlh_ctx *c, double *states6, int64_t *n_out) {
+    if (!c || !n_out) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    MlpState &m = c->mlp;
+    *n_out = 0;
+    if (!m.collect || !m.collect_n) return MLH_OK;
+    unsigned long long n = 0;
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(&n, m.collect_n, sizeof(n), hipMemcpyDeviceToHost));
+    if (n > (unsigned long long)m.collect_cap) n = (unsigned long long)m.collect_cap;
+    if (states6 && n)
+        HIPCHK(hipMemcpy(states6, m.collect, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemset(m.collect_n, 0, sizeof(unsigned long long)));
+    *n_out = (int64_t)n;
     return MLH_OK;
 }
 

@@ -18,6 +18,7 @@
 #include "kernels.h"
 #include "riemann.h"
 #include "mlp_device.h"
+#include "mlp_mfma.h"
 
 #pragma clang fp contract(off)
 
@@ -225,9 +226,12 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                 if (ngbmin[f] > fj) ngbmin[f] = fj;
             }
         }
-        // pass 3: slope limiter
+        // pass 3: slope limiter.  The reconstructed extrema and the ratios
+        // are evaluated in FP64 in both precisions: with the reference's
+        // int abs() a rounding tie f_ij == f_i turns into a 2^31 scaling,
+        // and FP32 reconstructions would create such ties where FP64 has none.
         if (a.slope_limiting) {
-            T midmax[NF], midmin[NF];
+            double midmax[NF], midmin[NF];
             for (int f = 0; f < NF; ++f) {
                 midmax[f] = tmin_pos<T>();
                 midmin[f] = tmax_val<T>();
@@ -236,26 +240,29 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                 double xj[3];
                 int s[3];
                 nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
-                T xijxi[D];
+                double xijxi[D];
                 for (int ax = 0; ax < D; ++ax) {
                     double xij = a.foqp ? (xi[ax] + xj[ax]) / 2.
                                         : xi[ax] + a.g.h / 4. * (xj[ax] - xi[ax]);
-                    xijxi[ax] = T(xij - xi[ax]);
+                    xijxi[ax] = xij - xi[ax];
                 }
                 for (int f = 0; f < NF; ++f) {
-                    T fij = fi[f] + dotT<D, T>(grad[f], xijxi);
+                    double gd[D];
+                    for (int ax = 0; ax < D; ++ax) gd[ax] = (double)grad[f][ax];
+                    double fij = (double)fi[f] + dotT<D, double>(gd, xijxi);
                     if (midmax[f] < fij) midmax[f] = fij;
                     if (midmin[f] > fij) midmin[f] = fij;
                 }
             }
-            const T beta = T(a.beta);
+            const double beta = a.beta;
             for (int f = 0; f < NF; ++f) {
-                T amax = lim_abs<T>((ngbmax[f] - fi[f]) / (midmax[f] - fi[f]), a.limiter_abs);
-                T amin = lim_abs<T>((fi[f] - ngbmin[f]) / (fi[f] - midmin[f]), a.limiter_abs);
-                if (amin <= amax && beta * amin < T(1)) {
-                    for (int ax = 0; ax < D; ++ax) grad[f][ax] *= amin;
-                } else if (amax <= amin && beta * amax < T(1)) {
-                    for (int ax = 0; ax < D; ++ax) grad[f][ax] *= amax;
+                double ff = (double)fi[f];
+                double amax = lim_abs<double>(((double)ngbmax[f] - ff) / (midmax[f] - ff), a.limiter_abs);
+                double amin = lim_abs<double>((ff - (double)ngbmin[f]) / (ff - midmin[f]), a.limiter_abs);
+                if (amin <= amax && beta * amin < 1.) {
+                    for (int ax = 0; ax < D; ++ax) grad[f][ax] = T((double)grad[f][ax] * amin);
+                } else if (amax <= amin && beta * amax < 1.) {
+                    for (int ax = 0; ax < D; ++ax) grad[f][ax] = T((double)grad[f][ax] * amax);
                 }
             }
         }
@@ -283,6 +290,7 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
 __global__ __launch_bounds__(256) void k_dt_finalize(const double *__restrict__ dt_block,
                                                      int nblocks, double dt_fixed,
                                                      double t_now, double t_clip,
+                                                     const double *__restrict__ dt_in,
                                                      double *__restrict__ dt,
                                                      double *__restrict__ dt_hist, int step) {
     __shared__ double sh[256];
@@ -302,7 +310,9 @@ __global__ __launch_bounds__(256) void k_dt_finalize(const double *__restrict__ 
         double d = sh[0];
         double clipped = 0.;
         dt[1] = d;
-        if (dt_fixed > 0.) {
+        if (dt_in) {
+            d = dt_in[step];
+        } else if (dt_fixed > 0.) {
             d = dt_fixed;
         } else if (t_clip > 0. && t_now + d >= t_clip) {
             d = t_clip - t_now;
@@ -384,9 +394,15 @@ __device__ inline T pairwise_limiter(const HydroArgs &a, T phi0, T phi_i, T phi_
 // :2533-2683) and the Riemann flux (Riemann.cpp:7-229).  F = {mass,
 // energy, momentum...} as seen from L.
 template <int D, typename T>
-__device__ inline void face_flux(const HydroArgs &a, const PData<D, T> &L,
-                                 const PData<D, T> &H, const int *s, bool image,
-                                 T dt, T *F, FaceStats &fs) {
+struct FacePrep {
+    T A[D], hA[D], WR[D + 2], WL[D + 2], vF[D];
+    T ui, uj, ti0, tj0;  // normal / transverse (2D) velocities in the face frame
+};
+
+template <int D, typename T>
+__device__ inline void face_prepare(const HydroArgs &a, const PData<D, T> &L,
+                                    const PData<D, T> &H, const int *s, bool image,
+                                    T dt, FacePrep<D, T> &fp, FaceStats &fs) {
     constexpr int NF = D + 2;
     const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
     double g[3], gp[3];
@@ -399,7 +415,7 @@ __device__ inline void face_flux(const HydroArgs &a, const PData<D, T> &L,
     T r2 = tsqrt(T(dist2<D>(H.x, gp)));
     T w1 = kernel_w<T>(r1, h2, sig) / PD_OM(L);
     T w2 = kernel_w<T>(r2, h2, sig) / PD_OM(H);
-    T A[D];
+    T *A = fp.A;
     for (int p = 0; p < D; ++p) {
         T p1 = T(0), p2 = T(0);
         for (int q = 0; q < D; ++q) {
@@ -421,7 +437,7 @@ __device__ inline void face_flux(const HydroArgs &a, const PData<D, T> &L,
             xijxj[ax] = T(xij - g[ax]);
         }
     }
-    T vF[D];
+    T *vF = fp.vF;
     if (!a.move_particles) {
         for (int ax = 0; ax < D; ++ax) vF[ax] = T(0);
     } else if (a.foqp) {
@@ -430,7 +446,7 @@ __device__ inline void face_flux(const HydroArgs &a, const PData<D, T> &L,
         T dp = dotT<D, T>(xijxi, xjxi), ds = dotT<D, T>(xjxi, xjxi);
         for (int ax = 0; ax < D; ++ax) vF[ax] = PD_V(L, ax) + (PD_V(H, ax) - PD_V(L, ax)) * dp / ds;
     }
-    T WR[NF], WL[NF];
+    T *WR = fp.WR, *WL = fp.WL;
     WR[0] = PD_RHO(L); WL[0] = PD_RHO(H);
     WR[1] = PD_P(L);   WL[1] = PD_P(H);
     for (int ax = 0; ax < D; ++ax) {
@@ -499,7 +515,7 @@ __device__ inline void face_flux(const HydroArgs &a, const PData<D, T> &L,
     }
     // Riemann (Riemann.cpp:7-141): face frame along A/|A|; solver left = L
     T An = tsqrt(dotT<D, T>(A, A));
-    T hA[D];
+    T *hA = fp.hA;
     for (int ax = 0; ax < D; ++ax) hA[ax] = T(1) / An * A[ax];
     T ui, uj;
     T ti0 = T(0), tj0 = T(0);
@@ -516,11 +532,19 @@ __device__ inline void face_flux(const HydroArgs &a, const PData<D, T> &L,
         ui = dotT<D, T>(hA, &WR[2]);
         uj = dotT<D, T>(hA, &WL[2]);
     }
+    fp.ui = ui; fp.uj = uj; fp.ti0 = ti0; fp.tj0 = tj0;
+}
+
+// exact Riemann solve from the FacePrep states and the flux projected on A
+// (Riemann.cpp:83-229).  seed > 0: Newton starts there (learned seed).
+template <int D, typename T>
+__device__ inline void face_finish(const HydroArgs &a, const FacePrep<D, T> &fp, float seed,
+                                   T *F, FaceStats &fs) {
+    const T *A = fp.A, *hA = fp.hA, *WR = fp.WR, *WL = fp.WL, *vF = fp.vF;
+    const T ui = fp.ui, uj = fp.uj, ti0 = fp.ti0, tj0 = fp.tj0;
+    const T gam = T(a.gamma);
     T rs, us, Ps;
     int flag;
-    float seed = -1.f;
-    if (a.learned) seed = mlp_seed_scalar(a.mlp, a.gamma, (double)WR[0], (double)ui, (double)WR[1],
-                                          (double)WL[0], (double)uj, (double)WL[1]);
     if constexpr (sizeof(T) == 8) {
         GasConst<double> gc(a.gamma);
         RSol<double> r = solve_fp64(gc, WR[0], ui, WR[1], WL[0], uj, WL[1], a.riemann_tol,
@@ -569,20 +593,37 @@ __device__ inline void face_flux(const HydroArgs &a, const PData<D, T> &L,
     F[1] = e;
 }
 
-template <int D, typename T>
+
+// Faces of particle i.  LEARNED: the loop runs to the wavefront's largest
+// neighbour count so that every lane takes part in the BF16 MFMA inference
+// of the p* seed for the wavefront's 64 faces of that iteration.
+template <int D, typename T, bool LEARNED>
 __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
     constexpr int NF = D + 2;
+    __shared__ uint16_t lds[LEARNED ? 4 : 1][LEARNED ? 64 * 16 : 1];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < a.N;
     FaceStats fs{};
-    if (i < a.N) {
-        const int N = a.N;
-        PData<D, T> S, O, L, H;
-        load_pdata<D, T>(a, i, S);
-        const double dtd = a.dt[0];
-        const T dt = T(dtd);
-        int nr = a.noi[i], cnt = nr + a.nog[i];
-        double mF = 0., eF = 0., vFs[3] = {0., 0., 0.};
-        for (int n = 0; n < cnt; ++n) {
+    const int N = a.N;
+    PData<D, T> S, O, L, H;
+    if (live) load_pdata<D, T>(a, i, S);
+    const double dtd = a.dt[0];
+    const T dt = T(dtd);
+    int cnt = live ? a.noi[i] + a.nog[i] : 0;
+    int lim = cnt;
+    if (LEARNED) {
+        for (int off = 32; off > 0; off >>= 1) {
+            int o = __shfl_xor(lim, off, 64);
+            lim = o > lim ? o : lim;
+        }
+    }
+    const bool collect = a.collect && live && (i % a.collect_stride) == 0;
+    double mF = 0., eF = 0., vFs[3] = {0., 0., 0.};
+    for (int n = 0; n < lim; ++n) {
+        const bool act = n < cnt;
+        FacePrep<D, T> fp;
+        bool self_lo = true;
+        if (act) {
             uint32_t e = a.nbr[(size_t)n * N + i];
             int j = (int)nb_index(e);
             uint32_t code = nb_code(e);
@@ -591,7 +632,7 @@ __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
             if (image)
                 for (int ax = 0; ax < D; ++ax) s[ax] = code_shift(code, ax);
             // side that solves the pair: the lower index (images: parent >= i)
-            bool self_lo = !a.enforce_flux_sym || (image ? j >= i : j > i);
+            self_lo = !a.enforce_flux_sym || (image ? j >= i : j > i);
             load_pdata<D, T>(a, j, O);
             for (int ax = 0; ax < 3; ++ax) {
                 L.x[ax] = self_lo ? S.x[ax] : O.x[ax];
@@ -603,13 +644,56 @@ __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
             }
             int sl[3];
             for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
+            face_prepare<D, T>(a, L, H, sl, image, dt, fp, fs);
+        }
+        float seed = -1.f;
+        if (LEARNED) {
+            // the network only runs for wavefronts holding a face with a
+            // real jump; near-uniform faces keep the standard guess, which
+            // is already converged to O(jump^2) there
+            bool strong = false;
+            if (act) {
+                T dp = fp.WR[1] - fp.WL[1], du = fp.uj - fp.ui;
+                T psum = fp.WR[1] + fp.WL[1];
+                T c2 = T(a.gamma) * (fp.WR[1] / fp.WR[0] + fp.WL[1] / fp.WL[0]);
+                strong = dp * dp > T(kSeedJump * kSeedJump) * psum * psum ||
+                         du * du > T(2 * kSeedJump * kSeedJump) * c2;
+            }
+            if (__ballot(strong)) {
+                float f[kMlpIn];
+                float pref = 1.f;
+                bool ok = strong && fp.WR[0] > T(0) && fp.WL[0] > T(0) && fp.WR[1] > T(0) &&
+                          fp.WL[1] > T(0);
+                if (ok)
+                    pref = mlp_features_fast((float)a.gamma, (float)fp.WR[0], (float)fp.ui,
+                                             (float)fp.WR[1], (float)fp.WL[0], (float)fp.uj,
+                                             (float)fp.WL[1], f);
+                else
+                    for (int q = 0; q < kMlpIn; ++q) f[q] = 0.f;
+                float y = mlp_infer_wave_bf16_mem(a.mlp_frags, f, lds[threadIdx.x >> 6]);
+                seed = ok ? mlp_seed_from_output(y, f, pref) : -1.f;
+            }
+        }
+        if (act) {
             T F[NF];
-            face_flux<D, T>(a, L, H, sl, image, dt, F, fs);
+            int it0 = fs.st.iters;
+            face_finish<D, T>(a, fp, seed, F, fs);
+            // training data: faces the exact solver needed >= 2 iterations for
+            if (collect && self_lo && fs.st.iters - it0 >= 2) {
+                unsigned long long slot = atomicAdd(a.collect_n, 1ull);
+                if (slot < (unsigned long long)a.collect_cap) {
+                    double *c = a.collect + slot * 6;
+                    c[0] = fp.WR[0]; c[1] = fp.ui; c[2] = fp.WR[1];
+                    c[3] = fp.WL[0]; c[4] = fp.uj; c[5] = fp.WL[1];
+                }
+            }
             double sgn = self_lo ? 1. : -1.;
             mF += sgn * (double)F[0];
             for (int ax = 0; ax < D; ++ax) vFs[ax] += sgn * (double)F[2 + ax];
             eF += sgn * (double)F[1];
         }
+    }
+    if (live) {
         if (a.write_fluxes) {
             a.mF[i] = mF;
             a.eF[i] = eF;
@@ -654,8 +738,7 @@ __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
         (unsigned long long)fs.neg_real, (unsigned long long)fs.neg_ghost,
         (unsigned long long)fs.vacuum, (unsigned long long)fs.st.nonconv, 0ull,
         (unsigned long long)fs.st.fallback, (unsigned long long)fs.st.iters,
-        (unsigned long long)0};
-    if (i < a.N) c[kCntFaces] = (unsigned long long)(a.noi[i] + a.nog[i]);
+        (unsigned long long)cnt};
     for (int k = 0; k < kNumCounters; ++k) {
         unsigned long long v = wave_sum(c[k]);
         if (lane_id() == 0 && v) atomicAdd(&a.counters[k], v);
@@ -711,12 +794,23 @@ void launch_gradients(const HydroArgs &a, int D, int precision, hipStream_t s) {
     MLH_DISPATCH(k_gradients, D, precision, dim3((a.N + 255) / 256), a);
 }
 void launch_dt_finalize(const HydroArgs &a, double dt_fixed, double t_now, double t_clip,
-                        double *dt_hist, int step, hipStream_t s) {
+                        const double *dt_in, double *dt_hist, int step, hipStream_t s) {
     hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s, a.dt_block, dt_blocks(a.N),
-                       dt_fixed, t_now, t_clip, a.dt, dt_hist, step);
+                       dt_fixed, t_now, t_clip, dt_in, a.dt, dt_hist, step);
 }
 void launch_faces(const HydroArgs &a, int D, int precision, hipStream_t s) {
-    MLH_DISPATCH(k_faces, D, precision, dim3((a.N + 255) / 256), a);
+    dim3 grid((a.N + 255) / 256);
+    if (a.learned) {
+        if (D == 2 && precision == 0) hipLaunchKernelGGL((k_faces<2, double, true>), grid, dim3(256), 0, s, a);
+        else if (D == 2) hipLaunchKernelGGL((k_faces<2, float, true>), grid, dim3(256), 0, s, a);
+        else if (precision == 0) hipLaunchKernelGGL((k_faces<3, double, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_faces<3, float, true>), grid, dim3(256), 0, s, a);
+    } else {
+        if (D == 2 && precision == 0) hipLaunchKernelGGL((k_faces<2, double, false>), grid, dim3(256), 0, s, a);
+        else if (D == 2) hipLaunchKernelGGL((k_faces<2, float, false>), grid, dim3(256), 0, s, a);
+        else if (precision == 0) hipLaunchKernelGGL((k_faces<3, double, false>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_faces<3, float, false>), grid, dim3(256), 0, s, a);
+    }
 }
 void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s) {
     MLH_DISPATCH(k_sums, D, precision, dim3((a.N + 255) / 256), a, out);

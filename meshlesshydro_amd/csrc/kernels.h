@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "device_common.h"
+#include "mlp_api.h"
 
 namespace mlh {
 
@@ -41,7 +42,11 @@ struct HydroArgs {
     double *mF, *vF, *eF;        // optional flux sums (write_fluxes)
     unsigned long long *counters;  // mlh_diag counters
     int *status;
-    const float *mlp;            // learned seed parameters (device)
+    const struct MlpFrags *mlp_frags;  // learned seed network (device)
+    double *collect;             // face collection [cap][6] (or null)
+    unsigned long long *collect_n;
+    long long collect_cap;
+    int collect_stride;
 };
 
 enum CounterIdx {
@@ -63,7 +68,7 @@ void launch_density(const HydroArgs &a, int D, int precision, hipStream_t s);
 void launch_gradients(const HydroArgs &a, int D, int precision, hipStream_t s);
 int dt_blocks(int N);
 void launch_dt_finalize(const HydroArgs &a, double dt_fixed, double t_now, double t_clip,
-                        double *dt_hist, int step, hipStream_t s);
+                        const double *dt_in, double *dt_hist, int step, hipStream_t s);
 void launch_faces(const HydroArgs &a, int D, int precision, hipStream_t s);
 void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s);
 

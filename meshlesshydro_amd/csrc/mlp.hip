@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_riemann_batch(
     const double *__restrict__ pr, double *__restrict__ ds, double *__restrict__ us,
     double *__restrict__ ps, int *__restrict__ flag, int *__restrict__ iters, int precision,
     const MlpFrags *__restrict__ F, double tol, int max_iter, int iters32,
-    unsigned long long *__restrict__ fallbacks) {
+    unsigned long long *__restrict__ fallbacks, double *__restrict__ pstar) {
     __shared__ uint16_t lds[4][64 * 16];
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool active = k < n;
@@ -212,21 +212,26 @@ __global__ __launch_bounds__(256) void k_riemann_batch(
         // every lane of the wavefront takes part in the MFMA inference
         float f[kMlpIn];
         bool ok = a0 > 0. && b0 > 0. && a2 > 0. && b2 > 0.;
-        float pref = ok ? mlp_features((float)gamma, (float)a0, (float)a1, (float)a2, (float)b0,
-                                       (float)b1, (float)b2, f)
+        float pref = ok ? mlp_features_fast((float)gamma, (float)a0, (float)a1, (float)a2,
+                                            (float)b0, (float)b1, (float)b2, f)
                         : 1.f;
         if (!ok)
             for (int q = 0; q < kMlpIn; ++q) f[q] = 0.f;
         MlpRegs R;
         mlp_load_regs(F, R);
         float y = mlp_infer_wave_bf16(R, f, lds[threadIdx.x >> 6]);
-        y = fminf(fmaxf(y, -40.f), 5.f);
-        seed = ok ? pref * __expf(y) : -1.f;
+        seed = ok ? mlp_seed_from_output(y, f, pref) : -1.f;
     }
     if (!active) return;
     SolveStats st{};
     double rd, ru, rp;
     int fl;
+    if (pstar) {
+        GasConst<double> gc(gamma);
+        pstar[k] = star_pressure_fp64(gc, a0, a1, a2, b0, b1, b2, tol, max_iter, st);
+        if (iters) iters[k] = st.iters;
+        return;
+    }
     if (precision == MLH_FP64) {
         GasConst<double> gc(gamma);
         RSol<double> r = solve_fp64(gc, a0, a1, a2, b0, b1, b2, tol, max_iter, (double)seed, st);
@@ -464,13 +469,42 @@ int mlh_riemann_exact_batch(mlh_ctx *c, int64_t n, double gamma, const double *r
     int max_iter = 50;
     hipLaunchKernelGGL(k_riemann_batch, dim3((n + 255) / 256), dim3(256), 0, s, n, gamma, buf,
                        buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n, buf + 5 * n, buf + 6 * n,
-                       buf + 7 * n, buf + 8 * n, ib, ib + n, precision, F, tol, max_iter, 3, fb);
+                       buf + 7 * n, buf + 8 * n, ib, ib + n, precision, F, tol, max_iter, 3, fb,
+                       nullptr);
     hipMemcpyAsync(rho_s, buf + 6 * n, b, hipMemcpyDeviceToHost, s);
     hipMemcpyAsync(u_s, buf + 7 * n, b, hipMemcpyDeviceToHost, s);
     hipMemcpyAsync(P_s, buf + 8 * n, b, hipMemcpyDeviceToHost, s);
     if (flag) hipMemcpyAsync(flag, ib, sizeof(int) * n, hipMemcpyDeviceToHost, s);
     if (iters) hipMemcpyAsync(iters, ib + n, sizeof(int) * n, hipMemcpyDeviceToHost, s);
     hipError_t e = hipStreamSynchronize(s);
+    hipFree(buf);
+    hipFree(ib);
+    hipFree(fb);
+    if (e != hipSuccess) return ctx_fail(MLH_ERR_DEVICE, hipGetErrorString(e));
+    return MLH_OK;
+}
+
+int mlh_riemann_pstar_batch(int64_t n, double gamma, const double *rhoL, const double *uL,
+                            const double *PL, const double *rhoR, const double *uR,
+                            const double *PR, double tol, double *p_star, int32_t *iters) {
+    if (n < 0 || !rhoL || !uL || !PL || !rhoR || !uR || !PR || !p_star)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0) return MLH_OK;
+    double *buf = nullptr;
+    int *ib = nullptr;
+    unsigned long long *fb = nullptr;
+    CHK(dmalloc(&buf, sizeof(double) * n * 7));
+    CHK(dmalloc(&ib, sizeof(int) * n));
+    CHK(dmalloc(&fb, sizeof(unsigned long long)));
+    size_t b = sizeof(double) * n;
+    const double *in[6] = {rhoL, uL, PL, rhoR, uR, PR};
+    for (int k = 0; k < 6; ++k) hipMemcpy(buf + k * n, in[k], b, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_riemann_batch, dim3((n + 255) / 256), dim3(256), 0, 0, n, gamma, buf,
+                       buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n, buf + 5 * n, buf, buf, buf,
+                       nullptr, ib, MLH_FP64, nullptr, tol, 50, 3, fb, buf + 6 * n);
+    hipMemcpy(p_star, buf + 6 * n, b, hipMemcpyDeviceToHost);
+    if (iters) hipMemcpy(iters, ib, sizeof(int) * n, hipMemcpyDeviceToHost);
+    hipError_t e = hipDeviceSynchronize();
     hipFree(buf);
     hipFree(ib);
     hipFree(fb);

@@ -10,7 +10,8 @@
 //   f4 = log(p_PVRS/p_ref)    f5 = log(p_TRRS/p_ref)
 //   f6 = log(p_TSRS/p_ref)    f7 = log(p_guess/p_ref) (Toro's adaptive pick)
 //   f8 = (gamma-1)/(gamma+1)  f9 = log(rhoL/rhoR)/4
-// Output: y ~ log(p*/p_ref); the seed is p_ref * exp(y).
+// Output: y ~ log(p*/p_guess), the correction of Toro's adaptive guess;
+// the seed is p_ref * exp(y + f7).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -75,6 +76,63 @@ __host__ __device__ inline float mlp_features(float gamma, float dl, float ul, f
     return pref;
 }
 
+// The same features with the hardware log2 / exp2 / reciprocal (about
+// 1e-6 relative), used inside the face kernel; BF16 inputs keep ~3
+// significant digits anyway.
+__device__ inline float fast_log(float x) { return __builtin_amdgcn_logf(x) * 0.69314718f; }
+__device__ inline float fast_pow(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+__device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float clog_fast(float r) {
+    float v = fast_log(r > 1e-30f ? r : 1e-30f);
+    v = fminf(fmaxf(v, -40.f), 5.f);
+    return v != v ? -40.f : v;
+}
+
+__device__ inline float mlp_features_fast(float gamma, float dl, float ul, float pl, float dr,
+                                          float ur, float pr, float *f) {
+    float g1 = (gamma - 1.f) / (2.f * gamma), g3 = 2.f * gamma / (gamma - 1.f);
+    float g4 = 2.f / (gamma - 1.f), g5 = 2.f / (gamma + 1.f), g6 = (gamma - 1.f) / (gamma + 1.f),
+          g7 = (gamma - 1.f) / 2.f;
+    float cl = __builtin_sqrtf(gamma * pl * fast_rcp(dl)), cr = __builtin_sqrtf(gamma * pr * fast_rcp(dr));
+    float pref = 0.5f * (pl + pr);
+    float ipref = fast_rcp(pref);
+    float du = ur - ul;
+    float ppv = 0.5f * (pl + pr) - 0.125f * du * (dl + dr) * (cl + cr);
+    if (ppv < 0.f) ppv = 0.f;
+    float icl = fast_rcp(cl), icr = fast_rcp(cr);
+    float pq = fast_pow(pl * fast_rcp(pr), g1);
+    float um = (pq * ul * icl + ur * icr + g4 * (pq - 1.f)) * fast_rcp(pq * icl + icr);
+    float ptl = 1.f + g7 * (ul - um) * icl;
+    float ptr = 1.f + g7 * (um - ur) * icr;
+    float ptrr = (ptl > 0.f && ptr > 0.f) ? 0.5f * (pl * fast_pow(ptl, g3) + pr * fast_pow(ptr, g3)) : 0.f;
+    float tiny = 1e-6f * pref;
+    float p0 = ppv > tiny ? ppv : tiny;
+    float gel = __builtin_sqrtf((g5 * fast_rcp(dl)) * fast_rcp(g6 * pl + p0));
+    float ger = __builtin_sqrtf((g5 * fast_rcp(dr)) * fast_rcp(g6 * pr + p0));
+    float pts = (gel * pl + ger * pr - du) * fast_rcp(gel + ger);
+    float pmin = pl < pr ? pl : pr, pmax = pl > pr ? pl : pr;
+    float pg;
+    if (pmax <= 2.f * pmin && pmin <= ppv && ppv <= pmax) pg = ppv;
+    else if (ppv < pmin) pg = ptrr;
+    else pg = pts;
+    f[0] = clog_fast(pl * ipref);
+    f[1] = clog_fast(pr * ipref);
+    f[2] = du * fast_rcp(cl + cr);
+    f[3] = (cl - cr) * fast_rcp(cl + cr);
+    f[4] = clog_fast(ppv * ipref);
+    f[5] = clog_fast(ptrr * ipref);
+    f[6] = clog_fast(pts * ipref);
+    f[7] = clog_fast(pg * ipref);
+    f[8] = g6;
+    f[9] = clog_fast(dl * fast_rcp(dr)) * 0.25f;
+    return pref;
+}
+
+// relative pressure / velocity jump above which a face uses the learned seed
+constexpr float kSeedJump = 1e-2f;
+
 __device__ inline float mlp_forward_scalar(const float *__restrict__ P, const float *x) {
     float h1[kMlpH];
     for (int o = 0; o < kMlpH; ++o) {
@@ -89,20 +147,6 @@ __device__ inline float mlp_forward_scalar(const float *__restrict__ P, const fl
         y += P[kMlpW3 + o] * tanhf(s);
     }
     return y;
-}
-
-// seed p0 for one face from the network (FP32 VALU inference)
-__device__ inline float mlp_seed_scalar(const float *__restrict__ P, double gamma, double dl,
-                                        double ul, double pl, double dr, double ur,
-                                        double pr) {
-    if (!P || !(dl > 0.0) || !(dr > 0.0) || !(pl > 0.0) || !(pr > 0.0)) return -1.f;
-    float f[kMlpIn];
-    float pref = mlp_features((float)gamma, (float)dl, (float)ul, (float)pl, (float)dr,
-                              (float)ur, (float)pr, f);
-    float y = mlp_forward_scalar(P, f);
-    if (y < -40.f) y = -40.f;
-    if (y > 5.f) y = 5.f;
-    return pref * expf(y);
 }
 
 }  // namespace mlh

@@ -114,4 +114,22 @@ __device__ inline float mlp_infer_wave_bf16(const MlpRegs &R, const float *x, ui
     return out + R.b3;
 }
 
+// Same as mlp_infer_wave_bf16 with the fragments read from memory (L1 /
+// L2 resident, 384 bytes per lane) at each call, for kernels that cannot
+// keep ~100 VGPRs of weights live.
+__device__ inline float mlp_infer_wave_bf16_mem(const MlpFrags *__restrict__ F, const float *x,
+                                                uint16_t *lds) {
+    MlpRegs R;
+    mlp_load_regs(F, R);
+    return mlp_infer_wave_bf16(R, x, lds);
+}
+
+// The network predicts the correction of log p* over Toro's adaptive guess
+// (feature 7): seed = p_ref * exp(y + f7), clamped like the features.
+__device__ inline float mlp_seed_from_output(float y, const float *f, float pref) {
+    float l = y + f[7];
+    l = fminf(fmaxf(l, -40.f), 5.f);
+    return pref * __expf(l);
+}
+
 }  // namespace mlh

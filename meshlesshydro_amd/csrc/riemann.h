@@ -209,6 +209,31 @@ __device__ inline RSol<double> solve_fp64(const GasConst<double> &c, double dl,
     return sample(c, p, um, dl, ul, pl, cl, dr, ur, pr, cr);
 }
 
+// Converged star pressure p* of the plain FP64 solver (0 for vacuum).
+__device__ inline double star_pressure_fp64(const GasConst<double> &c, double dl, double ul,
+                                            double pl, double dr, double ur, double pr,
+                                            double tol, int max_iter, SolveStats &st) {
+    if (!(dl > 0.0) || !(dr > 0.0)) return 0.0;
+    double cl = sqrt(c.g * pl / dl), cr = sqrt(c.g * pr / dr);
+    double du = ur - ul;
+    if (c.g4 * (cl + cr) <= du) return 0.0;
+    double tiny = tol * 0.5 * (pl + pr);
+    double pold = guess_p(c, dl, ul, pl, cl, dr, ur, pr, cr, tiny);
+    double p = pold, fl, fld, fr, frd;
+    int it;
+    for (it = 1; it <= max_iter; ++it) {
+        prefun(c, pold, dl, pl, cl, fl, fld);
+        prefun(c, pold, dr, pr, cr, fr, frd);
+        p = pold - (fl + fr + du) / (fld + frd);
+        double change = 2.0 * fabs((p - pold) / (p + pold));
+        if (change <= tol) break;
+        if (p < 0.0) p = tiny;
+        pold = p;
+    }
+    st.iters += it;
+    return p;
+}
+
 // FP32 Newton from p_seed, FP64 residual step, FP64 fallback.  Inputs are
 // the FP32 face states; p_seed <= 0 selects the standard FP32 guess.
 __device__ inline RSol<float> solve_mixed(const GasConst<float> &cf,

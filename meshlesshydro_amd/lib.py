@@ -28,7 +28,7 @@ SIGNATURES = {
     "mlh_upload_state": (C.c_int, [_vp] + [_dp] * 8),
     "mlh_download_state": (C.c_int, [_vp] + [_dp] * 8),
     "mlh_step": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _dp, _ip]),
-    "mlh_run": (C.c_int, [_vp, C.c_int32, C.c_double, _dp]),
+    "mlh_run": (C.c_int, [_vp, C.c_int32, C.c_double, _dp, _dp]),
     "mlh_neighbours": (C.c_int, [_vp]),
     "mlh_density": (C.c_int, [_vp]),
     "mlh_timestep": (C.c_int, [_vp, _dp]),
@@ -49,6 +49,9 @@ SIGNATURES = {
     "mlh_mlp_loss_grad": (C.c_int, [_vp, C.c_int64, _fp, _fp, _fp, _fp]),
     "mlh_mlp_train": (C.c_int, [_vp, C.c_int64, _fp, _fp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]),
     "mlh_mlp_infer": (C.c_int, [_vp, C.c_int64, _fp, _fp, C.c_int32]),
+    "mlh_riemann_pstar_batch": (C.c_int, [C.c_int64, C.c_double] + [_dp] * 6 + [C.c_double, _dp, _ip]),
+    "mlh_set_face_collection": (C.c_int, [_vp, C.c_int64, C.c_int32]),
+    "mlh_get_collected_faces": (C.c_int, [_vp, _dp, C.POINTER(C.c_int64)]),
 }
 
 _lib = None
@@ -131,9 +134,13 @@ class Simulation:
         _chk(self.L.mlh_step(self.h, dt_fixed, t_now, t_clip, C.byref(dt), C.byref(cl)), "mlh_step")
         return dt.value
 
-    def run(self, n_steps, dt_fixed=-1.0):
+    def run(self, n_steps, dt_fixed=-1.0, dts_in=None):
         dts = np.zeros(max(n_steps, 1))
-        _chk(self.L.mlh_run(self.h, n_steps, dt_fixed, _d(dts)), "mlh_run")
+        din = None
+        if dts_in is not None:
+            din = np.ascontiguousarray(dts_in[:n_steps], dtype=np.float64)
+            assert len(din) == n_steps
+        _chk(self.L.mlh_run(self.h, n_steps, dt_fixed, _d(din) if din is not None else None, _d(dts)), "mlh_run")
         return dts[:n_steps]
 
     # stages
@@ -229,6 +236,17 @@ class Simulation:
         _chk(self.L.mlh_mlp_infer(self.h, len(feats), _f(feats), _f(out), int(bf16)), "mlp_infer")
         return out
 
+    def set_face_collection(self, cap, particle_stride):
+        self._collect_cap = int(cap)
+        _chk(self.L.mlh_set_face_collection(self.h, int(cap), int(particle_stride)), "set_face_collection")
+
+    def collected_faces(self):
+        cap = getattr(self, "_collect_cap", 0)
+        out = np.zeros((max(cap, 1), 6))
+        n = C.c_int64()
+        _chk(self.L.mlh_get_collected_faces(self.h, _d(out), C.byref(n)), "get_collected_faces")
+        return out[: n.value].copy()
+
     def riemann(self, rl, ul, pl, rr, ur, pr, precision=MLH_FP64, seed=MLH_SEED_STANDARD, tol=1e-6):
         return riemann_batch(self.p.gamma, rl, ul, pl, rr, ur, pr, precision, seed, tol, ctx=self.h)
 
@@ -241,6 +259,16 @@ def riemann_batch(gamma, rl, ul, pl, rr, ur, pr, precision=MLH_FP64, seed=MLH_SE
     flag, it = np.zeros(n, np.int32), np.zeros(n, np.int32)
     _chk(L.mlh_riemann_exact_batch(ctx, n, gamma, *[_d(a) for a in arrs], _d(rs), _d(us), _d(ps), _i(flag), _i(it), precision, seed, tol), "mlh_riemann_exact_batch")
     return rs, us, ps, flag, it
+
+
+def riemann_pstar(gamma, rl, ul, pl, rr, ur, pr, tol=1e-6):
+    L = lib()
+    n = len(rl)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rl, ul, pl, rr, ur, pr)]
+    ps = np.zeros(n)
+    it = np.zeros(n, np.int32)
+    _chk(L.mlh_riemann_pstar_batch(n, gamma, *[_d(a) for a in arrs], tol, _d(ps), _i(it)), "mlh_riemann_pstar_batch")
+    return ps, it
 
 
 def mlp_features(gamma, rl, ul, pl, rr, ur, pr):
