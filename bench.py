@@ -57,11 +57,26 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=0, help="side of the CPU sample (default: --n)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="profile helper: one 1 GiB device-to-device copy kernel (torch) "
+                         "to scale FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--only", choices=["fp64", "fp32", "learned"], default=None,
+                    help="profile helper: run one mode for --steps steps and print its timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this MFV step has no multi-GPU path "
               "(sharding is described in DESIGN.md); use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.calib_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+        return
 
     from meshlesshydro_amd import ics, lib
     from meshlesshydro_amd import learned as L
@@ -74,6 +89,27 @@ def main():
     base = dict(dim=3, kernel_size=ics.blast_kernel_size(n), max_interactions=96,
                 max_ghost_interactions=96, debug_level=0)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+
+    if args.only:
+        # one mode only, for rocprofv3 runs (no error measurement)
+        prec = MLH_FP64 if args.only == "fp64" else MLH_FP32
+        p = default_params(precision=prec, **base)
+        s = lib.Simulation(p, state)
+        if args.only == "learned":
+            faces = L.collect_faces(default_params(precision=MLH_FP64, **base), state,
+                                    steps=max(W, 1), stride=8)
+            params_mlp, _ = L.train_seed_from_faces(s, faces, p.gamma, n_epochs=2, log=log)
+            s.close()
+            p.riemann_seed = MLH_SEED_LEARNED
+            s = lib.Simulation(p, state)
+            s.mlp_set_params(params_mlp)
+        s.run(W)
+        t0 = time.perf_counter()
+        s.run(K)
+        t = time.perf_counter() - t0
+        print(json.dumps({"mode": args.only, "particle_steps_per_s": N * K / t,
+                          "kernel_ms": [float(x) for x in s.timings()], "diag": s.diag()}))
+        return
 
     # 1. FP64 plain, own CFL steps; the warmup steps also record the exact
     #    solver's faces (every 8th particle) as training data for the seed

@@ -51,6 +51,7 @@ struct mlh_ctx {
     double *sums_block = nullptr;
     MlpState mlp;
     int stage = 0;  // 0 state, 1 neighbours, 2 density, 3 gradients, 4 fluxes
+    int img_window = 2;
     std::vector<hipEvent_t> evpool;
     float timings[4] = {0, 0, 0, 0};
     std::vector<void *> allocs;
@@ -132,6 +133,15 @@ static int make_geom(mlh_ctx *c) {
     }
     if (ncells > (1ll << 27)) return fail(MLH_ERR_GRID, "too many search cells");
     g.ncells = (int)ncells;
+    // Two points with computed d^2 < h^2 differ by less than h per axis.
+    // Their computed cell indices differ by at most 1 when the cell size
+    // exceeds h by more than the rounding of floor((x-min)/cs) and of the
+    // image coordinates (~1e-14 relative), so a +-1 window of virtual cells
+    // finds every periodic image; otherwise (cs == h up to rounding, e.g. a
+    // box of exactly 25 kernel sizes) a +-2 window is used.
+    c->img_window = 1;
+    for (int a = 0; a < c->D; ++a)
+        if (!(g.cs[a] > p.kernel_size * (1.0 + 1e-10))) c->img_window = 2;
     return 0;
 }
 
@@ -311,6 +321,7 @@ static void enqueue_neighbours(mlh_ctx *c) {
     a.nog = c->nog;
     a.status = c->status;
     a.max_counts = c->max_counts;
+    a.img_window = c->img_window;
     launch_neighbours(a, c->D, c->s);
 }
 

@@ -285,8 +285,12 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
         return;
     }
     int gcnt = 0;
-    if (g.periodic) {
-        const int W = 2;
+    // images can only come from virtual cells within W of the particle's
+    // cell; a particle with no such cell wrapping skips the image search
+    const int W = a.img_window;
+    bool near_edge = false;
+    for (int ax = 0; ax < D; ++ax) near_edge |= cc[ax] - W < 0 || cc[ax] + W >= g.nc[ax];
+    if (g.periodic && near_edge) {
         for (int vz = (D == 3 ? cc[2] - W : 0); vz <= (D == 3 ? cc[2] + W : 0) && !err; ++vz)
             for (int vy = cc[1] - W; vy <= cc[1] + W && !err; ++vy)
                 for (int vx = cc[0] - W; vx <= cc[0] + W && !err; ++vx) {

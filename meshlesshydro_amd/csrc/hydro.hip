@@ -325,33 +325,30 @@ __global__ __launch_bounds__(256) void k_dt_finalize(const double *__restrict__ 
 }
 
 // -------------------------------------------------------------------- faces
+// A particle's face inputs, read from its records where they are used
+// (positions, primitive record, gradients + B): selecting the solving side
+// selects an index, not ~70 registers of data.
 template <int D, typename T>
-struct PData {
-    static constexpr int NF = D + 2;
-    static constexpr int NT = 6 + NF * D + D * D;
+struct View {
     double x[3];
-    T t[NT];  // rho, P, omega, v[3], grad[NF][D], B[D][D]
+    const T *p;  // primitive record
+    const T *f;  // face record
 };
 template <int D, typename T>
-__device__ inline void load_pdata(const HydroArgs &a, int j, PData<D, T> &d) {
-    double4 p = a.pos[j];
-    d.x[0] = p.x; d.x[1] = p.y; d.x[2] = p.z;
-    const T *pr = (const T *)a.prim + (size_t)j * kPrimN;
-    d.t[0] = pr[kPrimRho];
-    d.t[1] = pr[kPrimP];
-    d.t[2] = pr[kPrimOmega];
-    d.t[3] = pr[kPrimV + 0];
-    d.t[4] = pr[kPrimV + 1];
-    d.t[5] = pr[kPrimV + 2];
-    const T *fr = (const T *)a.frec + (size_t)j * FRec<D>::N;
-    for (int k = 0; k < FRec<D>::N; ++k) d.t[6 + k] = fr[k];
+__device__ inline View<D, T> make_view(const HydroArgs &a, int j) {
+    View<D, T> v;
+    double4 q = a.pos[j];
+    v.x[0] = q.x; v.x[1] = q.y; v.x[2] = q.z;
+    v.p = (const T *)a.prim + (size_t)j * kPrimN;
+    v.f = (const T *)a.frec + (size_t)j * FRec<D>::N;
+    return v;
 }
-#define PD_RHO(d) (d).t[0]
-#define PD_P(d) (d).t[1]
-#define PD_OM(d) (d).t[2]
-#define PD_V(d, ax) (d).t[3 + (ax)]
-#define PD_G(d, f, ax) (d).t[6 + (f) * D + (ax)]
-#define PD_B(d, p, q) (d).t[6 + (D + 2) * D + (p) * D + (q)]
+#define PD_RHO(d) (d).p[kPrimRho]
+#define PD_P(d) (d).p[kPrimP]
+#define PD_OM(d) (d).p[kPrimOmega]
+#define PD_V(d, ax) (d).p[kPrimV + (ax)]
+#define PD_G(d, fi_, ax) (d).f[(fi_) * D + (ax)]
+#define PD_B(d, p_, q_) (d).f[FRec<D>::B + (p_) * D + (q_)]
 
 struct FaceStats {
     int neg_real, neg_ghost, vacuum;
@@ -400,8 +397,8 @@ struct FacePrep {
 };
 
 template <int D, typename T>
-__device__ inline void face_prepare(const HydroArgs &a, const PData<D, T> &L,
-                                    const PData<D, T> &H, const int *s, bool image,
+__device__ inline void face_prepare(const HydroArgs &a, const View<D, T> &L,
+                                    const View<D, T> &H, const int *s, bool image,
                                     T dt, FacePrep<D, T> &fp, FaceStats &fs) {
     constexpr int NF = D + 2;
     const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
@@ -543,6 +540,12 @@ __device__ inline void face_finish(const HydroArgs &a, const FacePrep<D, T> &fp,
     const T *A = fp.A, *hA = fp.hA, *WR = fp.WR, *WL = fp.WL, *vF = fp.vF;
     const T ui = fp.ui, uj = fp.uj, ti0 = fp.ti0, tj0 = fp.tj0;
     const T gam = T(a.gamma);
+    // A = 0 (a neighbour at r = h exactly: W = 0 on both sides) has no
+    // normal; the face carries no flux
+    if (!(dotT<D, T>(A, A) > T(0))) {
+        for (int k = 0; k < D + 2; ++k) F[k] = T(0);
+        return;
+    }
     T rs, us, Ps;
     int flag;
     if constexpr (sizeof(T) == 8) {
@@ -567,10 +570,11 @@ __device__ inline void face_finish(const HydroArgs &a, const FacePrep<D, T> &fp,
         vS[0] = K0 * us + K1 * t0;
         vS[1] = K2 * us + K3 * t0;
     } else {
-        const T *vup = flag == 1 ? &WL[2] : &WR[2];
         T un = flag == 1 ? uj : ui;
-        for (int ax = 0; ax < D; ++ax)
-            vS[ax] = flag == 0 ? us * hA[ax] : us * hA[ax] + (vup[ax] - un * hA[ax]);
+        for (int ax = 0; ax < D; ++ax) {
+            T vup = flag == 1 ? WL[2 + ax] : WR[2 + ax];
+            vS[ax] = flag == 0 ? us * hA[ax] : us * hA[ax] + (vup - un * hA[ax]);
+        }
     }
     // fluxes projected on A (Riemann.cpp:164-181, 3D :204-227)
     T vLab[D];
@@ -605,8 +609,6 @@ __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
     const bool live = i < a.N;
     FaceStats fs{};
     const int N = a.N;
-    PData<D, T> S, O, L, H;
-    if (live) load_pdata<D, T>(a, i, S);
     const double dtd = a.dt[0];
     const T dt = T(dtd);
     int cnt = live ? a.noi[i] + a.nog[i] : 0;
@@ -633,15 +635,8 @@ __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
                 for (int ax = 0; ax < D; ++ax) s[ax] = code_shift(code, ax);
             // side that solves the pair: the lower index (images: parent >= i)
             self_lo = !a.enforce_flux_sym || (image ? j >= i : j > i);
-            load_pdata<D, T>(a, j, O);
-            for (int ax = 0; ax < 3; ++ax) {
-                L.x[ax] = self_lo ? S.x[ax] : O.x[ax];
-                H.x[ax] = self_lo ? O.x[ax] : S.x[ax];
-            }
-            for (int k = 0; k < PData<D, T>::NT; ++k) {
-                L.t[k] = self_lo ? S.t[k] : O.t[k];
-                H.t[k] = self_lo ? O.t[k] : S.t[k];
-            }
+            const View<D, T> L = make_view<D, T>(a, self_lo ? i : j);
+            const View<D, T> H = make_view<D, T>(a, self_lo ? j : i);
             int sl[3];
             for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
             face_prepare<D, T>(a, L, H, sl, image, dt, fp, fs);

@@ -18,6 +18,7 @@ struct NbrArgs {
     int *noi, *nog;
     int *status;                 // bit 1: real overflow, bit 3: image overflow
     int *max_counts;             // [2]
+    int img_window;              // virtual-cell reach of the image search
 };
 
 // Arguments of the physics kernels.  Storage type of derived per-particle

@@ -22,10 +22,28 @@
 
 namespace mlh {
 
+// Fifth root in FP64: hardware log2/exp2 estimate (~1e-7), then two
+// Newton steps y <- (4y + x/y^4)/5 (error ~1e-14, then round-off).
+__device__ inline double root5(double x) {
+    if (!(x > 1e-30) || !(x < 1e30)) return pow(x, 0.2);
+    double y = (double)__builtin_amdgcn_exp2f(0.2f * __builtin_amdgcn_logf((float)x));
+    for (int k = 0; k < 2; ++k) {
+        double y2 = y * y;
+        double y4 = y2 * y2;
+        y = 0.8 * y + 0.2 * (x / y4);
+    }
+    return y;
+}
+__device__ inline float root5(float x) {
+    return __builtin_amdgcn_exp2f(0.2f * __builtin_amdgcn_logf(x));
+}
+
 template <typename T>
 struct GasConst {
     T g, g1, g2, g3, g4, g5, g6, g7, ginv;
+    bool g53;  // gamma == 5/3: g1 = 1/5, 1/gamma = 3/5, g3 = 5, g4 = 3
     __host__ __device__ explicit GasConst(double gam) {
+        g53 = gam == 5.0 / 3.0;
         g = T(gam);
         g1 = T((gam - 1.0) / (2.0 * gam));
         g2 = T((gam + 1.0) / (2.0 * gam));
@@ -43,6 +61,24 @@ __device__ inline double rpow(double x, double y) { return pow(x, y); }
 __device__ inline float rpow(float x, float y) {
     return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
 }
+// x^g1, x^(1/gamma), x^g4, x^g3 of the solver; closed forms for 5/3
+template <typename T>
+__device__ inline T pw_g1(const GasConst<T> &c, T x) { return c.g53 ? root5(x) : rpow(x, c.g1); }
+template <typename T>
+__device__ inline T pw_ginv(const GasConst<T> &c, T x) {
+    if (!c.g53) return rpow(x, c.ginv);
+    T r = root5(x);
+    return r * r * r;
+}
+template <typename T>
+__device__ inline T pw_g4(const GasConst<T> &c, T x) { return c.g53 ? x * x * x : rpow(x, c.g4); }
+template <typename T>
+__device__ inline T pw_g3(const GasConst<T> &c, T x) {
+    if (!c.g53) return rpow(x, c.g3);
+    T x2 = x * x;
+    return x2 * x2 * x;
+}
+
 __device__ inline double rsqrt_(double x) { return sqrt(x); }
 __device__ inline float rsqrt_(float x) { return __builtin_sqrtf(x); }
 
@@ -51,9 +87,11 @@ template <typename T>
 __device__ inline void prefun(const GasConst<T> &c, T p, T rk, T pk, T ck,
                               T &f, T &fd) {
     if (p <= pk) {
+        // (p/pk)^(-g2) = (p/pk)^g1 / (p/pk) since g2 = 1 - g1: one power
         T prat = p / pk;
-        f = c.g4 * ck * (rpow(prat, c.g1) - T(1));
-        fd = (T(1) / (rk * ck)) * rpow(prat, -c.g2);
+        T y = pw_g1(c, prat);
+        f = c.g4 * ck * (y - T(1));
+        fd = (T(1) / (rk * ck)) * (y / prat);
     } else {
         T ak = c.g5 / rk;
         T bk = c.g6 * pk;
@@ -77,11 +115,11 @@ __device__ inline T guess_p(const GasConst<T> &c, T dl, T ul, T pl, T cl, T dr,
     if (qmax <= T(2) && pmin <= ppv && ppv <= pmax) {
         pm = ppv;
     } else if (ppv < pmin) {
-        T pq = rpow(pl / pr, c.g1);
+        T pq = pw_g1(c, pl / pr);
         T um = (pq * ul / cl + ur / cr + c.g4 * (pq - T(1))) / (pq / cl + T(1) / cr);
         T ptl = T(1) + c.g7 * (ul - um) / cl;
         T ptr = T(1) + c.g7 * (um - ur) / cr;
-        pm = T(0.5) * (pl * rpow(ptl, c.g3) + pr * rpow(ptr, c.g3));
+        pm = T(0.5) * (pl * pw_g3(c, ptl) + pr * pw_g3(c, ptr));
     } else {
         T p0 = ppv > tiny ? ppv : tiny;
         T gel = rsqrt_((c.g5 / dl) / (c.g6 * pl + p0));
@@ -112,8 +150,8 @@ __device__ inline RSol<T> sample_vacuum(const GasConst<T> &c, T dl, T ul, T pl,
         if (s <= shl) return RSol<T>{dl, ul, pl, -1};
         if (s < stl) {
             T cc = c.g5 * (cl + c.g7 * (ul - s));
-            return RSol<T>{dl * rpow(cc / cl, c.g4), c.g5 * (cl + c.g7 * ul + s),
-                           pl * rpow(cc / cl, c.g3), -1};
+            return RSol<T>{dl * pw_g4(c, cc / cl), c.g5 * (cl + c.g7 * ul + s),
+                           pl * pw_g3(c, cc / cl), -1};
         }
         if (rv) return vac;
     }
@@ -122,8 +160,8 @@ __device__ inline RSol<T> sample_vacuum(const GasConst<T> &c, T dl, T ul, T pl,
     if (s >= shr) return RSol<T>{dr, ur, pr, 1};
     if (s > str) {
         T cc = c.g5 * (cr - c.g7 * (ur - s));
-        return RSol<T>{dr * rpow(cc / cr, c.g4), c.g5 * (-cr + c.g7 * ur + s),
-                       pr * rpow(cc / cr, c.g3), 1};
+        return RSol<T>{dr * pw_g4(c, cc / cr), c.g5 * (-cr + c.g7 * ur + s),
+                       pr * pw_g3(c, cc / cr), 1};
     }
     return vac;
 }
@@ -137,12 +175,12 @@ __device__ inline RSol<T> sample(const GasConst<T> &c, T pm, T um, T dl, T ul,
         if (pm <= pl) {
             T shl = ul - cl;
             if (s <= shl) return RSol<T>{dl, ul, pl, -1};
-            T cml = cl * rpow(pm / pl, c.g1);
+            T cml = cl * pw_g1(c, pm / pl);
             T stl = um - cml;
-            if (s > stl) return RSol<T>{dl * rpow(pm / pl, c.ginv), um, pm, -1};
+            if (s > stl) return RSol<T>{dl * pw_ginv(c, pm / pl), um, pm, -1};
             T cc = c.g5 * (cl + c.g7 * (ul - s));
-            return RSol<T>{dl * rpow(cc / cl, c.g4), c.g5 * (cl + c.g7 * ul + s),
-                           pl * rpow(cc / cl, c.g3), -1};
+            return RSol<T>{dl * pw_g4(c, cc / cl), c.g5 * (cl + c.g7 * ul + s),
+                           pl * pw_g3(c, cc / cl), -1};
         } else {
             T pml = pm / pl;
             T sl = ul - cl * rsqrt_(c.g2 * pml + c.g1);
@@ -158,12 +196,12 @@ __device__ inline RSol<T> sample(const GasConst<T> &c, T pm, T um, T dl, T ul,
         } else {
             T shr = ur + cr;
             if (s >= shr) return RSol<T>{dr, ur, pr, 1};
-            T cmr = cr * rpow(pm / pr, c.g1);
+            T cmr = cr * pw_g1(c, pm / pr);
             T str = um + cmr;
-            if (s <= str) return RSol<T>{dr * rpow(pm / pr, c.ginv), um, pm, 1};
+            if (s <= str) return RSol<T>{dr * pw_ginv(c, pm / pr), um, pm, 1};
             T cc = c.g5 * (cr - c.g7 * (ur - s));
-            return RSol<T>{dr * rpow(cc / cr, c.g4), c.g5 * (-cr + c.g7 * ur + s),
-                           pr * rpow(cc / cr, c.g3), 1};
+            return RSol<T>{dr * pw_g4(c, cc / cr), c.g5 * (-cr + c.g7 * ur + s),
+                           pr * pw_g3(c, cc / cr), 1};
         }
     }
 }
@@ -235,52 +273,52 @@ __device__ inline double star_pressure_fp64(const GasConst<double> &c, double dl
 }
 
 // FP32 Newton from p_seed, FP64 residual step, FP64 fallback.  Inputs are
-// the FP32 face states; p_seed <= 0 selects the standard FP32 guess.
+// the FP32 face states; p_seed <= 0 selects the standard FP32 guess.  One
+// call site of the FP64 solver (vacuum states and rejected faces share it).
 __device__ inline RSol<float> solve_mixed(const GasConst<float> &cf,
                                           const GasConst<double> &cd, float dl,
                                           float ul, float pl, float dr,
                                           float ur, float pr, double tol,
                                           int max_iter, int iters32,
                                           float p_seed, SolveStats &st) {
-    if (!(dl > 0.f) || !(dr > 0.f) ||
-        cf.g4 * (__builtin_sqrtf(cf.g * pl / dl) + __builtin_sqrtf(cf.g * pr / dr)) <= ur - ul) {
-        RSol<double> r = solve_fp64(cd, dl, ul, pl, dr, ur, pr, tol, max_iter, -1.0, st);
-        return RSol<float>{float(r.d), float(r.u), float(r.p), r.flag};
-    }
-    float cl = __builtin_sqrtf(cf.g * pl / dl), cr = __builtin_sqrtf(cf.g * pr / dr);
+    bool ok = dl > 0.f && dr > 0.f;
+    float cl = ok ? __builtin_sqrtf(cf.g * pl / dl) : 0.f;
+    float cr = ok ? __builtin_sqrtf(cf.g * pr / dr) : 0.f;
     float du = ur - ul;
-    float tiny = 1e-6f * 0.5f * (pl + pr);
-    float p = p_seed > 0.f ? p_seed : guess_p(cf, dl, ul, pl, cl, dr, ur, pr, cr, tiny);
-    int it = 0;
-    for (; it < iters32; ++it) {
-        float fl, fld, fr, frd;
-        prefun(cf, p, dl, pl, cl, fl, fld);
-        prefun(cf, p, dr, pr, cr, fr, frd);
-        float pn = p - (fl + fr + du) / (fld + frd);
-        if (!(pn > 0.f)) pn = tiny;
-        float ch = __builtin_fabsf(pn - p) / (pn + p);
-        p = pn;
-        if (ch < 2e-6f) { ++it; break; }
-    }
-    // FP64 residual: one Newton step at the FP32 root, accepted under the
-    // exact solver's relative-change test
-    double dld = dl, uld = ul, pld = pl, drd = dr, urd = ur, prd = pr;
-    double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
-    double pd = p, fl, fld, fr, frd;
-    prefun(cd, pd, dld, pld, cld, fl, fld);
-    prefun(cd, pd, drd, prd, crd, fr, frd);
-    double dp = (fl + fr + (urd - uld)) / (fld + frd);
-    double pn = pd - dp;
-    double change = 2.0 * fabs(dp / (pn + pd));
-    if (!(change <= tol) || !(pn > 0.0)) {
-        st.fallback += 1;
+    ok = ok && cf.g4 * (cl + cr) > du;
+    double pn = 0.0, um = 0.0;
+    if (ok) {
+        float tiny = 1e-6f * 0.5f * (pl + pr);
+        float p = p_seed > 0.f ? p_seed : guess_p(cf, dl, ul, pl, cl, dr, ur, pr, cr, tiny);
+        int it = 0;
+        for (; it < iters32; ++it) {
+            float fl, fld, fr, frd;
+            prefun(cf, p, dl, pl, cl, fl, fld);
+            prefun(cf, p, dr, pr, cr, fr, frd);
+            float q = p - (fl + fr + du) / (fld + frd);
+            if (!(q > 0.f)) q = tiny;
+            float ch = __builtin_fabsf(q - p) / (q + p);
+            p = q;
+            if (ch < 2e-6f) { ++it; break; }
+        }
+        // FP64 residual: one Newton step at the FP32 root, accepted under the
+        // exact solver's relative-change test
+        double dld = dl, pld = pl, drd = dr, prd = pr;
+        double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
+        double pd = p, fl, fld, fr, frd;
+        prefun(cd, pd, dld, pld, cld, fl, fld);
+        prefun(cd, pd, drd, prd, crd, fr, frd);
+        double dp = (fl + fr + ((double)ur - (double)ul)) / (fld + frd);
+        pn = pd - dp;
+        double change = 2.0 * fabs(dp / (pn + pd));
+        ok = change <= tol && pn > 0.0;
         st.iters += it + 1;
-        RSol<double> r = solve_fp64(cd, dld, uld, pld, drd, urd, prd, tol, max_iter, -1.0, st);
-        return RSol<float>{float(r.d), float(r.u), float(r.p), r.flag};
+        if (!ok) st.fallback += 1;
+        um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (frd - fld) * dp);
     }
-    st.iters += it + 1;
-    double um = 0.5 * (uld + urd + (fr - fl) - (frd - fld) * dp);
-    return sample(cf, float(pn), float(um), dl, ul, pl, cl, dr, ur, pr, cr);
+    if (ok) return sample(cf, float(pn), float(um), dl, ul, pl, cl, dr, ur, pr, cr);
+    RSol<double> r = solve_fp64(cd, dl, ul, pl, dr, ur, pr, tol, max_iter, -1.0, st);
+    return RSol<float>{float(r.d), float(r.u), float(r.p), r.flag};
 }
 
 }  // namespace mlh
