@@ -65,6 +65,13 @@ __device__ inline int nb_pos(const HydroArgs &a, uint32_t e, double *xj, int *s)
     return j;
 }
 
+template <typename T, int N>
+__device__ inline void load_vec(T *dst, const T *__restrict__ src) {
+    static_assert((N * sizeof(T)) % 16 == 0, "record size");
+    const T *s = (const T *)__builtin_assume_aligned(src, 16);
+    __builtin_memcpy(dst, s, N * sizeof(T));
+}
+
 __device__ inline unsigned long long wave_sum(unsigned long long v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
@@ -127,13 +134,22 @@ __device__ inline void invert(double *A) {
     for (int r = 0; r < D; ++r)
         for (int c = 0; c < 2 * D; ++c) M[r][c] = c < D ? A[r * D + c] : (c - D == r ? 1. : 0.);
     for (int k = 0; k < D; ++k) {
+        // first row with the largest |M[r][k]|, swapped with row k; rows are
+        // addressed with compile-time indices only (no scratch memory)
         int piv = k;
-        for (int r = k + 1; r < D; ++r)
-            if (fabs(M[r][k]) > fabs(M[piv][k])) piv = r;
-        if (piv != k)
+        double best = fabs(M[k][k]);
+        for (int r = k + 1; r < D; ++r) {
+            double v = fabs(M[r][k]);
+            if (v > best) { best = v; piv = r; }
+        }
+        for (int r = k + 1; r < D; ++r) {
+            bool sw = piv == r;
             for (int c = 0; c < 2 * D; ++c) {
-                double t = M[k][c]; M[k][c] = M[piv][c]; M[piv][c] = t;
+                double a0 = M[k][c], b0 = M[r][c];
+                M[k][c] = sw ? b0 : a0;
+                M[r][c] = sw ? a0 : b0;
             }
+        }
         double d = M[k][k];
         for (int c = 0; c < 2 * D; ++c) M[k][c] /= d;
         for (int r = 0; r < D; ++r) {
@@ -218,7 +234,8 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                 psi[p] = T(0);
                 for (int q = 0; q < D; ++q) psi[p] += B[p * D + q] * T(xj[q] - xi[q]) * w;
             }
-            const T *prj = prim + (size_t)j * kPrimN;
+            T prj[kPrimN];
+            load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
             for (int f = 0; f < NF; ++f) {
                 T fj = prj[field_slot<D>(f)];
                 for (int p = 0; p < D; ++p) grad[f][p] += (fj - fi[f]) * psi[p];
@@ -325,22 +342,24 @@ __global__ __launch_bounds__(256) void k_dt_finalize(const double *__restrict__ 
 }
 
 // -------------------------------------------------------------------- faces
-// A particle's face inputs, read from its records where they are used
-// (positions, primitive record, gradients + B): selecting the solving side
-// selects an index, not ~70 registers of data.
+// A particle's face inputs held in registers: position, primitive record
+// and face record, each read with 16-byte vector loads (the records are
+// 16-byte aligned).  The solving side is chosen by index, so no per-field
+// selects are needed.
 template <int D, typename T>
 struct View {
+    static constexpr int NP = kPrimN, NFR = FRec<D>::N;
     double x[3];
-    const T *p;  // primitive record
-    const T *f;  // face record
+    T p[NP];
+    T f[NFR];
 };
 template <int D, typename T>
 __device__ inline View<D, T> make_view(const HydroArgs &a, int j) {
     View<D, T> v;
     double4 q = a.pos[j];
     v.x[0] = q.x; v.x[1] = q.y; v.x[2] = q.z;
-    v.p = (const T *)a.prim + (size_t)j * kPrimN;
-    v.f = (const T *)a.frec + (size_t)j * FRec<D>::N;
+    load_vec<T, View<D, T>::NP>(v.p, (const T *)a.prim + (size_t)j * kPrimN);
+    load_vec<T, View<D, T>::NFR>(v.f, (const T *)a.frec + (size_t)j * FRec<D>::N);
     return v;
 }
 #define PD_RHO(d) (d).p[kPrimRho]

@@ -140,6 +140,14 @@ def test_stages_fp32(name, hip, oracle_mod):
     sim.fluxes(dt); orc.fluxes(dt)
     gm, gv, ge = sim.get_fluxes()
     om, ov, oe = orc.get_fluxes()
-    for a, b in ((gm, om), (gv, ov), (ge, oe)):
-        assert relerr(a, b) < 1e-4, (name, relerr(a, b))
+    # flux sums cancel (a steady region has net flux ~0), so the FP32 error
+    # is measured against the size of one face's flux: rho (|v| + c) h^(D-1)
+    st = orc.state()
+    vmax = max(np.max(np.abs(st[k])) for k in ("vx", "vy", "vz")[: p.dim])
+    cmax = np.max(np.sqrt(p.gamma * oP / orr))
+    fm = np.max(orr) * (vmax + cmax) * p.kernel_size ** (p.dim - 1)
+    for a, b, scale in ((gm, om, fm), (gv, ov, fm * (vmax + cmax) + np.max(oP) * p.kernel_size ** (p.dim - 1)),
+                        (ge, oe, fm * (vmax + cmax) ** 2 + np.max(oP) * (vmax + cmax) * p.kernel_size ** (p.dim - 1))):
+        err = np.max(np.abs(a - b)) / scale
+        assert err < 1e-5, (name, err)
     sim.close()
