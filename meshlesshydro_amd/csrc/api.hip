@@ -165,6 +165,7 @@ static HydroArgs hydro_args(mlh_ctx *c) {
     a.quirk3d = p.quirk_3d_predictor;
     a.write_fluxes = 0;
     a.learned = p.riemann_seed == MLH_SEED_LEARNED;
+    a.sidx = c->sorted_idx;
     a.pos = c->pos;
     a.nbr = c->nbr;
     a.noi = c->noi;
@@ -246,6 +247,11 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     hipMemsetAsync(c->max_counts, 0, sizeof(int) * 2, c->s);
     hipMemsetAsync(c->noi, 0, 4 * N, c->s);
     hipMemsetAsync(c->nog, 0, 4 * N, c->s);
+    // the cell order starts as the identity permutation, so that every
+    // kernel indexing through it stays in bounds before the first sort
+    std::vector<uint32_t> iota(N);
+    for (size_t q = 0; q < N; ++q) iota[q] = (uint32_t)q;
+    hipMemcpy(c->sorted_idx, iota.data(), 4 * N, hipMemcpyHostToDevice);
     if (hipStreamSynchronize(c->s) != hipSuccess) {
         mlh_destroy(c);
         return fail(MLH_ERR_DEVICE, "initialisation failed");
@@ -304,16 +310,15 @@ int mlh_download_state(mlh_ctx *c, double *x, double *y, double *z, double *vx, 
 // ---- stage launchers (asynchronous) ----------------------------------------
 static void enqueue_neighbours(mlh_ctx *c) {
     launch_cells(c->g, c->D, c->x[0], c->x[1], c->D == 3 ? c->x[2] : c->x[0], c->N, c->pos,
-                 c->cell_key, c->sorted_idx, c->sorted_key, nullptr, c->sort_scratch, c->cstart,
-                 c->cend, c->s);
+                 c->cell_key, c->sorted_idx, c->sorted_key, c->sort_scratch, c->cstart, c->cend,
+                 c->s);
     NbrArgs a{};
     a.g = c->g;
     a.N = c->N;
     a.maxr = c->p.max_interactions;
     a.maxg = c->p.max_ghost_interactions;
     a.pos = c->pos;
-    a.cell_key = c->cell_key;
-    a.sorted_idx = c->sorted_idx;
+    a.sorted_key = c->sorted_key;
     a.cstart = c->cstart;
     a.cend = c->cend;
     a.nbr = c->nbr;
@@ -323,6 +328,13 @@ static void enqueue_neighbours(mlh_ctx *c) {
     a.max_counts = c->max_counts;
     a.img_window = c->img_window;
     launch_neighbours(a, c->D, c->s);
+}
+
+// cell-order position -> caller's particle index
+static int read_order(mlh_ctx *c, std::vector<uint32_t> &perm) {
+    perm.resize(c->N);
+    HIPCHK(hipMemcpy(perm.data(), c->sorted_idx, 4 * (size_t)c->N, hipMemcpyDeviceToHost));
+    return MLH_OK;
 }
 
 static int check_status(mlh_ctx *c) {
@@ -534,14 +546,26 @@ int mlh_get_neighbours(mlh_ctx *c, int32_t *noi, int32_t *noi_ghost, int32_t *nb
     if (!c || !noi || !noi_ghost || !nbr) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
     if (c->stage < 1) return fail(MLH_ERR_CALL_ORDER, "no neighbour lists yet");
     size_t N = c->N;
-    HIPCHK(hipMemcpy(noi, c->noi, 4 * N, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(noi_ghost, c->nog, 4 * N, hipMemcpyDeviceToHost));
+    // device rows and entries are in cell order: map both back to the
+    // caller's particle indices
+    std::vector<uint32_t> perm;
+    int st = read_order(c, perm);
+    if (st) return st;
+    std::vector<int32_t> cn(N), cg(N);
+    HIPCHK(hipMemcpy(cn.data(), c->noi, 4 * N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cg.data(), c->nog, 4 * N, hipMemcpyDeviceToHost));
     std::vector<uint32_t> t((size_t)c->stride * N);
     HIPCHK(hipMemcpy(t.data(), c->nbr, 4 * t.size(), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < N; ++i) {
-        int cnt = noi[i] + noi_ghost[i];
-        for (int n = 0; n < c->stride; ++n)
-            nbr[i * c->stride + n] = n < cnt ? (int32_t)t[(size_t)n * N + i] : 0;
+    for (size_t q = 0; q < N; ++q) {
+        size_t i = perm[q];
+        noi[i] = cn[q];
+        noi_ghost[i] = cg[q];
+        int cnt = cn[q] + cg[q];
+        for (int n = 0; n < c->stride; ++n) {
+            uint32_t e = t[(size_t)n * N + q];
+            nbr[i * c->stride + n] =
+                n < cnt ? (int32_t)(perm[nb_index(e)] | (nb_code(e) << kIdxBits)) : 0;
+        }
     }
     return MLH_OK;
 }
@@ -571,10 +595,13 @@ int mlh_get_density(mlh_ctx *c, double *rho, double *P, double *omega) {
     std::vector<double> pr;
     int st = read_derived(c, c->prim, (size_t)c->N * kPrimN, pr);
     if (st) return st;
-    for (int i = 0; i < c->N; ++i) {
-        if (rho) rho[i] = pr[(size_t)i * kPrimN + kPrimRho];
-        if (P) P[i] = pr[(size_t)i * kPrimN + kPrimP];
-        if (omega) omega[i] = pr[(size_t)i * kPrimN + kPrimOmega];
+    std::vector<uint32_t> perm;
+    if ((st = read_order(c, perm))) return st;
+    for (int q = 0; q < c->N; ++q) {
+        size_t i = perm[q];
+        if (rho) rho[i] = pr[(size_t)q * kPrimN + kPrimRho];
+        if (P) P[i] = pr[(size_t)q * kPrimN + kPrimP];
+        if (omega) omega[i] = pr[(size_t)q * kPrimN + kPrimOmega];
     }
     return MLH_OK;
 }
@@ -586,9 +613,12 @@ int mlh_get_gradients(mlh_ctx *c, double *grads, double *B) {
     std::vector<double> fr;
     int st = read_derived(c, c->frec, (size_t)c->N * frn, fr);
     if (st) return st;
+    std::vector<uint32_t> perm;
+    if ((st = read_order(c, perm))) return st;
     size_t N = c->N;
-    for (size_t i = 0; i < N; ++i) {
-        const double *r = &fr[i * frn];
+    for (size_t q = 0; q < N; ++q) {
+        const double *r = &fr[q * frn];
+        size_t i = perm[q];
         if (grads)
             for (int f = 0; f < NF; ++f)
                 for (int a = 0; a < D; ++a) grads[((size_t)f * N + i) * D + a] = r[f * D + a];

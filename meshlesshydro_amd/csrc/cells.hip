@@ -12,17 +12,24 @@
 //      lane), and across the block's waves through LDS counts;
 //   3. per-cell [start, end) ranges of the sorted order (ascending particle
 //      index inside a cell, as the reference's push_back lists);
-//   4. one thread per particle, in cell order: the 3^D stencil of
+//   4. positions gathered into cell order as double4 (x, y, z, original
+//      index): every later kernel of the step works in this order, so that a
+//      wavefront's particles and their neighbours are a few adjacent cells;
+//   5. one thread per particle, in cell order: the 3^D stencil of
 //      non-periodic cells (x outer ... z inner, Domain.cpp:83-118) with the
 //      strict test d^2 < h^2 in the reference's operation order, then the
-//      periodic images: every candidate in the 5^D window of virtual cells
+//      periodic images: every candidate in the window of virtual cells
 //      around the particle that wraps in at least one axis, kept when the
 //      image exists under the reference's creation test and d^2 < h^2, and
-//      inserted in ghost-index order.  A periodic image within h of x_i is
-//      less than one cell size away per axis, so the +-2 window is
-//      complete: the result equals the reference's exhaustive ghost loop.
-// Neighbour rows are stored transposed (entry n of particle i at
-// nbr[n*N + i]) so that per-particle kernels read them coalesced.
+//      inserted in ghost-index order (original parent index, then axis
+//      subset).  A periodic image within h of x_i is less than one cell size
+//      away per axis, so the window is complete: the result equals the
+//      reference's exhaustive ghost loop.
+// Neighbour entries hold cell-order positions; rows are stored transposed
+// (entry n of the particle at cell-order position t at nbr[n*N + t]) so that
+// the writes here and the reads of the per-particle kernels are coalesced.
+// Inside one cell the cell order is ascending original index (stable sort of
+// the identity), so the reference's neighbour order is kept.
 #include <hip/hip_runtime.h>
 #include "device_common.h"
 #include "kernels.h"
@@ -40,8 +47,7 @@ template <int D>
 __global__ void k_assign_cells(Geom g, const double *__restrict__ x0,
                                const double *__restrict__ x1,
                                const double *__restrict__ x2, int N,
-                               double4 *__restrict__ pos, uint32_t *__restrict__ key,
-                               uint32_t *__restrict__ idx) {
+                               uint32_t *__restrict__ key, uint32_t *__restrict__ idx) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double xs[3] = {x0[i], x1[i], D == 3 ? x2[i] : 0.0};
@@ -54,9 +60,18 @@ __global__ void k_assign_cells(Geom g, const double *__restrict__ x0,
         c += f * mul;
         mul *= g.nc[a];
     }
-    pos[i] = make_double4(xs[0], xs[1], xs[2], 0.0);
     key[i] = (uint32_t)c;
     idx[i] = (uint32_t)i;
+}
+
+// positions in cell order; w = original index (exact in a double)
+__global__ void k_gather_pos(const double *__restrict__ x0, const double *__restrict__ x1,
+                             const double *__restrict__ x2, int D, int N,
+                             const uint32_t *__restrict__ sidx, double4 *__restrict__ pos) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    uint32_t i = sidx[t];
+    pos[t] = make_double4(x0[i], x1[i], D == 3 ? x2[i] : 0.0, (double)i);
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -245,14 +260,14 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
     if (t >= a.N) return;
     const Geom &g = a.g;
     const int N = a.N;
-    int i = (int)a.sorted_idx[t];
-    double4 pi = a.pos[i];
+    double4 pi = a.pos[t];
     double xi[3] = {pi.x, pi.y, pi.z};
-    uint32_t ck = a.cell_key[i];
+    uint32_t ck = a.sorted_key[t];
     int cc[3];
     cc[0] = (int)(ck % (uint32_t)g.nc[0]);
     cc[1] = (int)((ck / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
     cc[2] = D == 3 ? (int)(ck / (uint32_t)(g.nc[0] * g.nc[1])) : 0;
+    uint32_t *row = a.nbr + t;
     int cnt = 0, err = 0;
     // real neighbours, reference stencil order
     for (int k = cc[0] - 1; k <= cc[0] + 1 && !err; ++k) {
@@ -264,23 +279,22 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
                 int c = k + l * g.nc[0] + (D == 3 ? m * g.nc[0] * g.nc[1] : 0);
                 int q1 = a.cend[c];
                 for (int q = a.cstart[c]; q < q1; ++q) {
-                    int j = (int)a.sorted_idx[q];
-                    if (j == i) continue;
-                    double4 pj = a.pos[j];
+                    if (q == t) continue;
+                    double4 pj = a.pos[q];
                     double xj[3] = {pj.x, pj.y, pj.z};
                     double d = dist2<D>(xj, xi);
                     if (d < g.hsq) {
                         if (cnt >= a.maxr) { err = 1; break; }
-                        a.nbr[(size_t)cnt * N + i] = (uint32_t)j | (kCodeReal << kIdxBits);
+                        row[(size_t)cnt * N] = (uint32_t)q | (kCodeReal << kIdxBits);
                         ++cnt;
                     }
                 }
             }
         }
     }
-    a.noi[i] = cnt;
+    a.noi[t] = cnt;
     if (err) {  // rows stay valid: the entries written so far, no images
-        a.nog[i] = 0;
+        a.nog[t] = 0;
         atomicOr(a.status, 1 << 1);
         return;
     }
@@ -291,6 +305,7 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
     bool near_edge = false;
     for (int ax = 0; ax < D; ++ax) near_edge |= cc[ax] - W < 0 || cc[ax] + W >= g.nc[ax];
     if (g.periodic && near_edge) {
+        uint32_t *grow = row + (size_t)cnt * N;
         for (int vz = (D == 3 ? cc[2] - W : 0); vz <= (D == 3 ? cc[2] + W : 0) && !err; ++vz)
             for (int vy = cc[1] - W; vy <= cc[1] + W && !err; ++vy)
                 for (int vx = cc[0] - W; vx <= cc[0] + W && !err; ++vx) {
@@ -307,8 +322,7 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
                     uint32_t code = (uint32_t)((s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1));
                     int q1 = a.cend[c];
                     for (int q = a.cstart[c]; q < q1; ++q) {
-                        int j = (int)a.sorted_idx[q];
-                        double4 pj = a.pos[j];
+                        double4 pj = a.pos[q];
                         double xj[3] = {pj.x, pj.y, pj.z};
                         bool exists = true;
                         for (int ax = 0; ax < D; ++ax) {
@@ -322,53 +336,52 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
                         double d = dist2<D>(xj, xi);
                         if (!(d < g.hsq)) continue;
                         if (gcnt >= a.maxg) { err = 1; break; }
-                        uint32_t e = (uint32_t)j | (code << kIdxBits);
-                        uint64_t ke = ghost_key(e);
+                        uint64_t ke = ghost_key((uint32_t)pj.w, code);
                         int p = gcnt;
                         while (p > 0) {
-                            uint32_t prev = a.nbr[(size_t)(cnt + p - 1) * N + i];
-                            if (ghost_key(prev) <= ke) break;
-                            a.nbr[(size_t)(cnt + p) * N + i] = prev;
+                            uint32_t prev = grow[(size_t)(p - 1) * N];
+                            uint32_t pp = (uint32_t)a.pos[nb_index(prev)].w;
+                            if (ghost_key(pp, nb_code(prev)) <= ke) break;
+                            grow[(size_t)p * N] = prev;
                             --p;
                         }
-                        a.nbr[(size_t)(cnt + p) * N + i] = e;
+                        grow[(size_t)p * N] = (uint32_t)q | (code << kIdxBits);
                         ++gcnt;
                     }
                 }
         if (err) atomicOr(a.status, 1 << 3);
     }
-    a.nog[i] = gcnt;
+    a.nog[t] = gcnt;
     atomicMax(&a.max_counts[0], cnt);
     atomicMax(&a.max_counts[1], gcnt);
 }
 
 template __global__ void k_assign_cells<2>(Geom, const double *, const double *, const double *,
-                                           int, double4 *, uint32_t *, uint32_t *);
+                                           int, uint32_t *, uint32_t *);
 template __global__ void k_assign_cells<3>(Geom, const double *, const double *, const double *,
-                                           int, double4 *, uint32_t *, uint32_t *);
+                                           int, uint32_t *, uint32_t *);
 template __global__ void k_neighbours<2>(NbrArgs);
 template __global__ void k_neighbours<3>(NbrArgs);
 
 void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
                   const double *x2, int N, double4 *pos, uint32_t *key, uint32_t *idx,
-                  uint32_t *key2, uint32_t *idx2, uint32_t *scratch, int *cstart,
-                  int *cend, hipStream_t s) {
+                  uint32_t *skey, uint32_t *scratch, int *cstart, int *cend, hipStream_t s) {
     int nb = (N + 255) / 256;
     if (D == 2)
-        hipLaunchKernelGGL(k_assign_cells<2>, dim3(nb), dim3(256), 0, s, g, x0, x1, x2, N, pos, key, idx);
+        hipLaunchKernelGGL(k_assign_cells<2>, dim3(nb), dim3(256), 0, s, g, x0, x1, x2, N, key, idx);
     else
-        hipLaunchKernelGGL(k_assign_cells<3>, dim3(nb), dim3(256), 0, s, g, x0, x1, x2, N, pos, key, idx);
-    // sort a copy: key/idx stay per-particle (key[i] = cell of particle i)
-    hipMemcpyAsync(key2, key, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, s);
+        hipLaunchKernelGGL(k_assign_cells<3>, dim3(nb), dim3(256), 0, s, g, x0, x1, x2, N, key, idx);
+    // sort a copy: key[i] stays the cell of particle i
+    hipMemcpyAsync(skey, key, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, s);
     int bits = 1;
     while ((1 << bits) < g.ncells) ++bits;
     uint32_t *k1 = scratch, *v1 = scratch + N, *rs = scratch + 2 * (size_t)N;
-    radix_sort_pairs(key2, idx, k1, v1, N, bits, rs, s);
+    radix_sort_pairs(skey, idx, k1, v1, N, bits, rs, s);
     int nc = g.ncells;
     hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, s, cstart, 0, nc);
     hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, s, cend, 0, nc);
-    hipLaunchKernelGGL(k_cell_ranges, dim3(nb), dim3(256), 0, s, key2, N, cstart, cend);
-    (void)idx2;
+    hipLaunchKernelGGL(k_cell_ranges, dim3(nb), dim3(256), 0, s, skey, N, cstart, cend);
+    hipLaunchKernelGGL(k_gather_pos, dim3(nb), dim3(256), 0, s, x0, x1, x2, D, N, idx, pos);
 }
 
 void launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {

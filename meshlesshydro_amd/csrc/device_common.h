@@ -27,10 +27,10 @@ __host__ __device__ inline int code_mask(uint32_t code) {
         if (code_shift(code, a) != 0) m |= 1 << a;
     return m;
 }
-// ghost-index order of the reference: parent ascending, then the image
-// subsets of one parent in descending axis-mask order
-__host__ __device__ inline uint64_t ghost_key(uint32_t e) {
-    return (uint64_t)nb_index(e) * 8u + (uint64_t)(7 - code_mask(nb_code(e)));
+// ghost-index order of the reference: parent (original index) ascending,
+// then the image subsets of one parent in descending axis-mask order
+__host__ __device__ inline uint64_t ghost_key(uint32_t parent, uint32_t code) {
+    return (uint64_t)parent * 8u + (uint64_t)(7 - code_mask(code));
 }
 
 // Search geometry (Domain::createGrid, Domain.cpp:9-54).

@@ -1,10 +1,13 @@
 // hydro.hip -- MFV physics kernels: density, gradients + limiter + CFL step,
 // faces + exact Riemann + flux sums + state update.
 //
-// One thread per particle; neighbour rows come from cells.hip in the
+// One thread per particle, in cell order (position t; the particle's
+// original index is sidx[t]); neighbour rows come from cells.hip in the
 // reference's order (real neighbours in stencil order, then periodic images
 // in ghost-index order), so every per-particle sum below runs in the
-// reference's summation order.
+// reference's summation order.  Per-particle records (pos, prim, frec) are
+// read and written at cell-order positions; the state, its successor and
+// the flux sums at original indices.
 //
 // Faces are evaluated i-centric and without atomics: particle i visits each
 // of its neighbours and solves the pair from the side the reference solves
@@ -83,18 +86,19 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
 // for the time step (:1453).
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.N) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.N) return;
     const int N = a.N;
-    double4 pi = a.pos[i];
+    const int i = (int)a.sidx[t];
+    double4 pi = a.pos[t];
     double xi[3] = {pi.x, pi.y, pi.z};
     const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
-    int nr = a.noi[i], ng = a.nog[i];
+    int nr = a.noi[t], ng = a.nog[t];
     T omg = T(0);
     for (int n = 0; n < nr; ++n) {
         double xj[3];
         int s[3];
-        nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
+        nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
         T r = tsqrt(T(dist2<D>(xi, xj)));
         omg += kernel_w<T>(r, h2, sig);
     }
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
         for (int n = nr; n < nr + ng; ++n) {
             double xj[3];
             int s[3];
-            nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
+            nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
             T r = tsqrt(T(dist2<D>(xi, xj)));
             og += kernel_w<T>(r, h2, sig);
         }
@@ -113,7 +117,7 @@ __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
     double rho = a.m[i] * (double)om;
     double P = (a.gamma - 1.) * rho * a.u[i];
     double c = sqrt(a.gamma * P / rho);
-    T *pr = (T *)a.prim + (size_t)i * kPrimN;
+    T *pr = (T *)a.prim + (size_t)t * kPrimN;
     T rec[kPrimN];
     rec[kPrimRho] = T(rho);
     rec[kPrimV + 0] = T(a.v[0][i]);
@@ -171,19 +175,20 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
     constexpr int NF = D + 2;
     constexpr int FRN = FRec<D>::N;
     __shared__ double sdt[256];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
     double dt_i = DBL_MAX;
-    if (i < a.N) {
+    if (t < a.N) {
         const int N = a.N;
-        double4 pi = a.pos[i];
+        const int i = (int)a.sidx[t];
+        double4 pi = a.pos[t];
         double xi[3] = {pi.x, pi.y, pi.z};
         const T *prim = (const T *)a.prim;
-        const T *pri = prim + (size_t)i * kPrimN;
+        const T *pri = prim + (size_t)t * kPrimN;
         T om_i = pri[kPrimOmega];
         T fi[NF];
         for (int f = 0; f < NF; ++f) fi[f] = pri[field_slot<D>(f)];
         const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
-        int cnt = a.noi[i] + a.nog[i], nr = a.noi[i];
+        int cnt = a.noi[t] + a.nog[t], nr = a.noi[t];
         // pass 1: E_i and the signal speed (real neighbours only)
         double E[D * D];
         for (int k = 0; k < D * D; ++k) E[k] = 0.;
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
         for (int n = 0; n < cnt; ++n) {
             double xj[3];
             int s[3];
-            int j = nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
+            int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
             T r = tsqrt(T(dist2<D>(xi, xj)));
             double w = (double)(kernel_w<T>(r, h2, sig) / om_i);
             for (int p = 0; p < D; ++p)
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
         for (int n = 0; n < cnt; ++n) {
             double xj[3];
             int s[3];
-            int j = nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
+            int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
             T r = tsqrt(T(dist2<D>(xi, xj)));
             T w = kernel_w<T>(r, h2, sig) / om_i;
             T psi[D];
@@ -256,7 +261,7 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
             for (int n = 0; n < cnt; ++n) {
                 double xj[3];
                 int s[3];
-                nb_pos<D>(a, a.nbr[(size_t)n * N + i], xj, s);
+                nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
                 double xijxi[D];
                 for (int ax = 0; ax < D; ++ax) {
                     double xij = a.foqp ? (xi[ax] + xj[ax]) / 2.
@@ -283,7 +288,7 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                 }
             }
         }
-        T *fr = (T *)a.frec + (size_t)i * FRN;
+        T *fr = (T *)a.frec + (size_t)t * FRN;
         for (int f = 0; f < NF; ++f)
             for (int ax = 0; ax < D; ++ax) fr[FRec<D>::G + f * D + ax] = grad[f][ax];
         for (int k = 0; k < D * D; ++k) fr[FRec<D>::B + k] = B[k];
@@ -624,13 +629,14 @@ template <int D, typename T, bool LEARNED>
 __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
     constexpr int NF = D + 2;
     __shared__ uint16_t lds[LEARNED ? 4 : 1][LEARNED ? 64 * 16 : 1];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < a.N;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < a.N;
     FaceStats fs{};
     const int N = a.N;
+    const int i = live ? (int)a.sidx[t] : 0;
     const double dtd = a.dt[0];
     const T dt = T(dtd);
-    int cnt = live ? a.noi[i] + a.nog[i] : 0;
+    int cnt = live ? a.noi[t] + a.nog[t] : 0;
     int lim = cnt;
     if (LEARNED) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -645,17 +651,19 @@ __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
         FacePrep<D, T> fp;
         bool self_lo = true;
         if (act) {
-            uint32_t e = a.nbr[(size_t)n * N + i];
+            uint32_t e = a.nbr[(size_t)n * N + t];
             int j = (int)nb_index(e);
             uint32_t code = nb_code(e);
             bool image = code != kCodeReal;
             int s[3] = {0, 0, 0};
             if (image)
                 for (int ax = 0; ax < D; ++ax) s[ax] = code_shift(code, ax);
-            // side that solves the pair: the lower index (images: parent >= i)
-            self_lo = !a.enforce_flux_sym || (image ? j >= i : j > i);
-            const View<D, T> L = make_view<D, T>(a, self_lo ? i : j);
-            const View<D, T> H = make_view<D, T>(a, self_lo ? j : i);
+            // side that solves the pair: the lower original index (images:
+            // parent >= i)
+            const int jo = (int)a.pos[j].w;
+            self_lo = !a.enforce_flux_sym || (image ? jo >= i : jo > i);
+            const View<D, T> L = make_view<D, T>(a, self_lo ? t : j);
+            const View<D, T> H = make_view<D, T>(a, self_lo ? j : t);
             int sl[3];
             for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
             face_prepare<D, T>(a, L, H, sl, image, dt, fp, fs);
@@ -763,10 +771,11 @@ __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_sums(HydroArgs a, double *out) {
     __shared__ double sh[6][256];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
     double v[6] = {0, 0, 0, 0, 0, 0};
-    if (i < a.N) {
-        const T *pr = (const T *)a.prim + (size_t)i * kPrimN;
+    if (t < a.N) {
+        const int i = (int)a.sidx[t];
+        const T *pr = (const T *)a.prim + (size_t)t * kPrimN;
         double om = (double)pr[kPrimOmega];
         double m = a.m[i];
         double vx = a.v[0][i], vy = a.v[1][i], vz = D == 3 ? a.v[2][i] : 0.;

@@ -10,22 +10,24 @@ namespace mlh {
 struct NbrArgs {
     Geom g;
     int N, maxr, maxg;
-    const double4 *pos;
-    const uint32_t *cell_key;    // cell of particle i
-    const uint32_t *sorted_idx;  // particles in cell order
-    const int *cstart, *cend;
-    uint32_t *nbr;               // [maxr+maxg][N]
-    int *noi, *nog;
+    const double4 *pos;          // cell order: x, y, z, original index
+    const uint32_t *sorted_key;  // cell of the particle at each cell-order position
+    const int *cstart, *cend;    // cell -> [start, end) of the cell order
+    uint32_t *nbr;               // [maxr+maxg][N], cell-order rows and entries
+    int *noi, *nog;              // per cell-order position
     int *status;                 // bit 1: real overflow, bit 3: image overflow
     int *max_counts;             // [2]
     int img_window;              // virtual-cell reach of the image search
 };
 
 // Arguments of the physics kernels.  Storage type of derived per-particle
-// data (prim, frec) is T; positions / state are FP64.
+// data (prim, frec) is T; positions / state are FP64.  Threads, neighbour
+// rows, pos, prim and frec are in cell order; the state arrays, the next
+// state and the flux sums are in the caller's particle order (sidx maps).
 struct HydroArgs {
     Geom g;
     int N, stride;               // stride = rows of nbr (N)
+    const uint32_t *sidx;        // cell-order position -> original index
     double gamma, cfl, beta, psi1, psi2, riemann_tol;
     int riemann_max_iter, learned_iters;
     int limiter_abs, slope_limiting, pairwise_limiter, mfm, enforce_flux_sym;
@@ -57,8 +59,7 @@ enum CounterIdx {
 
 void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
                   const double *x2, int N, double4 *pos, uint32_t *key, uint32_t *idx,
-                  uint32_t *key2, uint32_t *idx2, uint32_t *scratch, int *cstart,
-                  int *cend, hipStream_t s);
+                  uint32_t *skey, uint32_t *scratch, int *cstart, int *cend, hipStream_t s);
 void launch_neighbours(const NbrArgs &a, int D, hipStream_t s);
 size_t radix_sort_scratch_words(int N);
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *tmp,
