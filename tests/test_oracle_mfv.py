@@ -162,9 +162,55 @@ def test_gradient_exact_for_linear_fields(oracle_mod):
     G, B = o.get_gradients()
     inner = (X > 0.2) & (X < 0.8) & (Y > 0.2) & (Y < 0.8)
     assert np.max(np.abs(G[1][inner] - a)) < 1e-10
-    # B_i is the inverse of E_i = sum (xj-xi)(xj-xi)^T W/omega: B symmetric
-    # positive definite; check symmetry and the inverse via a rebuilt E
+    # B_i is the inverse of E_i = sum (xj-xi)(xj-xi)^T W/omega: symmetric,
+    # and B_i E_i = I with E_i rebuilt here from the neighbour lists (real
+    # entries and periodic images decoded from the packed image code)
     assert np.max(np.abs(B - B.transpose(0, 2, 1))) < 1e-9 * np.max(np.abs(B))
+    noi, nog, nl = o.get_neighbours()
+    _, _, om = o.get_density()
+    h = p.kernel_size
+    pos = np.stack([X, Y], 1)
+    worst = 0.0
+    for i in range(0, n * n, 7):
+        E = np.zeros((2, 2))
+        for e in nl[i, : noi[i] + nog[i]]:
+            e = int(e) & 0xFFFFFFFF  # int32 storage of the packed entry
+            j, code = e & ((1 << 27) - 1), e >> 27
+            sh = np.array([code % 3 - 1, (code // 3) % 3 - 1])
+            xj = pos[j] + sh  # unit box: image = x +- (max - min)
+            d = xj - pos[i]
+            E += np.outer(d, d) * oracle_mod.kernel(2, np.sqrt(d @ d), h) / om[i]
+        worst = max(worst, np.max(np.abs(B[i] @ E - np.eye(2))))
+    assert worst < 1e-10, worst
+
+
+def test_timestep_closed_form(oracle_mod):
+    """compGlobalTimestep (Particles.cpp:1446-1485) on a 3D lattice with
+    h = 2.4 dx (56 neighbours, largest offset sqrt(5) dx).  At rest every
+    signal speed is 2c.  Under the compression v = -a (x - 1/2) every real
+    pair approaches at a |x_ij|, so the largest signal speed is
+    2c + a sqrt(5) dx."""
+    n = 10
+    dx = 1.0 / n
+    p = default_params(dim=3, kernel_size=2.4 * dx, max_interactions=96, max_ghost_interactions=96)
+    s = ics.blast_3d(n)
+    s["u"] = np.full(n ** 3, 0.9)
+    o = oracle_mod.Oracle(p, s)
+    o.neighbours()
+    o.density()
+    rho, P, _ = o.get_density()
+    c = np.sqrt(p.gamma * P / rho)
+    assert np.ptp(c) < 1e-12 * c[0]
+    dt0 = o.timestep()
+    assert dt0 == pytest.approx(p.cfl * p.kernel_size / (2 * c[0]), rel=1e-14)
+    a = 3.0
+    for k, v in (("x", "vx"), ("y", "vy"), ("z", "vz")):
+        s[v] = -a * (s[k] - 0.5)
+    o = oracle_mod.Oracle(p, s)
+    o.neighbours()
+    o.density()
+    dt1 = o.timestep()
+    assert dt1 == pytest.approx(p.cfl * p.kernel_size / (2 * c[0] + a * np.sqrt(5.0) * dx), rel=1e-12)
 
 
 def test_free_stream_preservation(oracle_mod):
