@@ -62,6 +62,10 @@ def main():
                          "to scale FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--only", choices=["fp64", "fp32", "learned"], default=None,
                     help="profile helper: run one mode for --steps steps and print its timings")
+    ap.add_argument("--order", choices=["auto", "cell", "caller"], default="auto",
+                    help="work order of the kernels (results are identical)")
+    ap.add_argument("--shuffle", action="store_true",
+                    help="randomly permute the particle order of the initial state")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this MFV step has no multi-GPU path "
@@ -86,8 +90,12 @@ def main():
     N = n ** 3
     W, K = args.warmup, args.steps
     state = ics.blast_3d(n)
+    if args.shuffle:
+        perm = np.random.default_rng(77).permutation(N)
+        state = {k: v[perm] for k, v in state.items()}
+    order = {"auto": 0, "cell": 1, "caller": 2}[args.order]
     base = dict(dim=3, kernel_size=ics.blast_kernel_size(n), max_interactions=96,
-                max_ghost_interactions=96, debug_level=0)
+                max_ghost_interactions=96, debug_level=0, work_order=order)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
 
     if args.only:

@@ -65,6 +65,17 @@ typedef enum mlh_limiter_abs {
     MLH_ABS_FABS = 1
 } mlh_limiter_abs;
 
+typedef enum mlh_work_order {
+    /* Order in which the kernels visit particles.  Results do not depend on
+     * it (sums run in the reference's neighbour order, faces are solved by
+     * the lower original index); only memory locality does. */
+    MLH_ORDER_AUTO = 0,    /* caller's order when consecutive particles are  *
+                            * mostly in adjacent search cells (lattices,     *
+                            * jittered lattices), else cell order            */
+    MLH_ORDER_CELL = 1,    /* particles sorted by search cell                */
+    MLH_ORDER_CALLER = 2   /* the caller's particle order                    */
+} mlh_work_order;
+
 /* Compile-time flags of parameter.h plus the config.info values, as one
  * runtime struct.  Fill with mlh_params_default() and override. */
 typedef struct mlh_params {
@@ -93,6 +104,7 @@ typedef struct mlh_params {
     double riemann_tol;             /* relative Newton step accepted as converged */
     int32_t riemann_max_iter;       /* FP64 Newton iteration cap              */
     int32_t learned_iters;          /* FP32 Newton iterations after the MLP seed */
+    int32_t work_order;             /* mlh_work_order                         */
 } mlh_params;
 
 /* Counters of soft conditions the reference only logs (WARN lines). */

@@ -23,6 +23,7 @@ MLH_SEED_STANDARD = 0
 MLH_SEED_LEARNED = 1
 MLH_ABS_INT_X86 = 0
 MLH_ABS_FABS = 1
+MLH_ORDER_AUTO, MLH_ORDER_CELL, MLH_ORDER_CALLER = 0, 1, 2
 
 MLP_IN = 10
 MLP_HIDDEN = 64
@@ -57,6 +58,7 @@ class Params(C.Structure):
         ("riemann_tol", C.c_double),
         ("riemann_max_iter", C.c_int32),
         ("learned_iters", C.c_int32),
+        ("work_order", C.c_int32),
     ]
 
 
@@ -110,6 +112,7 @@ def default_params(**kw):
     p.riemann_tol = 1e-6
     p.riemann_max_iter = 50
     p.learned_iters = 3
+    p.work_order = MLH_ORDER_AUTO
     for k, v in kw.items():
         if k in ("box_min", "box_max"):
             for a, x in enumerate(v):

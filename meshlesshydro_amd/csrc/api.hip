@@ -36,7 +36,10 @@ struct mlh_ctx {
     Geom g{};
     double *x[3] = {}, *v[3] = {}, *m = nullptr, *u = nullptr;
     double *xn[3] = {}, *vn[3] = {}, *mn = nullptr, *un = nullptr;
-    double4 *pos = nullptr;
+    double4 *pos = nullptr;    // cell order
+    double4 *pos_w = nullptr;  // caller order (work order = caller order)
+    unsigned long long *coh = nullptr;
+    int order = 0;             // resolved work order (MLH_ORDER_CELL / _CALLER), 0 = not yet
     uint32_t *cell_key = nullptr, *sorted_idx = nullptr, *sorted_key = nullptr,
              *sort_scratch = nullptr;
     int *cstart = nullptr, *cend = nullptr;
@@ -87,6 +90,7 @@ void mlh_params_default(mlh_params *p) {
     p->riemann_tol = 1e-6;
     p->riemann_max_iter = 50;
     p->learned_iters = 3;
+    p->work_order = MLH_ORDER_AUTO;
 }
 
 int mlh_version(void) { return MLH_ABI_VERSION; }
@@ -165,8 +169,9 @@ static HydroArgs hydro_args(mlh_ctx *c) {
     a.quirk3d = p.quirk_3d_predictor;
     a.write_fluxes = 0;
     a.learned = p.riemann_seed == MLH_SEED_LEARNED;
-    a.sidx = c->sorted_idx;
-    a.pos = c->pos;
+    const bool co = c->order != MLH_ORDER_CELL;
+    a.sidx = co ? nullptr : c->sorted_idx;
+    a.pos = co ? c->pos_w : c->pos;
     a.nbr = c->nbr;
     a.noi = c->noi;
     a.nog = c->nog;
@@ -202,6 +207,8 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
         return fail(MLH_ERR_INVALID_ARGUMENT, "bad interaction capacities");
     if (p->precision != MLH_FP64 && p->precision != MLH_FP32)
         return fail(MLH_ERR_INVALID_ARGUMENT, "bad precision");
+    if (p->work_order < MLH_ORDER_AUTO || p->work_order > MLH_ORDER_CALLER)
+        return fail(MLH_ERR_INVALID_ARGUMENT, "bad work_order");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(MLH_ERR_DEVICE, "no HIP device");
@@ -227,6 +234,8 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     }
     A_(&c->m, dsz); A_(&c->u, dsz); A_(&c->mn, dsz); A_(&c->un, dsz);
     A_(&c->pos, sizeof(double4) * N);
+    A_(&c->pos_w, sizeof(double4) * N);
+    A_(&c->coh, sizeof(unsigned long long));
     A_(&c->cell_key, 4 * N); A_(&c->sorted_idx, 4 * N); A_(&c->sorted_key, 4 * N);
     A_(&c->sort_scratch, 4 * (2 * N + radix_sort_scratch_words(n)));
     A_(&c->cstart, 4 * (size_t)c->g.ncells); A_(&c->cend, 4 * (size_t)c->g.ncells);
@@ -287,6 +296,7 @@ int mlh_upload_state(mlh_ctx *c, const double *x, const double *y, const double 
     HIPCHK(hipMemcpyAsync(c->u, u, b, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     c->stage = 0;
+    if (c->p.work_order == MLH_ORDER_AUTO) c->order = 0;  // decided at the next search
     return MLH_OK;
 }
 
@@ -308,17 +318,40 @@ int mlh_download_state(mlh_ctx *c, double *x, double *y, double *z, double *vx, 
 }  // extern "C"
 
 // ---- stage launchers (asynchronous) ----------------------------------------
+// MLH_ORDER_AUTO: the caller's order when at least 3/4 of consecutive
+// particles sit in adjacent search cells (one device reduction and one
+// synchronisation per upload), else cell order
+static void resolve_order(mlh_ctx *c) {
+    if (c->p.work_order != MLH_ORDER_AUTO) {
+        c->order = c->p.work_order;
+        return;
+    }
+    unsigned long long n = 0;
+    hipMemsetAsync(c->coh, 0, sizeof(n), c->s);
+    launch_order_coherence(c->g, c->D, c->cell_key, c->N, c->coh, c->s);
+    hipMemcpyAsync(&n, c->coh, sizeof(n), hipMemcpyDeviceToHost, c->s);
+    hipStreamSynchronize(c->s);
+    c->order = (double)n >= 0.75 * (double)(c->N - 1) ? MLH_ORDER_CALLER : MLH_ORDER_CELL;
+}
+
 static void enqueue_neighbours(mlh_ctx *c) {
     launch_cells(c->g, c->D, c->x[0], c->x[1], c->D == 3 ? c->x[2] : c->x[0], c->N, c->pos,
                  c->cell_key, c->sorted_idx, c->sorted_key, c->sort_scratch, c->cstart, c->cend,
                  c->s);
+    if (c->order == 0) resolve_order(c);
+    const bool co = c->order == MLH_ORDER_CALLER;
+    if (co)
+        launch_pack_pos(c->x[0], c->x[1], c->D == 3 ? c->x[2] : c->x[0], c->D, c->N, c->pos_w,
+                        c->s);
     NbrArgs a{};
     a.g = c->g;
     a.N = c->N;
     a.maxr = c->p.max_interactions;
     a.maxg = c->p.max_ghost_interactions;
-    a.pos = c->pos;
-    a.sorted_key = c->sorted_key;
+    a.pos_cell = c->pos;
+    a.pos_w = co ? c->pos_w : c->pos;
+    a.ck_w = co ? c->cell_key : c->sorted_key;
+    a.caller_order = co ? 1 : 0;
     a.cstart = c->cstart;
     a.cend = c->cend;
     a.nbr = c->nbr;
@@ -330,9 +363,13 @@ static void enqueue_neighbours(mlh_ctx *c) {
     launch_neighbours(a, c->D, c->s);
 }
 
-// cell-order position -> caller's particle index
+// work position -> caller's particle index
 static int read_order(mlh_ctx *c, std::vector<uint32_t> &perm) {
     perm.resize(c->N);
+    if (c->order != MLH_ORDER_CELL) {
+        for (int q = 0; q < c->N; ++q) perm[q] = (uint32_t)q;
+        return MLH_OK;
+    }
     HIPCHK(hipMemcpy(perm.data(), c->sorted_idx, 4 * (size_t)c->N, hipMemcpyDeviceToHost));
     return MLH_OK;
 }

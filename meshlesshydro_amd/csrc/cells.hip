@@ -254,15 +254,20 @@ __device__ inline double dist2(const double *a, const double *b) {
     return d;
 }
 
+// One thread per work position t.  Candidates are scanned in cell order
+// (pos_cell); an entry stores the candidate's work position: its cell-order
+// position, or its original index when the work order is the caller's.
 template <int D>
 __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= a.N) return;
     const Geom &g = a.g;
     const int N = a.N;
-    double4 pi = a.pos[t];
+    const bool co = a.caller_order != 0;
+    double4 pi = a.pos_w[t];
     double xi[3] = {pi.x, pi.y, pi.z};
-    uint32_t ck = a.sorted_key[t];
+    const uint32_t io = (uint32_t)pi.w;
+    uint32_t ck = a.ck_w[t];
     int cc[3];
     cc[0] = (int)(ck % (uint32_t)g.nc[0]);
     cc[1] = (int)((ck / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
@@ -279,13 +284,14 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
                 int c = k + l * g.nc[0] + (D == 3 ? m * g.nc[0] * g.nc[1] : 0);
                 int q1 = a.cend[c];
                 for (int q = a.cstart[c]; q < q1; ++q) {
-                    if (q == t) continue;
-                    double4 pj = a.pos[q];
+                    double4 pj = a.pos_cell[q];
+                    const uint32_t jo = (uint32_t)pj.w;
+                    if (jo == io) continue;
                     double xj[3] = {pj.x, pj.y, pj.z};
                     double d = dist2<D>(xj, xi);
                     if (d < g.hsq) {
                         if (cnt >= a.maxr) { err = 1; break; }
-                        row[(size_t)cnt * N] = (uint32_t)q | (kCodeReal << kIdxBits);
+                        row[(size_t)cnt * N] = (co ? jo : (uint32_t)q) | (kCodeReal << kIdxBits);
                         ++cnt;
                     }
                 }
@@ -322,7 +328,7 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
                     uint32_t code = (uint32_t)((s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1));
                     int q1 = a.cend[c];
                     for (int q = a.cstart[c]; q < q1; ++q) {
-                        double4 pj = a.pos[q];
+                        double4 pj = a.pos_cell[q];
                         double xj[3] = {pj.x, pj.y, pj.z};
                         bool exists = true;
                         for (int ax = 0; ax < D; ++ax) {
@@ -336,16 +342,17 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
                         double d = dist2<D>(xj, xi);
                         if (!(d < g.hsq)) continue;
                         if (gcnt >= a.maxg) { err = 1; break; }
-                        uint64_t ke = ghost_key((uint32_t)pj.w, code);
+                        const uint32_t jo = (uint32_t)pj.w;
+                        uint64_t ke = ghost_key(jo, code);
                         int p = gcnt;
                         while (p > 0) {
                             uint32_t prev = grow[(size_t)(p - 1) * N];
-                            uint32_t pp = (uint32_t)a.pos[nb_index(prev)].w;
+                            uint32_t pp = (uint32_t)a.pos_w[nb_index(prev)].w;
                             if (ghost_key(pp, nb_code(prev)) <= ke) break;
                             grow[(size_t)p * N] = prev;
                             --p;
                         }
-                        grow[(size_t)p * N] = (uint32_t)q | (code << kIdxBits);
+                        grow[(size_t)p * N] = (co ? jo : (uint32_t)q) | (code << kIdxBits);
                         ++gcnt;
                     }
                 }
@@ -360,6 +367,39 @@ template __global__ void k_assign_cells<2>(Geom, const double *, const double *,
                                            int, uint32_t *, uint32_t *);
 template __global__ void k_assign_cells<3>(Geom, const double *, const double *, const double *,
                                            int, uint32_t *, uint32_t *);
+// positions in the caller's order (work order = caller order)
+__global__ void k_pack_pos(const double *__restrict__ x0, const double *__restrict__ x1,
+                           const double *__restrict__ x2, int D, int N,
+                           double4 *__restrict__ pos) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    pos[i] = make_double4(x0[i], x1[i], D == 3 ? x2[i] : 0.0, (double)i);
+}
+
+// number of i in [0, N-1) whose successor i+1 lies in the 3^D stencil of
+// i's search cell: the spatial coherence of the caller's order
+__global__ __launch_bounds__(256) void k_order_coherence(Geom g, int D,
+                                                         const uint32_t *__restrict__ key,
+                                                         int N, unsigned long long *count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long hit = 0;
+    if (i + 1 < N) {
+        uint32_t a = key[i], b = key[i + 1];
+        bool adj = true;
+        uint32_t div = 1;
+        for (int ax = 0; ax < D; ++ax) {
+            int ca = (int)((a / div) % (uint32_t)g.nc[ax]);
+            int cb = (int)((b / div) % (uint32_t)g.nc[ax]);
+            int d = ca - cb;
+            adj &= d >= -1 && d <= 1;
+            div *= (uint32_t)g.nc[ax];
+        }
+        hit = adj ? 1ull : 0ull;
+    }
+    for (int off = 32; off > 0; off >>= 1) hit += __shfl_xor(hit, off, 64);
+    if (lane_id() == 0 && hit) atomicAdd(count, hit);
+}
+
 template __global__ void k_neighbours<2>(NbrArgs);
 template __global__ void k_neighbours<3>(NbrArgs);
 
@@ -382,6 +422,17 @@ void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
     hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, s, cend, 0, nc);
     hipLaunchKernelGGL(k_cell_ranges, dim3(nb), dim3(256), 0, s, skey, N, cstart, cend);
     hipLaunchKernelGGL(k_gather_pos, dim3(nb), dim3(256), 0, s, x0, x1, x2, D, N, idx, pos);
+}
+
+void launch_pack_pos(const double *x0, const double *x1, const double *x2, int D, int N,
+                     double4 *pos, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_pos, dim3((N + 255) / 256), dim3(256), 0, s, x0, x1, x2, D, N, pos);
+}
+
+void launch_order_coherence(const Geom &g, int D, const uint32_t *key, int N,
+                            unsigned long long *count, hipStream_t s) {
+    hipLaunchKernelGGL(k_order_coherence, dim3((N + 255) / 256), dim3(256), 0, s, g, D, key, N,
+                       count);
 }
 
 void launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {

@@ -86,6 +86,17 @@ struct FRec {
 
 __device__ inline int lane_id() { return __lane_id(); }
 
+// XCD-aware block index.  Workgroups are dispatched round-robin over the 8
+// XCDs of an MI355X (workgroup w runs on XCD w % 8); this bijection on
+// [0, gridDim.x) gives each XCD one contiguous range of logical blocks, so
+// that neighbouring particles, whose gathers overlap, share one L2.
+constexpr int kNumXcd = 8;
+__device__ inline int xcd_block() {
+    const int B = (int)gridDim.x, w = (int)blockIdx.x;
+    const int q = B / kNumXcd, r = B % kNumXcd, x = w % kNumXcd, k = w / kNumXcd;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // abs() of the reference's slope / pairwise limiter (see mlh_limiter_abs).
 template <typename T>
 __device__ inline T lim_abs(T x, int mode) {

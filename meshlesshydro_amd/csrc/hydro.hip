@@ -1,13 +1,14 @@
 // hydro.hip -- MFV physics kernels: density, gradients + limiter + CFL step,
 // faces + exact Riemann + flux sums + state update.
 //
-// One thread per particle, in cell order (position t; the particle's
-// original index is sidx[t]); neighbour rows come from cells.hip in the
-// reference's order (real neighbours in stencil order, then periodic images
-// in ghost-index order), so every per-particle sum below runs in the
-// reference's summation order.  Per-particle records (pos, prim, frec) are
-// read and written at cell-order positions; the state, its successor and
-// the flux sums at original indices.
+// One thread per particle, in work order (position t; the particle's
+// original index is sidx[t], or t when the work order is the caller's);
+// neighbour rows come from cells.hip in the reference's order (real
+// neighbours in stencil order, then periodic images in ghost-index order),
+// so every per-particle sum below runs in the reference's summation order.
+// Per-particle records (pos, prim, frec) are read and written at work
+// positions; the state, its successor and the flux sums at original
+// indices.  Blocks are mapped XCD-aware (xcd_block).
 //
 // Faces are evaluated i-centric and without atomics: particle i visits each
 // of its neighbours and solves the pair from the side the reference solves
@@ -86,10 +87,10 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
 // for the time step (:1453).
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_block() * blockDim.x + threadIdx.x;
     if (t >= a.N) return;
     const int N = a.N;
-    const int i = (int)a.sidx[t];
+    const int i = a.sidx ? (int)a.sidx[t] : t;
     double4 pi = a.pos[t];
     double xi[3] = {pi.x, pi.y, pi.z};
     const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
@@ -175,11 +176,11 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
     constexpr int NF = D + 2;
     constexpr int FRN = FRec<D>::N;
     __shared__ double sdt[256];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_block() * blockDim.x + threadIdx.x;
     double dt_i = DBL_MAX;
     if (t < a.N) {
         const int N = a.N;
-        const int i = (int)a.sidx[t];
+        const int i = a.sidx ? (int)a.sidx[t] : t;
         double4 pi = a.pos[t];
         double xi[3] = {pi.x, pi.y, pi.z};
         const T *prim = (const T *)a.prim;
@@ -629,11 +630,11 @@ template <int D, typename T, bool LEARNED>
 __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
     constexpr int NF = D + 2;
     __shared__ uint16_t lds[LEARNED ? 4 : 1][LEARNED ? 64 * 16 : 1];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
     const bool live = t < a.N;
     FaceStats fs{};
     const int N = a.N;
-    const int i = live ? (int)a.sidx[t] : 0;
+    const int i = live ? (a.sidx ? (int)a.sidx[t] : t) : 0;
     const double dtd = a.dt[0];
     const T dt = T(dtd);
     int cnt = live ? a.noi[t] + a.nog[t] : 0;
@@ -771,10 +772,10 @@ __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_sums(HydroArgs a, double *out) {
     __shared__ double sh[6][256];
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = xcd_block() * blockDim.x + threadIdx.x;
     double v[6] = {0, 0, 0, 0, 0, 0};
     if (t < a.N) {
-        const int i = (int)a.sidx[t];
+        const int i = a.sidx ? (int)a.sidx[t] : t;
         const T *pr = (const T *)a.prim + (size_t)t * kPrimN;
         double om = (double)pr[kPrimOmega];
         double m = a.m[i];

@@ -10,11 +10,13 @@ namespace mlh {
 struct NbrArgs {
     Geom g;
     int N, maxr, maxg;
-    const double4 *pos;          // cell order: x, y, z, original index
-    const uint32_t *sorted_key;  // cell of the particle at each cell-order position
+    const double4 *pos_cell;     // cell order: x, y, z, original index
+    const double4 *pos_w;        // the same in work order
+    const uint32_t *ck_w;        // cell of the particle at each work position
+    int caller_order;            // work order = caller order (else cell order)
     const int *cstart, *cend;    // cell -> [start, end) of the cell order
-    uint32_t *nbr;               // [maxr+maxg][N], cell-order rows and entries
-    int *noi, *nog;              // per cell-order position
+    uint32_t *nbr;               // [maxr+maxg][N], work-order rows and entries
+    int *noi, *nog;              // per work position
     int *status;                 // bit 1: real overflow, bit 3: image overflow
     int *max_counts;             // [2]
     int img_window;              // virtual-cell reach of the image search
@@ -22,12 +24,13 @@ struct NbrArgs {
 
 // Arguments of the physics kernels.  Storage type of derived per-particle
 // data (prim, frec) is T; positions / state are FP64.  Threads, neighbour
-// rows, pos, prim and frec are in cell order; the state arrays, the next
-// state and the flux sums are in the caller's particle order (sidx maps).
+// rows, pos, prim and frec are in work order (cell order or the caller's
+// order); the state arrays, the next state and the flux sums are in the
+// caller's particle order (sidx maps; null when the two orders coincide).
 struct HydroArgs {
     Geom g;
     int N, stride;               // stride = rows of nbr (N)
-    const uint32_t *sidx;        // cell-order position -> original index
+    const uint32_t *sidx;        // work position -> original index, or null
     double gamma, cfl, beta, psi1, psi2, riemann_tol;
     int riemann_max_iter, learned_iters;
     int limiter_abs, slope_limiting, pairwise_limiter, mfm, enforce_flux_sym;
@@ -61,6 +64,10 @@ void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
                   const double *x2, int N, double4 *pos, uint32_t *key, uint32_t *idx,
                   uint32_t *skey, uint32_t *scratch, int *cstart, int *cend, hipStream_t s);
 void launch_neighbours(const NbrArgs &a, int D, hipStream_t s);
+void launch_pack_pos(const double *x0, const double *x1, const double *x2, int D, int N,
+                     double4 *pos, hipStream_t s);
+void launch_order_coherence(const Geom &g, int D, const uint32_t *key, int N,
+                            unsigned long long *count, hipStream_t s);
 size_t radix_sort_scratch_words(int N);
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *tmp,
                         hipStream_t s);
