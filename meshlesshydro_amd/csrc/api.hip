@@ -430,10 +430,11 @@ static int ensure_events(mlh_ctx *c, int nsteps) {
 
 static void enqueue_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip,
                          const double *dt_in, double *dt_hist, int step, hipEvent_t *ev) {
-    HydroArgs a = hydro_args(c);
     hipMemsetAsync(c->status, 0, sizeof(int), c->s);
     if (ev) hipEventRecord(ev[0], c->s);
     enqueue_neighbours(c);
+    // after the search: it settles the work order the arguments depend on
+    HydroArgs a = hydro_args(c);
     if (ev) hipEventRecord(ev[1], c->s);
     launch_density(a, c->D, c->prec, c->s);
     if (ev) hipEventRecord(ev[2], c->s);

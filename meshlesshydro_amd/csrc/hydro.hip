@@ -623,11 +623,19 @@ __device__ inline void face_finish(const HydroArgs &a, const FacePrep<D, T> &fp,
 }
 
 
+// Waves per SIMD the face kernel is compiled for.  Two waves cap a lane at
+// 256 registers: the FP32 learned kernel then spills a few registers in the
+// network's branch but keeps the occupancy of the standard FP32 kernel.
+#ifndef MLH_FACES_WAVES
+#define MLH_FACES_WAVES(D, T, LEARNED) ((LEARNED && sizeof(T) == 4) ? 2 : 1)
+#endif
+
 // Faces of particle i.  LEARNED: the loop runs to the wavefront's largest
 // neighbour count so that every lane takes part in the BF16 MFMA inference
 // of the p* seed for the wavefront's 64 faces of that iteration.
 template <int D, typename T, bool LEARNED>
-__global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_faces(HydroArgs a) {
     constexpr int NF = D + 2;
     __shared__ uint16_t lds[LEARNED ? 4 : 1][LEARNED ? 64 * 16 : 1];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
@@ -693,7 +701,7 @@ __global__ __launch_bounds__(256) void k_faces(HydroArgs a) {
                                              (float)fp.WL[1], f);
                 else
                     for (int q = 0; q < kMlpIn; ++q) f[q] = 0.f;
-                float y = mlp_infer_wave_bf16_mem(a.mlp_frags, f, lds[threadIdx.x >> 6]);
+                float y = mlp_infer_wave_bf16_lean(a.mlp_frags, f, lds[threadIdx.x >> 6]);
                 seed = ok ? mlp_seed_from_output(y, f, pref) : -1.f;
             }
         }

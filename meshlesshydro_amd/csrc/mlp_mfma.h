@@ -124,6 +124,67 @@ __device__ inline float mlp_infer_wave_bf16_mem(const MlpFrags *__restrict__ F, 
     return mlp_infer_wave_bf16(R, x, lds);
 }
 
+// Register-lean variant for the face kernel: one face tile (16 faces) at a
+// time, layer 1 straight into the packed BF16 operand of layer 2 (the lane
+// that computes neurons 4g..4g+3 of face c is the lane that needs them), and
+// each weight fragment read from memory (L1 resident) where it is used.
+// About 30 VGPRs live instead of ~170; the arithmetic is that of
+// mlp_infer_wave_bf16 except for the order of the output-layer sums.
+__device__ inline float mlp_infer_wave_bf16_lean(const MlpFrags *__restrict__ F,
+                                                 const float *x, uint16_t *lds) {
+    const int l = __lane_id(), g = l >> 4, c = l & 15;
+    {
+        bf16x8_t lo, hi;
+        for (int j = 0; j < 8; ++j) lo[j] = (__bf16)x[j];
+        hi[0] = (__bf16)x[8];
+        hi[1] = (__bf16)x[9];
+        for (int j = 2; j < 8; ++j) hi[j] = (__bf16)0.f;
+        *(bf16x8_t *)&lds[l * 16] = lo;
+        *(bf16x8_t *)&lds[l * 16 + 8] = hi;
+    }
+    wave_lds_sync();
+    float out = 0.f;
+#pragma unroll 1
+    for (int nt = 0; nt < 4; ++nt) {
+        // keeps the fragment loads inside the loop (no hoisting into registers)
+        asm volatile("" ::: "memory");
+        bf16x8_t bx;
+        if (g < 2) {
+            bx = *(const bf16x8_t *)&lds[(16 * nt + c) * 16 + 8 * g];
+        } else {
+            for (int j = 0; j < 8; ++j) bx[j] = (__bf16)0.f;
+        }
+        bf16x8_t b0, b1;
+        for (int mt = 0; mt < 4; ++mt) {
+            bf16x8_t w = *(const bf16x8_t *)&F->w1[mt][l][0];
+            f32x4_t bias = *(const f32x4_t *)&F->b1[mt][l][0];
+            f32x4_t acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, bx, bias, 0, 0, 0);
+            for (int r = 0; r < 4; ++r) {
+                __bf16 h = (__bf16)tanh_fast(acc[r]);
+                if (mt == 0) b0[r] = h;
+                else if (mt == 1) b0[4 + r] = h;
+                else if (mt == 2) b1[r] = h;
+                else b1[4 + r] = h;
+            }
+        }
+        f32x4_t p = {0.f, 0.f, 0.f, 0.f};
+        for (int mo = 0; mo < 4; ++mo) {
+            bf16x8_t wa = *(const bf16x8_t *)&F->w2[mo][0][l][0];
+            bf16x8_t wb = *(const bf16x8_t *)&F->w2[mo][1][l][0];
+            f32x4_t bias = *(const f32x4_t *)&F->b2[mo][l][0];
+            f32x4_t w3 = *(const f32x4_t *)&F->w3[mo][l][0];
+            f32x4_t acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, b0, bias, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb, b1, acc, 0, 0, 0);
+            for (int r = 0; r < 4; ++r) p[r] += w3[r] * tanh_fast(acc[r]);
+        }
+        float v = p[0] + p[1] + p[2] + p[3];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (g == nt) out = v;
+    }
+    return out + F->b3;
+}
+
 // The network predicts the correction of log p* over Toro's adaptive guess
 // (feature 7): seed = p_ref * exp(y + f7), clamped like the features.
 __device__ inline float mlp_seed_from_output(float y, const float *f, float pref) {

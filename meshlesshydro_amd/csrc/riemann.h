@@ -11,11 +11,12 @@
 //
 // Precision variants:
 //   solve_fp64   every operation in FP64 (the plain exact solver);
-//   solve_mixed  FP32 Newton iterations from a seed (standard guess or the
-//                learned one), then one FP64 Newton step at the FP32 root:
-//                the face is accepted only when that FP64 step passes the
-//                same relative-change test, otherwise solve_fp64 runs from
-//                the standard guess.  Sampling uses the FP64-corrected p*.
+//   solve_mixed  an FP64 Newton step at the seed (standard guess or the
+//                learned one); if it fails the exact solver's relative-change
+//                test, FP32 Newton iterations from there and one more FP64
+//                step at the FP32 root.  A face is accepted only when an FP64
+//                step passes the test, otherwise solve_fp64 runs from the
+//                standard guess.  Sampling uses the FP64-corrected p*.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -272,9 +273,10 @@ __device__ inline double star_pressure_fp64(const GasConst<double> &c, double dl
     return p;
 }
 
-// FP32 Newton from p_seed, FP64 residual step, FP64 fallback.  Inputs are
-// the FP32 face states; p_seed <= 0 selects the standard FP32 guess.  One
-// call site of the FP64 solver (vacuum states and rejected faces share it).
+// FP64 residual step at p_seed, FP32 Newton if needed, FP64 fallback.
+// Inputs are the FP32 face states; p_seed <= 0 selects the standard FP32
+// guess.  One call site of the FP64 solver (vacuum states and rejected faces
+// share it).
 __device__ inline RSol<float> solve_mixed(const GasConst<float> &cf,
                                           const GasConst<double> &cd, float dl,
                                           float ul, float pl, float dr,
@@ -290,29 +292,37 @@ __device__ inline RSol<float> solve_mixed(const GasConst<float> &cf,
     if (ok) {
         float tiny = 1e-6f * 0.5f * (pl + pr);
         float p = p_seed > 0.f ? p_seed : guess_p(cf, dl, ul, pl, cl, dr, ur, pr, cr, tiny);
-        int it = 0;
-        for (; it < iters32; ++it) {
-            float fl, fld, fr, frd;
-            prefun(cf, p, dl, pl, cl, fl, fld);
-            prefun(cf, p, dr, pr, cr, fr, frd);
-            float q = p - (fl + fr + du) / (fld + frd);
-            if (!(q > 0.f)) q = tiny;
-            float ch = __builtin_fabsf(q - p) / (q + p);
-            p = q;
-            if (ch < 2e-6f) { ++it; break; }
+        const double dld = dl, pld = pl, drd = dr, prd = pr, dud = (double)ur - (double)ul;
+        const double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
+        double fl, fld, fr, frd, dp;
+        // FP64 Newton step at the start point: a start within the tolerance
+        // (smooth flow: the standard guess is exact to O(jump^2)) is accepted
+        // at once; otherwise FP32 Newton continues from that step and one
+        // more FP64 step at the FP32 root decides.
+        for (int pass = 0;; ++pass) {
+            double pd = p;
+            prefun(cd, pd, dld, pld, cld, fl, fld);
+            prefun(cd, pd, drd, prd, crd, fr, frd);
+            dp = (fl + fr + dud) / (fld + frd);
+            pn = pd - dp;
+            double change = 2.0 * fabs(dp / (pn + pd));
+            ok = change <= tol && pn > 0.0;
+            st.iters += 1;
+            if (ok || pass == 1) break;
+            if (pn > 0.0) p = (float)pn;
+            int it = 0;
+            for (; it < iters32; ++it) {
+                float gl, gld, gr, grd;
+                prefun(cf, p, dl, pl, cl, gl, gld);
+                prefun(cf, p, dr, pr, cr, gr, grd);
+                float q = p - (gl + gr + du) / (gld + grd);
+                if (!(q > 0.f)) q = tiny;
+                float ch = __builtin_fabsf(q - p) / (q + p);
+                p = q;
+                if (ch < 2e-6f) { ++it; break; }
+            }
+            st.iters += it;
         }
-        // FP64 residual: one Newton step at the FP32 root, accepted under the
-        // exact solver's relative-change test
-        double dld = dl, pld = pl, drd = dr, prd = pr;
-        double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
-        double pd = p, fl, fld, fr, frd;
-        prefun(cd, pd, dld, pld, cld, fl, fld);
-        prefun(cd, pd, drd, prd, crd, fr, frd);
-        double dp = (fl + fr + ((double)ur - (double)ul)) / (fld + frd);
-        pn = pd - dp;
-        double change = 2.0 * fabs(dp / (pn + pd));
-        ok = change <= tol && pn > 0.0;
-        st.iters += it + 1;
         if (!ok) st.fallback += 1;
         um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (frd - fld) * dp);
     }

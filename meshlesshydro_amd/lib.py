@@ -11,7 +11,9 @@ import numpy as np
 from ._abi import Diag, Params, MLP_IN, MLP_NPARAM, MLH_FP64, MLH_SEED_STANDARD
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmlh_hip.so")
+# MLH_LIBRARY selects another build of the same library (e.g. a variant
+# compiled with different launch attributes); default: the in-tree build
+LIB_PATH = os.environ.get("MLH_LIBRARY") or os.path.join(_HERE, "libmlh_hip.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

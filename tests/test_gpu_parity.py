@@ -161,7 +161,9 @@ def test_work_order_invariance(name, hip):
     from meshlesshydro_amd._abi import MLH_ORDER_CALLER, MLH_ORDER_CELL
     out = []
     for order in (MLH_ORDER_CELL, MLH_ORDER_CALLER):
-        p, s = CASES[name](work_order=order)
+        # debug_level 0: the random KH set has a few negative reconstructed
+        # face pressures, which abort a run at the reference's level 1
+        p, s = CASES[name](work_order=order, debug_level=0)
         sim = hip.Simulation(p, s)
         sim.neighbours()
         sim.density()
