@@ -213,6 +213,19 @@ int mlh_riemann_exact_batch(mlh_ctx *ctx, int64_t n, double gamma,
                             int32_t *flag, int32_t *iters, int32_t precision,
                             int32_t seed, double tol);
 
+/* ---- one face flux, batched (Riemann surface) ---------------------------
+ * Riemann::Riemann(WR, WL, vFrame, Aij, i) + Riemann::exact(Fij, gamma)
+ * (Riemann.h:19,26; Riemann.cpp:7-229) for n given faces, FP64, with ctx's
+ * dim, gamma, tolerance and MFM flag.  w_left / w_right: [n][dim+2]
+ * {rho, P, v[dim]} of the solver's left state (particle i: the ctor's WL
+ * member, Particles.cpp:1852) and right state (particle j), in the frame
+ * moving with v_frame [n][dim]; a_ij [n][dim].  f_ij: [n][dim+2] {mass,
+ * energy, momentum[dim]} (Riemann.cpp:164-181).  flag: sampled side (-1
+ * left, +1 right, 0 vacuum, 2 = zero face, no flux). */
+int mlh_riemann_flux_batch(mlh_ctx *ctx, int64_t n, const double *w_left,
+                           const double *w_right, const double *v_frame,
+                           const double *a_ij, double *f_ij, int32_t *flag);
+
 /* Converged star pressure p* of the plain FP64 exact solver for n face
  * problems (0 where vacuum forms): the training target of the learned seed. */
 int mlh_riemann_pstar_batch(int64_t n, double gamma, const double *rhoL,

@@ -712,6 +712,40 @@ int mlh_conserved_sums(mlh_ctx *c, double *out) {
     return MLH_OK;
 }
 
+int mlh_riemann_flux_batch(mlh_ctx *c, int64_t n, const double *w_left, const double *w_right,
+                           const double *v_frame, const double *a_ij, double *f_ij,
+                           int32_t *flag) {
+    if (!c || n < 0) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx or n < 0");
+    if (n == 0) return MLH_OK;
+    if (!w_left || !w_right || !v_frame || !a_ij || !f_ij || !flag)
+        return fail(MLH_ERR_INVALID_ARGUMENT, "null array");
+    const size_t D = c->D, nw = (size_t)n * (D + 2), nv = (size_t)n * D;
+    double *buf = nullptr;
+    int *fl = nullptr;
+    HIPCHK(hipMalloc(&buf, sizeof(double) * (3 * nw + 2 * nv)));
+    if (hipMalloc(&fl, sizeof(int) * (size_t)n) != hipSuccess) {
+        hipFree(buf);
+        return fail(MLH_ERR_DEVICE, "hipMalloc failed");
+    }
+    double *dwl = buf, *dwr = buf + nw, *dvf = buf + 2 * nw, *da = dvf + nv, *df = da + nv;
+    hipError_t e = hipMemcpyAsync(dwl, w_left, sizeof(double) * nw, hipMemcpyHostToDevice, c->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dwr, w_right, sizeof(double) * nw, hipMemcpyHostToDevice, c->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dvf, v_frame, sizeof(double) * nv, hipMemcpyHostToDevice, c->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(da, a_ij, sizeof(double) * nv, hipMemcpyHostToDevice, c->s);
+    if (e == hipSuccess) {
+        HydroArgs a = hydro_args(c);
+        launch_riemann_flux(a, c->D, (long long)n, dwl, dwr, dvf, da, df, fl, c->s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(f_ij, df, sizeof(double) * nw, hipMemcpyDeviceToHost, c->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(flag, fl, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+    hipFree(buf);
+    hipFree(fl);
+    if (e != hipSuccess) return fail(MLH_ERR_DEVICE, hipGetErrorString(e));
+    return MLH_OK;
+}
+
 int mlh_set_face_collection(mlh_ctx *c, int64_t cap, int32_t particle_stride) {
     if (!c || cap < 0) return fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
     MlpState &m = c->mlp;

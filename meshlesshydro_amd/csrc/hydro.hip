@@ -560,8 +560,8 @@ __device__ inline void face_prepare(const HydroArgs &a, const View<D, T> &L,
 // exact Riemann solve from the FacePrep states and the flux projected on A
 // (Riemann.cpp:83-229).  seed > 0: Newton starts there (learned seed).
 template <int D, typename T>
-__device__ inline void face_finish(const HydroArgs &a, const FacePrep<D, T> &fp, float seed,
-                                   T *F, FaceStats &fs) {
+__device__ inline int face_finish(const HydroArgs &a, const FacePrep<D, T> &fp, float seed,
+                                  T *F, FaceStats &fs) {
     const T *A = fp.A, *hA = fp.hA, *WR = fp.WR, *WL = fp.WL, *vF = fp.vF;
     const T ui = fp.ui, uj = fp.uj, ti0 = fp.ti0, tj0 = fp.tj0;
     const T gam = T(a.gamma);
@@ -569,7 +569,7 @@ __device__ inline void face_finish(const HydroArgs &a, const FacePrep<D, T> &fp,
     // normal; the face carries no flux
     if (!(dotT<D, T>(A, A) > T(0))) {
         for (int k = 0; k < D + 2; ++k) F[k] = T(0);
-        return;
+        return 2;  // no face
     }
     T rs, us, Ps;
     int flag;
@@ -620,6 +620,52 @@ __device__ inline void face_finish(const HydroArgs &a, const FacePrep<D, T> &fp,
     for (int b = 0; b < D; ++b)
         e += A[b] * (vS[b] * (Ps / (gam - T(1)) + rs * T(.5) * ekin) + Ps * vLab[b]);
     F[1] = e;
+    return flag;
+}
+
+// Riemann::Riemann + Riemann::exact (Riemann.cpp:7-229) on given faces, FP64:
+// wl / wr = {rho, P, v[D]} of the solver's left (particle i) and right
+// (particle j) states in the frame moving with vf; F = {mass, energy,
+// momentum[D]}; flag = sampled side (-1 left, +1 right, 0 vacuum, 2 A = 0).
+template <int D>
+__global__ __launch_bounds__(256) void k_riemann_flux(HydroArgs a, long long n,
+                                                      const double *__restrict__ wl,
+                                                      const double *__restrict__ wr,
+                                                      const double *__restrict__ vf,
+                                                      const double *__restrict__ aij,
+                                                      double *__restrict__ f,
+                                                      int *__restrict__ flag) {
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    FacePrep<D, double> fp;
+    for (int c = 0; c < D + 2; ++c) {
+        fp.WR[c] = wl[k * (D + 2) + c];
+        fp.WL[c] = wr[k * (D + 2) + c];
+    }
+    for (int ax = 0; ax < D; ++ax) {
+        fp.A[ax] = aij[k * D + ax];
+        fp.vF[ax] = vf[k * D + ax];
+    }
+    double An = sqrt(dotT<D, double>(fp.A, fp.A));
+    for (int ax = 0; ax < D; ++ax) fp.hA[ax] = 1. / An * fp.A[ax];
+    fp.ti0 = fp.tj0 = 0.;
+    if (D == 2) {
+        double L0 = fp.hA[0] * 1. + fp.hA[1] * 0.;
+        double L1 = -(fp.hA[0] * 0. - fp.hA[1] * 1.);
+        double L2 = -L1, L3 = L0;
+        fp.ui = L0 * fp.WR[2] + L1 * fp.WR[3];
+        fp.ti0 = L2 * fp.WR[2] + L3 * fp.WR[3];
+        fp.uj = L0 * fp.WL[2] + L1 * fp.WL[3];
+        fp.tj0 = L2 * fp.WL[2] + L3 * fp.WL[3];
+    } else {
+        fp.ui = dotT<D, double>(fp.hA, &fp.WR[2]);
+        fp.uj = dotT<D, double>(fp.hA, &fp.WL[2]);
+    }
+    FaceStats fs{};
+    double F[D + 2];
+    int fl = face_finish<D, double>(a, fp, -1.f, F, fs);
+    for (int c = 0; c < D + 2; ++c) f[k * (D + 2) + c] = F[c];
+    flag[k] = fl;
 }
 
 
@@ -844,6 +890,16 @@ void launch_faces(const HydroArgs &a, int D, int precision, hipStream_t s) {
         else hipLaunchKernelGGL((k_faces<3, float, false>), grid, dim3(256), 0, s, a);
     }
 }
+void launch_riemann_flux(const HydroArgs &a, int D, long long n, const double *wl,
+                         const double *wr, const double *vf, const double *aij, double *f,
+                         int *flag, hipStream_t s) {
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (D == 2)
+        hipLaunchKernelGGL(k_riemann_flux<2>, grid, dim3(256), 0, s, a, n, wl, wr, vf, aij, f, flag);
+    else
+        hipLaunchKernelGGL(k_riemann_flux<3>, grid, dim3(256), 0, s, a, n, wl, wr, vf, aij, f, flag);
+}
+
 void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s) {
     MLH_DISPATCH(k_sums, D, precision, dim3((a.N + 255) / 256), a, out);
 }

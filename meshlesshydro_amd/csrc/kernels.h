@@ -80,5 +80,8 @@ void launch_dt_finalize(const HydroArgs &a, double dt_fixed, double t_now, doubl
                         const double *dt_in, double *dt_hist, int step, hipStream_t s);
 void launch_faces(const HydroArgs &a, int D, int precision, hipStream_t s);
 void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s);
+void launch_riemann_flux(const HydroArgs &a, int D, long long n, const double *wl,
+                         const double *wr, const double *vf, const double *aij, double *f,
+                         int *flag, hipStream_t s);
 
 }  // namespace mlh

@@ -45,6 +45,7 @@ SIGNATURES = {
     "mlh_conserved_sums": (C.c_int, [_vp, _dp]),
     "mlh_get_timings": (C.c_int, [_vp, _fp]),
     "mlh_riemann_exact_batch": (C.c_int, [_vp, C.c_int64, C.c_double] + [_dp] * 9 + [_ip, _ip, C.c_int32, C.c_int32, C.c_double]),
+    "mlh_riemann_flux_batch": (C.c_int, [_vp, C.c_int64] + [_dp] * 5 + [_ip]),
     "mlh_mlp_features": (C.c_int, [C.c_int64, C.c_double] + [_dp] * 6 + [_fp, _dp]),
     "mlh_mlp_set_params": (C.c_int, [_vp, _fp]),
     "mlh_mlp_get_params": (C.c_int, [_vp, _fp]),
@@ -248,6 +249,21 @@ class Simulation:
         n = C.c_int64()
         _chk(self.L.mlh_get_collected_faces(self.h, _d(out), C.byref(n)), "get_collected_faces")
         return out[: n.value].copy()
+
+    def riemann_flux(self, w_left, w_right, v_frame, a_ij):
+        """Riemann::Riemann + exact on given faces (device, FP64): w_* [n][D+2]
+        {rho, P, v...}, v_frame / a_ij [n][D] -> F [n][D+2], flags [n]."""
+        D = self.p.dim
+        wl = np.ascontiguousarray(w_left, np.float64).reshape(-1, D + 2)
+        wr = np.ascontiguousarray(w_right, np.float64).reshape(-1, D + 2)
+        vf = np.ascontiguousarray(v_frame, np.float64).reshape(-1, D)
+        A = np.ascontiguousarray(a_ij, np.float64).reshape(-1, D)
+        n = len(wl)
+        F = np.zeros((n, D + 2))
+        fl = np.zeros(n, np.int32)
+        _chk(lib().mlh_riemann_flux_batch(self.h, n, _d(wl), _d(wr), _d(vf), _d(A), _d(F), _i(fl)),
+             "mlh_riemann_flux_batch")
+        return F, fl
 
     def riemann(self, rl, ul, pl, rr, ur, pr, precision=MLH_FP64, seed=MLH_SEED_STANDARD, tol=1e-6):
         return riemann_batch(self.p.gamma, rl, ul, pl, rr, ur, pr, precision, seed, tol, ctx=self.h)

@@ -903,8 +903,8 @@ static void face_states(const orc *o, int i, int j, const double *xj,
 
 /* Riemann::Riemann + Riemann::exact + rotateAndProjectFluxes2D/3D
  * (Riemann.cpp:7-229).  Wi = i-state (the solver's left), Wj = j-state. */
-static void riemann_flux(orc *o, const double *Wi_in, const double *Wj_in,
-                         const double *vFrame, const double *A, double *F) {
+static int riemann_flux(orc *o, const double *Wi_in, const double *Wj_in,
+                        const double *vFrame, const double *A, double *F) {
     int D = o->D;
     double g = o->p.gamma;
     double Wi[MAXD + 2], Wj[MAXD + 2];
@@ -978,6 +978,21 @@ static void riemann_flux(orc *o, const double *Wi_in, const double *Wj_in,
     for (int b = 0; b < D; ++b)
         e += A[b] * (vS[b] * (Ps / (g - 1.) + rs * .5 * ekin) + Ps * vLab[b]);
     F[1] = e;
+    return flag;
+}
+
+/* Riemann::Riemann + Riemann::exact (Riemann.cpp:7-229) on n given faces:
+ * w_left / w_right = {rho, P, v[D]} of the solver's left (particle i) and
+ * right (particle j) states in the frame moving with v_frame. */
+void orc_riemann_flux_batch(orc *o, int64_t n, const double *w_left, const double *w_right,
+                            const double *v_frame, const double *a_ij, double *f_ij,
+                            int32_t *flag) {
+    int D = o->D;
+    for (int64_t k = 0; k < n; ++k) {
+        int fl = riemann_flux(o, &w_left[k * (D + 2)], &w_right[k * (D + 2)], &v_frame[k * D],
+                              &a_ij[k * D], &f_ij[k * (D + 2)]);
+        if (flag) flag[k] = fl;
+    }
 }
 
 /* compEffectiveFace (:1290-1311, ghosts :2504-2531), compRiemannStatesLR,

@@ -50,6 +50,9 @@ int orc_conserved_sums(orc *o, double *out);
 int orc_get_diag(orc *o, mlh_diag *d);
 /* brute-force (O(N * N_ghost)) ghost search instead of the binned one */
 void orc_set_bruteforce_ghosts(orc *o, int on);
+void orc_riemann_flux_batch(orc *o, int64_t n, const double *w_left, const double *w_right,
+                            const double *v_frame, const double *a_ij, double *f_ij,
+                            int32_t *flag);
 void orc_set_threads(int n);
 
 /* exact Riemann solver (Toro, "Riemann Solvers and Numerical Methods for

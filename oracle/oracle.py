@@ -48,6 +48,7 @@ def lib():
         L.orc_conserved_sums.argtypes = [C.c_void_p, _dp]
         L.orc_get_diag.argtypes = [C.c_void_p, C.POINTER(Diag)]
         L.orc_set_bruteforce_ghosts.argtypes = [C.c_void_p, C.c_int]
+        L.orc_riemann_flux_batch.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 5 + [_ip]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_riemann_solve.restype = C.c_int
         L.orc_riemann_solve.argtypes = [C.c_double] * 7 + [C.c_double, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)]
@@ -138,6 +139,21 @@ class Oracle:
         dt = C.c_double()
         self._chk(self.L.orc_step(self.h, dt_fixed, dt_max, C.byref(dt)), "step")
         return dt.value
+
+    def riemann_flux(self, w_left, w_right, v_frame, a_ij):
+        """Riemann::Riemann + exact on given faces: w_* = [n][D+2] {rho, P,
+        v...} (left = the solver's particle i), v_frame / a_ij = [n][D].
+        Returns F [n][D+2] {mass, energy, momentum...} and the sample flags."""
+        D = self.D
+        wl = np.ascontiguousarray(w_left, np.float64).reshape(-1, D + 2)
+        wr = np.ascontiguousarray(w_right, np.float64).reshape(-1, D + 2)
+        vf = np.ascontiguousarray(v_frame, np.float64).reshape(-1, D)
+        A = np.ascontiguousarray(a_ij, np.float64).reshape(-1, D)
+        n = len(wl)
+        F = np.zeros((n, D + 2))
+        fl = np.zeros(n, np.int32)
+        self.L.orc_riemann_flux_batch(self.h, n, _d(wl), _d(wr), _d(vf), _d(A), _d(F), _i(fl))
+        return F, fl
 
     def get_neighbours(self):
         stride = self.p.max_interactions + self.p.max_ghost_interactions

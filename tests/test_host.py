@@ -1,0 +1,142 @@
+"""Host C++ layer on the CPU: parameter.h and config.info readers, HDF5 I/O
+in the reference's layout, the `mlh` driver's command line."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from meshlesshydro_amd import ics
+
+host = pytest.importorskip("meshlesshydro_amd.host")
+
+PARAMETER_H = """//
+// parameter file in the reference's format
+//
+#ifndef DEMONSTRATOR_PARAMETER_H
+#define DEMONSTRATOR_PARAMETER_H
+
+/// possible values 2 or 3 for 2D or 3D simulations
+#define DIM 3
+#define PERIODIC_BOUNDARIES 1
+#define ADAPTIVE_TIMESTEP 0
+#define CFL .3              // comment after a value
+#define MAX_NUM_INTERACTIONS 128
+/** maximum interactions with ghost particles
+ *  #define MAX_NUM_INTERACTIONS 7   (inside a block comment: ignored)
+**/
+#define MAX_NUM_GHOST_INTERACTIONS 96
+#define SLOPE_LIMITING 1
+#define BETA 2.
+#define PAIRWISE_LIMITER 1
+#define PSI_1 .4
+#define PSI_2 .2
+#define MESHLESS_FINITE_MASS 0
+#define ENFORCE_FLUX_SYM 1
+#define MOVE_PARTICLES 1
+#define DEBUG_LVL 0
+#define FIRST_ORDER_QUAD_POINT 1
+#define RUNSPH 0
+#define ARTVISC 1
+#define FLUX_SYM_TOL 1e-20
+#endif //DEMONSTRATOR_PARAMETER_H
+"""
+
+CONFIG = """; h5 file containing the initial particle distribution
+initFile ic.h5
+
+; output directory
+outDir "out dir"
+
+timeStep .005
+timeEnd 4.   ; end of the simulation
+h5DumpInterval 10
+
+periodicBoxLimits {
+    lowerX 0.
+    lowerY -1.
+    lowerZ 0.
+    upperX 1.
+    upperY 2.
+    upperZ 0.5
+}
+kernelSize .025
+gamma 1.6666666666666667
+"""
+
+
+@pytest.fixture(scope="module")
+def built():
+    if not os.path.exists(host.HOST_LIB):
+        pytest.skip("libmlh_host.so not built")
+    return host
+
+
+def test_parameter_header(built, tmp_path):
+    f = tmp_path / "parameter.h"
+    f.write_text(PARAMETER_H)
+    p, adaptive = built.read_parameters(str(f))
+    assert p.dim == 3 and p.periodic == 1 and adaptive is False
+    assert p.cfl == 0.3 and p.beta == 2.0 and p.psi1 == 0.4 and p.psi2 == 0.2
+    assert p.max_interactions == 128 and p.max_ghost_interactions == 96
+    assert p.pairwise_limiter == 1 and p.debug_level == 0 and p.first_order_quad_point == 1
+    # defaults for what the file does not set
+    assert p.gamma == pytest.approx(5 / 3) and p.riemann_tol == 1e-6
+    f.write_text(PARAMETER_H.replace("#define RUNSPH 0", "#define RUNSPH 1"))
+    with pytest.raises(built.HostError, match="SPH"):
+        built.read_parameters(str(f))
+    f.write_text("#define CFL abc\n")
+    with pytest.raises(built.HostError, match="CFL"):
+        built.read_parameters(str(f))
+
+
+def test_config_info(built, tmp_path):
+    f = tmp_path / "config.info"
+    f.write_text(CONFIG)
+    assert built.config_value(str(f), "initFile") == "ic.h5"
+    assert built.config_value(str(f), "outDir") == "out dir"
+    assert float(built.config_value(str(f), "timeEnd")) == 4.0
+    assert float(built.config_value(str(f), "periodicBoxLimits.lowerY")) == -1.0
+    assert float(built.config_value(str(f), "periodicBoxLimits.upperZ")) == 0.5
+    with pytest.raises(built.HostError, match="no key"):
+        built.config_value(str(f), "periodicBoxLimits.upperW")
+    f.write_text("a {\n b 1\n")
+    with pytest.raises(built.HostError, match="missing"):
+        built.config_value(str(f), "a.b")
+
+
+def test_h5_initial_distribution_roundtrip(built, tmp_path):
+    s = ics.blast_3d(6)
+    path = str(tmp_path / "ic.h5")
+    built.write_ic(path, s, 3)
+    X = built.read_dataset(path, "/x")
+    V = built.read_dataset(path, "/v")
+    assert X.shape == (216, 3) and V.shape == (216, 3)
+    assert np.array_equal(X[:, 0], s["x"]) and np.array_equal(X[:, 2], s["z"])
+    assert np.array_equal(built.read_dataset(path, "/u"), s["u"])
+    assert np.array_equal(built.read_dataset(path, "/m"), s["m"])
+    assert np.array_equal(built.read_dataset_int(path, "/materialId"), np.zeros(216, np.int32))
+    with pytest.raises(built.HostError):
+        built.read_dataset(path, "/rho")
+
+
+def test_driver_command_line(built, tmp_path):
+    r = subprocess.run([built.MLH_EXE, "-h"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "--config" in r.stdout
+    r = subprocess.run([built.MLH_EXE, "-v", "-s"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+    # a configuration that reads: the run then needs a GPU (status 21
+    # MLH_ERR_DEVICE here, 0 where one is present)
+    (tmp_path / "parameter.h").write_text(PARAMETER_H.replace("#define DIM 3", "#define DIM 2"))
+    (tmp_path / "config.info").write_text(CONFIG.replace("timeEnd 4.", "timeEnd 0.01")
+                                          .replace("kernelSize .025", "kernelSize .25"))
+    s = ics.kelvin_helmholtz(16)
+    built.write_ic(str(tmp_path / "ic.h5"), s, 2)
+    os.makedirs(tmp_path / "out dir", exist_ok=True)
+    r = subprocess.run([built.MLH_EXE, "-c", str(tmp_path / "config.info"), "-p",
+                        str(tmp_path / "parameter.h"), "-s"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode in (0, 21), r.stderr
+    r = subprocess.run([built.MLH_EXE, "-c", str(tmp_path / "missing.info")], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode != 0 and "cannot open" in r.stderr

@@ -126,3 +126,38 @@ def test_vacuum_generation(oracle_mod):
     cl = np.sqrt(1.4)
     assert fl[0] == -1
     assert abs(us[0] - 2.0 / 2.4 * cl) < 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_face_flux_uniform_state_is_euler_flux(oracle_mod, dim):
+    """Riemann::exact (Riemann.cpp:83-229) between two identical states: the
+    solution is the state itself, so F is the Euler flux through A in the
+    lab frame, F = (rho v.A, (E + P) v.A, rho v (v.A) + P A) with
+    v = w + v_frame and E = P/(gamma-1) + rho |v|^2/2.  Pins the rotation
+    back to the lab frame and the projection onto A (2D rotation and 3D
+    basis-free paths)."""
+    from meshlesshydro_amd._abi import default_params
+    from meshlesshydro_amd import ics
+    rng = np.random.default_rng(12 + dim)
+    n = 200
+    p = default_params(dim=dim, kernel_size=0.3, max_interactions=64, max_ghost_interactions=64)
+    st = ics.blast_3d(4) if dim == 3 else ics.kelvin_helmholtz(8)
+    o = oracle_mod.Oracle(p, st)
+    rho = rng.uniform(0.5, 2.0, n)
+    P = rng.uniform(0.3, 3.0, n)
+    w = rng.uniform(-1, 1, (n, dim))
+    vf = rng.uniform(-1, 1, (n, dim))
+    A = rng.uniform(-1, 1, (n, dim))
+    W = np.concatenate([rho[:, None], P[:, None], w], 1)
+    F, fl = o.riemann_flux(W, W, vf, A)
+    g = p.gamma
+    v = w + vf
+    vA = np.sum(v * A, 1)
+    wA = np.sum(w * A, 1)
+    E = P / (g - 1) + 0.5 * rho * np.sum(v * v, 1)
+    # in the face frame the gas moves with w: mass flux rho w.A, momentum
+    # rho v (w.A) + P A, energy (E_lab) w.A + P v.A
+    expect = np.concatenate([(rho * wA)[:, None], (E * wA + P * vA)[:, None],
+                             rho[:, None] * v * wA[:, None] + P[:, None] * A], 1)
+    assert np.allclose(F, expect, rtol=1e-9, atol=1e-12)
+    assert np.all(np.abs(fl) == 1)
