@@ -37,7 +37,8 @@ typedef enum mlh_status {
     MLH_ERR_DUMP_OVERRUN = 9,
     MLH_ERR_INVALID_ARGUMENT = 20,
     MLH_ERR_DEVICE = 21,              /* HIP runtime error / no device        */
-    MLH_ERR_GRID = 22,                /* fewer than 3 search cells per axis   */
+    MLH_ERR_GRID = 22,                /* periodic: < 3 search cells per axis; *
+                                       * open: extent below one kernel size   */
     MLH_ERR_CALL_ORDER = 23,          /* stage called before its inputs exist */
     MLH_ERR_NOT_TRAINED = 24          /* learned seed requested, no network   */
 } mlh_status;

@@ -43,6 +43,9 @@ struct mlh_ctx {
     uint32_t *cell_key = nullptr, *sorted_idx = nullptr, *sorted_key = nullptr,
              *sort_scratch = nullptr;
     int *cstart = nullptr, *cend = nullptr;
+    int cell_cap = 0;                // cells cstart / cend can hold
+    double *lim_scratch = nullptr;   // domain limits (non-periodic)
+    double *lim_out = nullptr;
     uint32_t *nbr = nullptr;
     int *noi = nullptr, *nog = nullptr;
     void *prim = nullptr, *frec = nullptr;
@@ -111,7 +114,9 @@ static int dalloc(mlh_ctx *c, P **ptr, size_t bytes) {
     return 0;
 }
 
-static int make_geom(mlh_ctx *c) {
+// Domain::createGrid (Domain.cpp:9-54) over [lo, hi]: the periodic box of
+// the parameters, or the particles' limits of this step (non-periodic)
+static int make_geom(mlh_ctx *c, const double *lo, const double *hi) {
     const mlh_params &p = c->p;
     Geom &g = c->g;
     g.periodic = p.periodic;
@@ -121,16 +126,18 @@ static int make_geom(mlh_ctx *c) {
     g.sigma = c->D == 2 ? 10. / (7. * M_PI * h2 * h2) : 1. / (M_PI * h2 * h2 * h2);
     long long ncells = 1;
     for (int a = 0; a < 3; ++a) {
-        g.bmin[a] = p.box_min[a];
-        g.bmax[a] = p.box_max[a];
+        g.bmin[a] = a < c->D ? lo[a] : 0.0;
+        g.bmax[a] = a < c->D ? hi[a] : 1.0;
         g.nc[a] = 1;
         g.cs[a] = 1.0;
     }
     for (int a = 0; a < c->D; ++a) {
         double ext = g.bmax[a] - g.bmin[a];
         int n = (int)floor(ext / p.kernel_size);
-        if (n < 3)
+        if (p.periodic && n < 3)
             return fail(MLH_ERR_GRID, "fewer than 3 search cells per axis (box / kernelSize)");
+        if (!(n >= 1))
+            return fail(MLH_ERR_GRID, "particle extent below one kernel size on an axis");
         g.nc[a] = n;
         g.cs[a] = ext / (double)n;
         ncells *= n;
@@ -201,8 +208,6 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     if (!p || !out || n <= 0) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument or n <= 0");
     if (p->dim != 2 && p->dim != 3) return fail(MLH_ERR_INVALID_ARGUMENT, "dim must be 2 or 3");
     if (n > (int32_t)kIdxMask) return fail(MLH_ERR_INVALID_ARGUMENT, "n exceeds 2^27-1");
-    if (!p->periodic)
-        return fail(MLH_ERR_INVALID_ARGUMENT, "non-periodic domains are not supported yet");
     if (p->max_interactions <= 0 || p->max_ghost_interactions < 0)
         return fail(MLH_ERR_INVALID_ARGUMENT, "bad interaction capacities");
     if (p->precision != MLH_FP64 && p->precision != MLH_FP32)
@@ -219,7 +224,7 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     c->prec = p->precision;
     c->tsz = c->prec == MLH_FP64 ? 8 : 4;
     c->stride = p->max_interactions + p->max_ghost_interactions;
-    int st = make_geom(c);
+    int st = p->periodic ? make_geom(c, p->box_min, p->box_max) : MLH_OK;
     if (st) { delete c; return st; }
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -238,7 +243,13 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     A_(&c->coh, sizeof(unsigned long long));
     A_(&c->cell_key, 4 * N); A_(&c->sorted_idx, 4 * N); A_(&c->sorted_key, 4 * N);
     A_(&c->sort_scratch, 4 * (2 * N + radix_sort_scratch_words(n)));
-    A_(&c->cstart, 4 * (size_t)c->g.ncells); A_(&c->cend, 4 * (size_t)c->g.ncells);
+    if (p->periodic) {
+        A_(&c->cstart, 4 * (size_t)c->g.ncells); A_(&c->cend, 4 * (size_t)c->g.ncells);
+        c->cell_cap = c->g.ncells;
+    } else {
+        A_(&c->lim_scratch, sizeof(double) * 3 * D * (size_t)limits_blocks(n));
+        A_(&c->lim_out, sizeof(double) * 2 * D);
+    }
     A_(&c->nbr, 4 * N * (size_t)c->stride);
     A_(&c->noi, 4 * N); A_(&c->nog, 4 * N);
     A_(&c->prim, c->tsz * kPrimN * N);
@@ -273,6 +284,10 @@ int mlh_destroy(mlh_ctx *c) {
     if (!c) return MLH_OK;
     if (c->s) hipStreamSynchronize(c->s);
     for (void *q : c->allocs) hipFree(q);
+    if (!c->p.periodic) {  // grown per step, not in allocs
+        if (c->cstart) hipFree(c->cstart);
+        if (c->cend) hipFree(c->cend);
+    }
     mlp_free(c->mlp);
     for (hipEvent_t e : c->evpool) hipEventDestroy(e);
     if (c->dt_hist) hipFree(c->dt_hist);
@@ -334,7 +349,37 @@ static void resolve_order(mlh_ctx *c) {
     c->order = (double)n >= 0.75 * (double)(c->N - 1) ? MLH_ORDER_CALLER : MLH_ORDER_CELL;
 }
 
-static void enqueue_neighbours(mlh_ctx *c) {
+// non-periodic: the domain limits of this step (one synchronisation), the
+// grid over them, and cell-range buffers of that size
+static int prepare_open_grid(mlh_ctx *c) {
+    double lim[6];
+    launch_domain_limits(c->x[0], c->x[1], c->D == 3 ? c->x[2] : c->x[0], c->D, c->N,
+                         c->lim_scratch, c->lim_out, c->s);
+    HIPCHK(hipMemcpyAsync(lim, c->lim_out, sizeof(double) * 2 * c->D, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    double lo[3], hi[3];
+    for (int a = 0; a < c->D; ++a) {
+        lo[a] = lim[2 * a];
+        hi[a] = lim[2 * a + 1];
+    }
+    int st = make_geom(c, lo, hi);
+    if (st) return st;
+    if (c->g.ncells > c->cell_cap) {
+        if (c->cstart) hipFree(c->cstart);
+        if (c->cend) hipFree(c->cend);
+        c->cstart = c->cend = nullptr;
+        HIPCHK(hipMalloc(&c->cstart, 4 * (size_t)c->g.ncells));
+        HIPCHK(hipMalloc(&c->cend, 4 * (size_t)c->g.ncells));
+        c->cell_cap = c->g.ncells;
+    }
+    return MLH_OK;
+}
+
+static int enqueue_neighbours(mlh_ctx *c) {
+    if (!c->p.periodic) {
+        int st = prepare_open_grid(c);
+        if (st) return st;
+    }
     launch_cells(c->g, c->D, c->x[0], c->x[1], c->D == 3 ? c->x[2] : c->x[0], c->N, c->pos,
                  c->cell_key, c->sorted_idx, c->sorted_key, c->sort_scratch, c->cstart, c->cend,
                  c->s);
@@ -361,6 +406,7 @@ static void enqueue_neighbours(mlh_ctx *c) {
     a.max_counts = c->max_counts;
     a.img_window = c->img_window;
     launch_neighbours(a, c->D, c->s);
+    return MLH_OK;
 }
 
 // work position -> caller's particle index
@@ -428,12 +474,14 @@ static int ensure_events(mlh_ctx *c, int nsteps) {
     return MLH_OK;
 }
 
-static void enqueue_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip,
+static int enqueue_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip,
                          const double *dt_in, double *dt_hist, int step, hipEvent_t *ev) {
     hipMemsetAsync(c->status, 0, sizeof(int), c->s);
     if (ev) hipEventRecord(ev[0], c->s);
-    enqueue_neighbours(c);
-    // after the search: it settles the work order the arguments depend on
+    int st = enqueue_neighbours(c);
+    if (st) return st;
+    // after the search: it settles the work order (and, without periodic
+    // boundaries, the grid) the arguments depend on
     HydroArgs a = hydro_args(c);
     if (ev) hipEventRecord(ev[1], c->s);
     launch_density(a, c->D, c->prec, c->s);
@@ -444,6 +492,7 @@ static void enqueue_step(mlh_ctx *c, double dt_fixed, double t_now, double t_cli
     launch_faces(a, c->D, c->prec, c->s);
     if (ev) hipEventRecord(ev[4], c->s);
     swap_state(c);
+    return MLH_OK;
 }
 
 // average device time per step of each kernel group over nsteps steps
@@ -467,7 +516,8 @@ int mlh_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip, double *d
     unsigned long long before[kNumCounters], after[kNumCounters];
     if ((st = read_counters(c, before))) return st;
     if ((st = ensure_events(c, 1))) return st;
-    enqueue_step(c, dt_fixed, t_now, t_clip, nullptr, nullptr, 0, c->evpool.data());
+    if ((st = enqueue_step(c, dt_fixed, t_now, t_clip, nullptr, nullptr, 0, c->evpool.data())))
+        return st;
     double dtv[3];
     HIPCHK(hipMemcpyAsync(dtv, c->dt, sizeof(dtv), hipMemcpyDeviceToHost, c->s));
     if ((st = check_status(c))) { c->stage = 0; return st; }
@@ -500,7 +550,8 @@ int mlh_run(mlh_ctx *c, int32_t n_steps, double dt_fixed, const double *dts_in,
     unsigned long long before[kNumCounters], after[kNumCounters];
     if ((st = read_counters(c, before))) return st;
     for (int k = 0; k < n_steps; ++k)
-        enqueue_step(c, dt_fixed, -1.0, -1.0, dev_in, c->dt_hist, k, &c->evpool[5 * k]);
+        if ((st = enqueue_step(c, dt_fixed, -1.0, -1.0, dev_in, c->dt_hist, k, &c->evpool[5 * k])))
+            return st;
     if ((st = check_status(c))) return st;
     if (n_steps > 0) collect_timings(c, n_steps);
     if (dts_out && n_steps > 0)
@@ -514,8 +565,8 @@ int mlh_run(mlh_ctx *c, int32_t n_steps, double dt_fixed, const double *dts_in,
 int mlh_neighbours(mlh_ctx *c) {
     if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
     HIPCHK(hipMemsetAsync(c->status, 0, sizeof(int), c->s));
-    enqueue_neighbours(c);
-    int st = check_status(c);
+    int st = enqueue_neighbours(c);
+    if (!st) st = check_status(c);
     c->stage = st ? 0 : 1;
     return st;
 }

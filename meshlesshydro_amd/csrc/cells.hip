@@ -367,6 +367,150 @@ template __global__ void k_assign_cells<2>(Geom, const double *, const double *,
                                            int, uint32_t *, uint32_t *);
 template __global__ void k_assign_cells<3>(Geom, const double *, const double *, const double *,
                                            int, uint32_t *, uint32_t *);
+// ------------------------------------------------------------ domain limits
+// Particles::getDomainLimits (Particles.cpp:228-268) as the reference runs
+// it, sequentially per axis: min starts at DBL_MAX, max at DBL_MIN (the
+// smallest positive double), and the max test is an else-branch of the min
+// test, so x_i only counts for the max when it is not a new running
+// minimum, i.e. when x_i >= min(x_0 .. x_{i-1}).  In parallel: block
+// minima, their exclusive prefix minima, then every element's exclusive
+// prefix minimum inside its block, the candidate test and a max reduction.
+// Blocks of 256 threads x 4 consecutive elements; gridDim.y = axis.
+constexpr int kLimItems = 4, kLimBlock = 256 * kLimItems;
+
+__device__ inline double block_min256(double v, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_limits_blockmin(const double *__restrict__ x0,
+                                                         const double *__restrict__ x1,
+                                                         const double *__restrict__ x2, int N,
+                                                         double *__restrict__ bmin) {
+    __shared__ double sh[256];
+    const double *x = blockIdx.y == 0 ? x0 : (blockIdx.y == 1 ? x1 : x2);
+    double m = DBL_MAX;
+    size_t base = (size_t)blockIdx.x * kLimBlock + threadIdx.x * kLimItems;
+    for (int k = 0; k < kLimItems; ++k)
+        if (base + k < (size_t)N) m = fmin(m, x[base + k]);
+    m = block_min256(m, sh);
+    if (threadIdx.x == 0) bmin[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = m;
+}
+
+// exclusive prefix minima of the block minima, one axis per block
+__global__ __launch_bounds__(256) void k_limits_scan(const double *__restrict__ bmin, int nb,
+                                                     double *__restrict__ bpre) {
+    __shared__ double sh[256];
+    const double *in = bmin + (size_t)blockIdx.x * nb;
+    double *out = bpre + (size_t)blockIdx.x * nb;
+    int per = (nb + 255) / 256;
+    int a = threadIdx.x * per, b = min(a + per, nb);
+    double m = DBL_MAX;
+    for (int k = a; k < b; ++k) m = fmin(m, in[k]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive scan of 256 chunk minima
+        double run = DBL_MAX;
+        for (int k = 0; k < 256; ++k) {
+            double v = sh[k];
+            sh[k] = run;
+            run = fmin(run, v);
+        }
+    }
+    __syncthreads();
+    double run = sh[threadIdx.x];
+    for (int k = a; k < b; ++k) {
+        out[k] = run;
+        run = fmin(run, in[k]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_limits_blockmax(const double *__restrict__ x0,
+                                                         const double *__restrict__ x1,
+                                                         const double *__restrict__ x2, int N,
+                                                         const double *__restrict__ bpre,
+                                                         double *__restrict__ bmax) {
+    __shared__ double sh[256];
+    const double *x = blockIdx.y == 0 ? x0 : (blockIdx.y == 1 ? x1 : x2);
+    size_t base = (size_t)blockIdx.x * kLimBlock + threadIdx.x * kLimItems;
+    double v[kLimItems];
+    double tm = DBL_MAX;
+    for (int k = 0; k < kLimItems; ++k) {
+        v[k] = base + k < (size_t)N ? x[base + k] : DBL_MAX;
+        tm = fmin(tm, v[k]);
+    }
+    // exclusive prefix minimum of the thread minima inside the block
+    sh[threadIdx.x] = tm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double run = bpre[(size_t)blockIdx.y * gridDim.x + blockIdx.x];
+        for (int k = 0; k < 256; ++k) {
+            double t = sh[k];
+            sh[k] = run;
+            run = fmin(run, t);
+        }
+    }
+    __syncthreads();
+    double pm = sh[threadIdx.x];
+    __syncthreads();
+    double mx = DBL_MIN;
+    for (int k = 0; k < kLimItems; ++k) {
+        if (base + k >= (size_t)N) break;
+        double xv = v[k];
+        if (xv < pm) pm = xv;
+        else if (xv > mx) mx = xv;
+    }
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bmax[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// out[2a] = min, out[2a+1] = max of axis a (block = axis)
+__global__ __launch_bounds__(256) void k_limits_final(const double *__restrict__ bmin,
+                                                      const double *__restrict__ bmax, int nb,
+                                                      double *__restrict__ out) {
+    __shared__ double sh[256];
+    double mn = DBL_MAX, mx = DBL_MIN;
+    for (int k = threadIdx.x; k < nb; k += 256) {
+        mn = fmin(mn, bmin[(size_t)blockIdx.x * nb + k]);
+        mx = fmax(mx, bmax[(size_t)blockIdx.x * nb + k]);
+    }
+    mn = block_min256(mn, sh);
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = mn;
+        out[2 * blockIdx.x + 1] = sh[0];
+    }
+}
+
+int limits_blocks(int N) { return (N + kLimBlock - 1) / kLimBlock; }
+
+void launch_domain_limits(const double *x0, const double *x1, const double *x2, int D, int N,
+                          double *scratch, double *out, hipStream_t s) {
+    int nb = limits_blocks(N);
+    double *bmin = scratch, *bpre = scratch + (size_t)D * nb, *bmax = scratch + 2 * (size_t)D * nb;
+    hipLaunchKernelGGL(k_limits_blockmin, dim3(nb, D), dim3(256), 0, s, x0, x1, x2, N, bmin);
+    hipLaunchKernelGGL(k_limits_scan, dim3(D), dim3(256), 0, s, bmin, nb, bpre);
+    hipLaunchKernelGGL(k_limits_blockmax, dim3(nb, D), dim3(256), 0, s, x0, x1, x2, N, bpre, bmax);
+    hipLaunchKernelGGL(k_limits_final, dim3(D), dim3(256), 0, s, bmin, bmax, nb, out);
+}
+
 // positions in the caller's order (work order = caller order)
 __global__ void k_pack_pos(const double *__restrict__ x0, const double *__restrict__ x1,
                            const double *__restrict__ x2, int D, int N,

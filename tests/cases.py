@@ -48,11 +48,31 @@ def jitter3d(n=12, seed=11, **kw):
     return p, dict(x=X, y=Y, z=Z, vx=vx, vy=vy, vz=vz, m=m, u=u)
 
 
+def jitter3d_open(n=10, seed=5, **kw):
+    """The jittered 3D set without periodic boundaries (PERIODIC_BOUNDARIES 0:
+    domain limits and grid from the particles every step).  The particle with
+    the largest x is moved to index 0, where the reference's else-if maximum
+    (Particles.cpp:240-244) skips it: the grid's upper x limit is the second
+    largest x and that particle lies past the last cell (clamped)."""
+    p, s = jitter3d(n, seed, periodic=0, **kw)
+    k = int(np.argmax(s["x"]))
+    order = np.r_[k, np.delete(np.arange(len(s["x"])), k)]
+    return p, {key: v[order] for key, v in s.items()}
+
+
+def kh_open(n=40, **kw):
+    """The KH lattice without periodic boundaries (2D open domain)."""
+    p = default_params(kernel_size=4.0 / n, periodic=0, **kw)
+    return p, ics.kelvin_helmholtz(n)
+
+
 CASES = {
     "kh100": kh_lattice,
     "khrand48": kh_random,
     "blast16": blast3d,
     "jitter12": jitter3d,
+    "khopen40": kh_open,
+    "jitteropen10": jitter3d_open,
 }
 
 

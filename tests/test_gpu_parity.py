@@ -183,3 +183,25 @@ def test_work_order_invariance(name, hip):
         assert np.array_equal(a, b)
     for k in s0:
         assert np.array_equal(s0[k], s1[k]), k
+
+
+@pytest.mark.parametrize("name", ["khopen40", "jitteropen10"])
+def test_multistep_open_domain(name, hip, oracle_mod):
+    """PERIODIC_BOUNDARIES 0 over several steps: the grid follows the moving
+    particles' limits every step; states stay within 1e-9 of the oracle and
+    mass, momentum and energy are conserved to round-off."""
+    p, s = CASES[name](debug_level=0)
+    sim = hip.Simulation(p, s)
+    orc = oracle_mod.Oracle(p, s)
+    c0 = sim.conserved()
+    for k in range(5):
+        dt_o = orc.step()
+        dt_g = sim.step()
+        assert abs(dt_g - dt_o) <= 1e-11 * dt_o
+    gs, os_ = sim.state(), orc.state()
+    for k in gs:
+        assert relerr(gs[k], os_[k]) < 1e-9, k
+    c1 = sim.conserved()
+    assert abs(c1[1] - c0[1]) <= 1e-13 * abs(c0[1])
+    assert abs(c1[2] - c0[2]) <= 1e-12 * abs(c0[2])
+    sim.close()

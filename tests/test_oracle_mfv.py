@@ -306,3 +306,45 @@ def test_limiter_int_abs_quirk(oracle_mod):
     for mode in (0, MLH_ABS_FABS):
         gl = _grads(oracle_mod, default_params(kernel_size=3.2 / n, limiter_abs=mode), lin)[1]
         assert np.allclose(gl[inner], [0.4, -0.9], atol=1e-10)
+
+
+def test_open_domain_neighbours_and_limits(oracle_mod):
+    """PERIODIC_BOUNDARIES 0: neighbour sets equal a numpy brute force (no
+    images), in the reference's order: the grid spans the particles' limits
+    as getDomainLimits computes them (the maximum is an else-branch of the
+    minimum test, so a particle that is a new running minimum never counts
+    for the maximum), stencil cells x outer .. z inner, ascending index inside
+    a cell, particles past the last cell clamped into it."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from cases import jitter3d_open
+    p, s = jitter3d_open()
+    o = oracle_mod.Oracle(p, s)
+    o.neighbours()
+    noi, nog, nl = o.get_neighbours()
+    assert np.all(nog == 0)
+    X = np.stack([s["x"], s["y"], s["z"]], 1)
+    N, h = len(X), p.kernel_size
+    lo, hi = np.zeros(3), np.zeros(3)
+    for a in range(3):
+        mn, mx = np.finfo(float).max, np.finfo(float).tiny
+        for v in X[:, a]:
+            if v < mn:
+                mn = v
+            elif v > mx:
+                mx = v
+        lo[a], hi[a] = mn, mx
+    assert hi[0] < X[0, 0]  # the quirk is exercised on x
+    nc = np.floor((hi - lo) / h).astype(int)
+    cs = (hi - lo) / nc
+    f = np.clip(np.floor((X - lo) / cs).astype(int), 0, nc - 1)
+    for i in range(0, N, 37):
+        d2 = ((X[:, 0] - X[i, 0]) * (X[:, 0] - X[i, 0]) + (X[:, 1] - X[i, 1]) * (X[:, 1] - X[i, 1])) \
+            + (X[:, 2] - X[i, 2]) * (X[:, 2] - X[i, 2])
+        hit = np.nonzero(d2 < h * h)[0]
+        hit = hit[hit != i]
+        # reference order: stencil cell (x outer, z inner), then index
+        key = [(f[j, 0] - f[i, 0], f[j, 1] - f[i, 1], f[j, 2] - f[i, 2], j) for j in hit]
+        assert all(max(abs(k[0]), abs(k[1]), abs(k[2])) <= 1 for k in key)
+        expect = [k[3] for k in sorted(key)]
+        assert list(nl[i, : noi[i]]) == [e | (13 << 27) for e in expect], i
