@@ -190,27 +190,56 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
         for (int f = 0; f < NF; ++f) fi[f] = pri[field_slot<D>(f)];
         const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
         int cnt = a.noi[t] + a.nog[t], nr = a.noi[t];
-        // pass 1: E_i and the signal speed (real neighbours only)
+        // pass 1: E_i and the signal speed (real neighbours only).  FP32
+        // mode also accumulates G_f = sum_j (f_j - f_i)(x_j - x_i) W/omega_i
+        // and the neighbour extrema here: grad f = sum_j (f_j - f_i) psi_j
+        // = B G_f, so pass 2 is not needed.  FP64 mode keeps the reference's
+        // per-neighbour psi_j and its summation order (pass 2).
+        constexpr bool kFused = sizeof(T) == 4;
         double E[D * D];
         for (int k = 0; k < D * D; ++k) E[k] = 0.;
         double vsig = DBL_MIN;
         double ci = (double)pri[kPrimC];
         double vi[3] = {a.v[0][i], a.v[1][i], D == 3 ? a.v[2][i] : 0.};
+        T grad[NF][D];
+        T ngbmax[NF], ngbmin[NF];
+        T G[kFused ? NF : 1][D];
+        for (int f = 0; f < NF; ++f) {
+            for (int ax = 0; ax < D; ++ax) grad[f][ax] = T(0);
+            ngbmax[f] = tmin_pos<T>();
+            ngbmin[f] = tmax_val<T>();
+        }
+        for (int f = 0; f < (kFused ? NF : 1); ++f)
+            for (int ax = 0; ax < D; ++ax) G[f][ax] = T(0);
         for (int n = 0; n < cnt; ++n) {
             double xj[3];
             int s[3];
             int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
             T r = tsqrt(T(dist2<D>(xi, xj)));
-            double w = (double)(kernel_w<T>(r, h2, sig) / om_i);
+            T wt = kernel_w<T>(r, h2, sig) / om_i;
+            double w = (double)wt;
             for (int p = 0; p < D; ++p)
                 for (int q = 0; q < D; ++q) E[D * p + q] += (xj[p] - xi[p]) * (xj[q] - xi[q]) * w;
+            T prj[kPrimN];
+            if (kFused) {
+                load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
+                T dx[D];
+                for (int p = 0; p < D; ++p) dx[p] = T(xj[p] - xi[p]) * wt;
+                for (int f = 0; f < (kFused ? NF : 1); ++f) {
+                    T fj = prj[field_slot<D>(f)];
+                    T df = fj - fi[f];
+                    for (int p = 0; p < D; ++p) G[f][p] += df * dx[p];
+                    if (ngbmax[f] < fj) ngbmax[f] = fj;
+                    if (ngbmin[f] > fj) ngbmin[f] = fj;
+                }
+            }
             if (n < nr) {
-                const T *prj = prim + (size_t)j * kPrimN;
-                double cj = (double)prj[kPrimC];
+                const T *pj = prim + (size_t)j * kPrimN;
+                double cj = kFused ? (double)prj[kPrimC] : (double)pj[kPrimC];
                 double xij[3], vij[3];
                 for (int ax = 0; ax < D; ++ax) {
                     xij[ax] = xi[ax] - xj[ax];
-                    vij[ax] = vi[ax] - (double)prj[kPrimV + ax];
+                    vij[ax] = vi[ax] - (kFused ? (double)prj[kPrimV + ax] : (double)pj[kPrimV + ax]);
                 }
                 double vx = dotT<D, double>(vij, xij) / sqrt(dotT<D, double>(xij, xij));
                 vx = vx < 0. ? vx : 0.;
@@ -221,32 +250,35 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
         invert<D>(E);
         T B[D * D];
         for (int k = 0; k < D * D; ++k) B[k] = T(E[k]);
-        // pass 2: gradients and neighbour extrema
-        T grad[NF][D];
-        T ngbmax[NF], ngbmin[NF];
-        for (int f = 0; f < NF; ++f) {
-            for (int ax = 0; ax < D; ++ax) grad[f][ax] = T(0);
-            ngbmax[f] = tmin_pos<T>();
-            ngbmin[f] = tmax_val<T>();
-        }
-        for (int n = 0; n < cnt; ++n) {
-            double xj[3];
-            int s[3];
-            int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
-            T r = tsqrt(T(dist2<D>(xi, xj)));
-            T w = kernel_w<T>(r, h2, sig) / om_i;
-            T psi[D];
-            for (int p = 0; p < D; ++p) {
-                psi[p] = T(0);
-                for (int q = 0; q < D; ++q) psi[p] += B[p * D + q] * T(xj[q] - xi[q]) * w;
-            }
-            T prj[kPrimN];
-            load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
-            for (int f = 0; f < NF; ++f) {
-                T fj = prj[field_slot<D>(f)];
-                for (int p = 0; p < D; ++p) grad[f][p] += (fj - fi[f]) * psi[p];
-                if (ngbmax[f] < fj) ngbmax[f] = fj;
-                if (ngbmin[f] > fj) ngbmin[f] = fj;
+        if (kFused) {
+            // grad f = B G_f, the product in FP64
+            for (int f = 0; f < (kFused ? NF : 1); ++f)
+                for (int p = 0; p < D; ++p) {
+                    double g = 0.;
+                    for (int q = 0; q < D; ++q) g += E[p * D + q] * (double)G[f][q];
+                    grad[f][p] = T(g);
+                }
+        } else {
+            // pass 2: gradients and neighbour extrema
+            for (int n = 0; n < cnt; ++n) {
+                double xj[3];
+                int s[3];
+                int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
+                T r = tsqrt(T(dist2<D>(xi, xj)));
+                T w = kernel_w<T>(r, h2, sig) / om_i;
+                T psi[D];
+                for (int p = 0; p < D; ++p) {
+                    psi[p] = T(0);
+                    for (int q = 0; q < D; ++q) psi[p] += B[p * D + q] * T(xj[q] - xi[q]) * w;
+                }
+                T prj[kPrimN];
+                load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
+                for (int f = 0; f < NF; ++f) {
+                    T fj = prj[field_slot<D>(f)];
+                    for (int p = 0; p < D; ++p) grad[f][p] += (fj - fi[f]) * psi[p];
+                    if (ngbmax[f] < fj) ngbmax[f] = fj;
+                    if (ngbmin[f] > fj) ngbmin[f] = fj;
+                }
             }
         }
         // pass 3: slope limiter.  The reconstructed extrema and the ratios
