@@ -113,7 +113,7 @@ typedef struct mlh_diag {
     int64_t neg_face_pressure_real;   /* Particles.cpp:1813-1834              */
     int64_t neg_face_pressure_ghost;  /* Particles.cpp:1873-1882              */
     int64_t vacuum_faces;             /* Riemann.cpp:112-114,125-127          */
-    int64_t nonconverged_faces;       /* Newton cap reached                   */
+    int64_t nonconverged_faces;       /* Newton cap reached (bisection solved) */
     int64_t missing_mirror_pairs;     /* neighbour seen from one side only    */
     int64_t learned_fallbacks;        /* learned seed rejected by FP64 test   */
     int64_t newton_iterations;        /* total over all solved faces          */

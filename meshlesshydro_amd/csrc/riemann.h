@@ -131,6 +131,35 @@ __device__ inline T guess_p(const GasConst<T> &c, T dl, T ul, T pl, T cl, T dr,
     return pm;
 }
 
+// Newton missed the change test within max_iter (near-vacuum states: f has
+// an infinite slope at p = 0 and the iterates jump between the negative-
+// pressure reset and overshoots).  f(p) = f_L + f_R + du increases with p
+// and f(0) = du - g4 (c_L + c_R) < 0 when no vacuum forms, so bisect a
+// bracket [0, hi] to the same relative change test.  nb: halvings done.
+__device__ inline double bisect_pstar(const GasConst<double> &c, double dl, double pl,
+                                      double cl, double dr, double pr, double cr, double du,
+                                      double tol, int &nb) {
+    double lo = 0.0, hi = pl > pr ? pl : pr, fl, fld, fr, frd;
+    for (int k = 0; k < 200; ++k) {  // f grows like sqrt(p) on the shock branch
+        prefun(c, hi, dl, pl, cl, fl, fld);
+        prefun(c, hi, dr, pr, cr, fr, frd);
+        if (fl + fr + du >= 0.0) break;
+        lo = hi;
+        hi *= 2.0;
+    }
+    nb = 0;
+    while (nb < 200) {
+        ++nb;
+        double mid = 0.5 * (lo + hi);
+        prefun(c, mid, dl, pl, cl, fl, fld);
+        prefun(c, mid, dr, pr, cr, fr, frd);
+        if (fl + fr + du < 0.0) lo = mid;
+        else hi = mid;
+        if (2.0 * (hi - lo) <= tol * (hi + lo)) break;
+    }
+    return 0.5 * (lo + hi);
+}
+
 template <typename T>
 struct RSol {
     T d, u, p;
@@ -240,7 +269,12 @@ __device__ inline RSol<double> solve_fp64(const GasConst<double> &c, double dl,
         if (p < 0.0) p = tiny;
         pold = p;
     }
-    if (it > max_iter) { st.nonconv += 1; it = max_iter + 1; }
+    if (it > max_iter) {
+        st.nonconv += 1;
+        int nb = 0;
+        p = bisect_pstar(c, dl, pl, cl, dr, pr, cr, du, tol, nb);
+        it = max_iter + 1 + nb;
+    }
     st.iters += it;
     prefun(c, p, dl, pl, cl, fl, fld);
     prefun(c, p, dr, pr, cr, fr, frd);
@@ -268,6 +302,11 @@ __device__ inline double star_pressure_fp64(const GasConst<double> &c, double dl
         if (change <= tol) break;
         if (p < 0.0) p = tiny;
         pold = p;
+    }
+    if (it > max_iter) {
+        int nb = 0;
+        p = bisect_pstar(c, dl, pl, cl, dr, pr, cr, du, tol, nb);
+        it = max_iter + 1 + nb;
     }
     st.iters += it;
     return p;

@@ -71,6 +71,8 @@ void orc_riemann_batch(int64_t n, double gamma, const double *rhoL,
                        int max_iter, double *rho_s, double *u_s, double *P_s,
                        int32_t *flag, int32_t *iters);
 /* the initial guess alone (for the learned-seed comparison) */
+double orc_riemann_pstar(double gamma, double dl, double ul, double pl, double dr, double ur,
+                         double pr, double tol, int max_iter, int *iters);
 double orc_riemann_guess(double gamma, double rhoL, double uL, double PL,
                          double rhoR, double uR, double PR);
 /* residual f(p) and f'(p) */

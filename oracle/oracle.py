@@ -54,6 +54,8 @@ def lib():
         L.orc_riemann_solve.argtypes = [C.c_double] * 7 + [C.c_double, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)]
         L.orc_riemann_batch.argtypes = [C.c_int64, C.c_double] + [_dp] * 6 + [C.c_double, C.c_int] + [_dp] * 3 + [_ip, _ip]
         L.orc_riemann_guess.restype = C.c_double
+        L.orc_riemann_pstar.argtypes = [C.c_double] * 8 + [C.c_int, C.POINTER(C.c_int)]
+        L.orc_riemann_pstar.restype = C.c_double
         L.orc_riemann_guess.argtypes = [C.c_double] * 7
         L.orc_riemann_f.argtypes = [C.c_double] * 8 + [_dp, _dp]
         L.orc_kernel.restype = C.c_double
@@ -248,3 +250,17 @@ def mlp_forward(params, feats):
     out = np.zeros(len(feats))
     lib().orc_mlp_forward(len(feats), _f(params), _f(feats), _d(out))
     return out
+
+
+def riemann_pstar(gamma, dl, ul, pl, dr, ur, pr, tol=1e-6, max_iter=50):
+    """Star pressure of the oracle's exact solver (0 for vacuum) and the
+    iterations it took (Newton, then bisection beyond max_iter)."""
+    L = lib()
+    n = len(dl)
+    out, its = np.zeros(n), np.zeros(n, np.int32)
+    it = C.c_int()
+    for k in range(n):
+        out[k] = L.orc_riemann_pstar(gamma, dl[k], ul[k], pl[k], dr[k], ur[k], pr[k], tol,
+                                     max_iter, C.byref(it))
+        its[k] = it.value
+    return out, its

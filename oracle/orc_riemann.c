@@ -171,6 +171,63 @@ static int sample_vacuum(const gconst *c, double dl, double ul, double pl,
     return 0;
 }
 
+/* Newton missed the change test within max_iter (near-vacuum states, where
+ * f has an infinite slope at p = 0): f(p) = fL + fR + du increases with p
+ * and f(0) = du - g4 (cL + cR) < 0 without vacuum, so bisect [0, hi] to the
+ * same relative change test.  *nb: halvings done. */
+static double bisect_pstar(const gconst *c, double dl, double pl, double cl, double dr,
+                           double pr, double cr, double du, double tol, int *nb) {
+    double lo = 0.0, hi = pl > pr ? pl : pr, fl, fld, fr, frd;
+    for (int k = 0; k < 200; ++k) {
+        prefun(c, hi, dl, pl, cl, &fl, &fld);
+        prefun(c, hi, dr, pr, cr, &fr, &frd);
+        if (fl + fr + du >= 0.0) break;
+        lo = hi;
+        hi *= 2.0;
+    }
+    *nb = 0;
+    while (*nb < 200) {
+        ++*nb;
+        double mid = 0.5 * (lo + hi);
+        prefun(c, mid, dl, pl, cl, &fl, &fld);
+        prefun(c, mid, dr, pr, cr, &fr, &frd);
+        if (fl + fr + du < 0.0) lo = mid;
+        else hi = mid;
+        if (2.0 * (hi - lo) <= tol * (hi + lo)) break;
+    }
+    return 0.5 * (lo + hi);
+}
+
+/* star pressure of the exact solver (0 for vacuum states), for tests */
+double orc_riemann_pstar(double gamma, double dl, double ul, double pl, double dr, double ur,
+                         double pr, double tol, int max_iter, int *iters) {
+    gconst c = mk(gamma);
+    *iters = 0;
+    if (!(dl > 0.0) || !(dr > 0.0)) return 0.0;
+    double cl = sqrt(gamma * pl / dl), cr = sqrt(gamma * pr / dr), du = ur - ul;
+    if (c.g4 * (cl + cr) <= du) return 0.0;
+    double tiny = tol * 0.5 * (pl + pr);
+    double pold = guess(&c, dl, ul, pl, cl, dr, ur, pr, cr, tiny);
+    double p = pold, fl, fld, fr, frd;
+    int it;
+    for (it = 1; it <= max_iter; ++it) {
+        prefun(&c, pold, dl, pl, cl, &fl, &fld);
+        prefun(&c, pold, dr, pr, cr, &fr, &frd);
+        p = pold - (fl + fr + du) / (fld + frd);
+        double change = 2.0 * fabs((p - pold) / (p + pold));
+        if (change <= tol) break;
+        if (p < 0.0) p = tiny;
+        pold = p;
+    }
+    *iters = it;
+    if (it > max_iter) {
+        int nb = 0;
+        p = bisect_pstar(&c, dl, pl, cl, dr, pr, cr, du, tol, &nb);
+        *iters = max_iter + 1 + nb;
+    }
+    return p;
+}
+
 int orc_riemann_solve(double gamma, double dl, double ul, double pl,
                       double dr, double ur, double pr, double tol,
                       int max_iter, double *d_s, double *u_s, double *p_s,
@@ -201,7 +258,12 @@ int orc_riemann_solve(double gamma, double dl, double ul, double pl,
         if (p < 0.0) p = tiny;
         pold = p;
     }
-    *iters = it > max_iter ? max_iter + 1 : it;
+    *iters = it;
+    if (it > max_iter) {
+        int nb = 0;
+        p = bisect_pstar(&c, dl, pl, cl, dr, pr, cr, du, tol, &nb);
+        *iters = max_iter + 1 + nb;
+    }
     prefun(&c, p, dl, pl, cl, &fl, &fld);
     prefun(&c, p, dr, pr, cr, &fr, &frd);
     double um = 0.5 * (ul + ur + fr - fl);

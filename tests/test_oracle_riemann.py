@@ -161,3 +161,41 @@ def test_face_flux_uniform_state_is_euler_flux(oracle_mod, dim):
                              rho[:, None] * v * wA[:, None] + P[:, None] * A], 1)
     assert np.allclose(F, expect, rtol=1e-9, atol=1e-12)
     assert np.all(np.abs(fl) == 1)
+
+
+def test_near_vacuum_newton_fallback(oracle_mod):
+    """States just short of vacuum generation (du slightly below
+    g4 (c_L + c_R)), with very unequal sides: f has an infinite slope at
+    p = 0 and Newton from Toro's guess bounces off the negative-pressure
+    reset.  The solver must still return the positive root of f to the
+    tolerance (bisection after the Newton cap), checked against an
+    independent bracketing bisection of Toro's f (4.6)-(4.7)."""
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_toro_kats import f_side
+    g = 5.0 / 3.0
+    rng = np.random.default_rng(77)
+    n = 400
+    dl = np.exp(rng.uniform(np.log(1e-2), np.log(1e2), n))
+    dr = np.exp(rng.uniform(np.log(1e-2), np.log(1e2), n))
+    pl = np.exp(rng.uniform(np.log(1e-3), np.log(1e2), n))
+    pr = np.exp(rng.uniform(np.log(1e-3), np.log(1e2), n))
+    cl, cr = np.sqrt(g * pl / dl), np.sqrt(g * pr / dr)
+    du = (2.0 / (g - 1.0)) * (cl + cr) * rng.uniform(0.9, 0.999, n)
+    ul = -0.5 * du
+    ur = 0.5 * du
+    rs, us, ps, fl, it = oracle_mod.riemann_batch(g, dl, ul, pl, dr, ur, pr, tol=1e-6, max_iter=50)
+    assert np.all(np.isfinite(rs)) and np.all(np.isfinite(us)) and np.all(np.isfinite(ps))
+    pstar, its = oracle_mod.riemann_pstar(g, dl, ul, pl, dr, ur, pr, tol=1e-6, max_iter=50)
+    assert np.all(pstar > 0)
+    assert np.any(its > 51), "no case needed the fallback"
+    for k in range(n):
+        F = lambda p: f_side(p, dl[k], pl[k], g) + f_side(p, dr[k], pr[k], g) + (ur[k] - ul[k])
+        lo, hi = 0.0, max(pl[k], pr[k])
+        while F(hi) < 0.0:
+            hi *= 2.0
+        for _ in range(300):
+            mid = 0.5 * (lo + hi)
+            lo, hi = (mid, hi) if F(mid) < 0.0 else (lo, mid)
+        p_ref = 0.5 * (lo + hi)
+        assert abs(pstar[k] - p_ref) <= 2e-6 * p_ref, (k, pstar[k], p_ref, its[k])
