@@ -241,6 +241,8 @@ def main():
         "kernel_ms": [round(float(x), 4) for x in head["tim"]],
         "newton_iters_per_face": head["diag"]["newton_iterations"] / max(1, head["diag"]["faces_solved"]),
         "fallback_rate": head["diag"]["learned_fallbacks"] / max(1, head["diag"]["faces_solved"]),
+        "seeded_rate": head["diag"]["learned_seeded"] / max(1, head["diag"]["faces_solved"]),
+        "nonconverged_rate": head["diag"]["nonconverged_faces"] / max(1, head["diag"]["faces_solved"]),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_faces (faces + Riemann + flux + update)"},

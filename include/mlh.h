@@ -118,6 +118,8 @@ typedef struct mlh_diag {
     int64_t learned_fallbacks;        /* learned seed rejected by FP64 test   */
     int64_t newton_iterations;        /* total over all solved faces          */
     int64_t faces_solved;             /* Riemann solves performed             */
+    int64_t learned_seeded;           /* faces whose Newton started from the  *
+                                       * network's p* (strong jumps only)     */
     int32_t max_neighbours;           /* largest real neighbour count          */
     int32_t max_ghost_neighbours;     /* largest periodic-image neighbour count */
 } mlh_diag;

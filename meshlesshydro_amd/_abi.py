@@ -72,6 +72,7 @@ class Diag(C.Structure):
         ("learned_fallbacks", C.c_int64),
         ("newton_iterations", C.c_int64),
         ("faces_solved", C.c_int64),
+        ("learned_seeded", C.c_int64),
         ("max_neighbours", C.c_int32),
         ("max_ghost_neighbours", C.c_int32),
     ]

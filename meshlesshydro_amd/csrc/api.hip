@@ -742,6 +742,7 @@ int mlh_get_diagnostics(mlh_ctx *c, mlh_diag *d) {
     d->learned_fallbacks = (int64_t)k[kCntFallback];
     d->newton_iterations = (int64_t)k[kCntIters];
     d->faces_solved = (int64_t)k[kCntFaces];
+    d->learned_seeded = (int64_t)k[kCntSeeded];
     d->max_neighbours = mc[0];
     d->max_ghost_neighbours = mc[1];
     return MLH_OK;

@@ -408,7 +408,7 @@ __device__ inline View<D, T> make_view(const HydroArgs &a, int j) {
 #define PD_B(d, p_, q_) (d).f[FRec<D>::B + (p_) * D + (q_)]
 
 struct FaceStats {
-    int neg_real, neg_ghost, vacuum;
+    int neg_real, neg_ghost, vacuum, seeded;
     SolveStats st;
 };
 
@@ -762,11 +762,14 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
             // is already converged to O(jump^2) there
             bool strong = false;
             if (act) {
+                // |dp| > k (pL + pR)  or  du^2 > 2 k^2 gamma (pR/rhoR + pL/rhoL),
+                // the latter multiplied through by rhoR rhoL (no divisions)
                 T dp = fp.WR[1] - fp.WL[1], du = fp.uj - fp.ui;
                 T psum = fp.WR[1] + fp.WL[1];
-                T c2 = T(a.gamma) * (fp.WR[1] / fp.WR[0] + fp.WL[1] / fp.WL[0]);
+                T rr = fp.WR[0] * fp.WL[0];
+                T c2r = T(a.gamma) * (fp.WR[1] * fp.WL[0] + fp.WL[1] * fp.WR[0]);
                 strong = dp * dp > T(kSeedJump * kSeedJump) * psum * psum ||
-                         du * du > T(2 * kSeedJump * kSeedJump) * c2;
+                         du * du * rr > T(2 * kSeedJump * kSeedJump) * c2r;
             }
             if (__ballot(strong)) {
                 float f[kMlpIn];
@@ -781,6 +784,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
                     for (int q = 0; q < kMlpIn; ++q) f[q] = 0.f;
                 float y = mlp_infer_wave_bf16_lean(a.mlp_frags, f, lds[threadIdx.x >> 6]);
                 seed = ok ? mlp_seed_from_output(y, f, pref) : -1.f;
+                fs.seeded += ok ? 1 : 0;
             }
         }
         if (act) {
@@ -847,7 +851,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
         (unsigned long long)fs.neg_real, (unsigned long long)fs.neg_ghost,
         (unsigned long long)fs.vacuum, (unsigned long long)fs.st.nonconv, 0ull,
         (unsigned long long)fs.st.fallback, (unsigned long long)fs.st.iters,
-        (unsigned long long)cnt};
+        (unsigned long long)cnt, (unsigned long long)fs.seeded};
     for (int k = 0; k < kNumCounters; ++k) {
         unsigned long long v = wave_sum(c[k]);
         if (lane_id() == 0 && v) atomicAdd(&a.counters[k], v);

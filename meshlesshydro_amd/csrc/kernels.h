@@ -57,7 +57,7 @@ struct HydroArgs {
 
 enum CounterIdx {
     kCntNegReal = 0, kCntNegGhost, kCntVacuum, kCntNonconv, kCntMissing,
-    kCntFallback, kCntIters, kCntFaces, kNumCounters
+    kCntFallback, kCntIters, kCntFaces, kCntSeeded, kNumCounters
 };
 
 void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
