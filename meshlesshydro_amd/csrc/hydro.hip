@@ -589,11 +589,27 @@ __device__ inline void face_prepare(const HydroArgs &a, const View<D, T> &L,
     fp.ui = ui; fp.uj = uj; fp.ti0 = ti0; fp.tj0 = tj0;
 }
 
+// FP32 faces: the first step of the mixed solver (an FP64 Newton step at
+// the standard guess) on the face's 1D problem; valid = false for a face
+// without area
+template <int D>
+__device__ inline MixedProbe face_probe(const HydroArgs &a, const FacePrep<D, float> &fp,
+                                        SolveStats &st) {
+    MixedProbe m{};
+    if (!(dotT<D, float>(fp.A, fp.A) > 0.f)) return m;
+    GasConst<float> gf(a.gamma);
+    GasConst<double> gc(a.gamma);
+    return mixed_probe(gf, gc, fp.WR[0], fp.ui, fp.WR[1], fp.WL[0], fp.uj, fp.WL[1],
+                       a.riemann_tol, -1.f, st);
+}
+
 // exact Riemann solve from the FacePrep states and the flux projected on A
-// (Riemann.cpp:83-229).  seed > 0: Newton starts there (learned seed).
+// (Riemann.cpp:83-229).  seed > 0: Newton starts there (learned seed).  FP32
+// with a probe: the solve continues from face_probe, and seed > 0 restarts
+// the FP32 iterations there.
 template <int D, typename T>
 __device__ inline int face_finish(const HydroArgs &a, const FacePrep<D, T> &fp, float seed,
-                                  T *F, FaceStats &fs) {
+                                  T *F, FaceStats &fs, const MixedProbe *probe = nullptr) {
     const T *A = fp.A, *hA = fp.hA, *WR = fp.WR, *WL = fp.WL, *vF = fp.vF;
     const T ui = fp.ui, uj = fp.uj, ti0 = fp.ti0, tj0 = fp.tj0;
     const T gam = T(a.gamma);
@@ -613,8 +629,11 @@ __device__ inline int face_finish(const HydroArgs &a, const FacePrep<D, T> &fp, 
     } else {
         GasConst<float> gf(a.gamma);
         GasConst<double> gc(a.gamma);
-        RSol<float> r = solve_mixed(gf, gc, WR[0], ui, WR[1], WL[0], uj, WL[1], a.riemann_tol,
-                                    a.riemann_max_iter, a.learned_iters, seed, fs.st);
+        RSol<float> r =
+            probe ? mixed_finish(gf, gc, WR[0], ui, WR[1], WL[0], uj, WL[1], a.riemann_tol,
+                                 a.riemann_max_iter, a.learned_iters, seed, *probe, fs.st)
+                  : solve_mixed(gf, gc, WR[0], ui, WR[1], WL[0], uj, WL[1], a.riemann_tol,
+                                a.riemann_max_iter, a.learned_iters, seed, fs.st);
         rs = r.d; us = r.u; Ps = r.p; flag = r.flag;
     }
     if (flag == 0) fs.vacuum++;
@@ -733,10 +752,13 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     }
     const bool collect = a.collect && live && (i % a.collect_stride) == 0;
     double mF = 0., eF = 0., vFs[3] = {0., 0., 0.};
+    constexpr bool kFp32 = sizeof(T) == 4;
     for (int n = 0; n < lim; ++n) {
         const bool act = n < cnt;
         FacePrep<D, T> fp;
+        MixedProbe mp{};
         bool self_lo = true;
+        const int it0 = fs.st.iters;
         if (act) {
             uint32_t e = a.nbr[(size_t)n * N + t];
             int j = (int)nb_index(e);
@@ -754,14 +776,19 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
             int sl[3];
             for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
             face_prepare<D, T>(a, L, H, sl, image, dt, fp, fs);
+            if constexpr (kFp32) mp = face_probe<D>(a, fp, fs.st);
         }
         float seed = -1.f;
         if (LEARNED) {
-            // the network only runs for wavefronts holding a face with a
-            // real jump; near-uniform faces keep the standard guess, which
-            // is already converged to O(jump^2) there
+            // FP32: the network runs for the faces whose FP64 step at the
+            // standard guess failed the change test (Newton has to iterate),
+            // on wavefronts holding at least one.  FP64: for faces with a
+            // real jump (near-uniform faces keep the standard guess, which is
+            // converged to O(jump^2) there).
             bool strong = false;
-            if (act) {
+            if (kFp32) {
+                strong = act && mp.valid && !mp.accepted;
+            } else if (act) {
                 // |dp| > k (pL + pR)  or  du^2 > 2 k^2 gamma (pR/rhoR + pL/rhoL),
                 // the latter multiplied through by rhoR rhoL (no divisions)
                 T dp = fp.WR[1] - fp.WL[1], du = fp.uj - fp.ui;
@@ -789,8 +816,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
         }
         if (act) {
             T F[NF];
-            int it0 = fs.st.iters;
-            face_finish<D, T>(a, fp, seed, F, fs);
+            face_finish<D, T>(a, fp, seed, F, fs, kFp32 ? &mp : nullptr);
             // training data: faces the exact solver needed >= 2 iterations for
             if (collect && self_lo && fs.st.iters - it0 >= 2) {
                 unsigned long long slot = atomicAdd(a.collect_n, 1ull);

@@ -11,12 +11,10 @@
 //
 // Precision variants:
 //   solve_fp64   every operation in FP64 (the plain exact solver);
-//   solve_mixed  an FP64 Newton step at the seed (standard guess or the
-//                learned one); if it fails the exact solver's relative-change
-//                test, FP32 Newton iterations from there and one more FP64
-//                step at the FP32 root.  A face is accepted only when an FP64
-//                step passes the test, otherwise solve_fp64 runs from the
-//                standard guess.  Sampling uses the FP64-corrected p*.
+//   solve_mixed  mixed_probe then mixed_finish (below): an FP64 Newton step
+//                at the seed or the standard guess, FP32 Newton only when
+//                that step fails the exact solver's relative-change test,
+//                an FP64 acceptance step, solve_fp64 as the fallback.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -312,62 +310,99 @@ __device__ inline double star_pressure_fp64(const GasConst<double> &c, double dl
     return p;
 }
 
-// FP64 residual step at p_seed, FP32 Newton if needed, FP64 fallback.
-// Inputs are the FP32 face states; p_seed <= 0 selects the standard FP32
-// guess.  One call site of the FP64 solver (vacuum states and rejected faces
-// share it).
+// The mixed solver in two steps, so that work between them (the learned
+// seed, for the faces that need one) can run on a whole wavefront:
+//   mixed_probe   one FP64 Newton step at the start point (a seed, or the
+//                 standard guess); in smooth flow the guess is exact to
+//                 O(jump^2) and this step already passes the exact solver's
+//                 relative-change test;
+//   mixed_finish  otherwise FP32 Newton iterations from a restart point (a
+//                 seed) or from the probe's FP64 step, then one FP64 step at
+//                 the FP32 root; a face is accepted only when an FP64 step
+//                 passes the test, else solve_fp64 runs from the standard
+//                 guess (vacuum states and rejected faces share that call).
+// Sampling uses the FP64-corrected p* and contact velocity.
+struct MixedProbe {
+    float p;          // FP32 iterate to continue from
+    float cl, cr;     // FP32 sound speeds
+    double pn, um;    // FP64 root and contact velocity once accepted
+    bool valid;       // positive densities and no vacuum generated
+    bool accepted;    // an FP64 step passed the change test
+};
+
+__device__ inline MixedProbe mixed_probe(const GasConst<float> &cf, const GasConst<double> &cd,
+                                         float dl, float ul, float pl, float dr, float ur,
+                                         float pr, double tol, float p_start, SolveStats &st) {
+    MixedProbe m{};
+    bool ok = dl > 0.f && dr > 0.f;
+    m.cl = ok ? __builtin_sqrtf(cf.g * pl / dl) : 0.f;
+    m.cr = ok ? __builtin_sqrtf(cf.g * pr / dr) : 0.f;
+    m.valid = ok && cf.g4 * (m.cl + m.cr) > ur - ul;
+    if (!m.valid) return m;
+    float tiny = 1e-6f * 0.5f * (pl + pr);
+    float p = p_start > 0.f ? p_start : guess_p(cf, dl, ul, pl, m.cl, dr, ur, pr, m.cr, tiny);
+    const double dld = dl, pld = pl, drd = dr, prd = pr, dud = (double)ur - (double)ul;
+    const double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
+    double fl, fld, fr, frd, pd = p;
+    prefun(cd, pd, dld, pld, cld, fl, fld);
+    prefun(cd, pd, drd, prd, crd, fr, frd);
+    double dp = (fl + fr + dud) / (fld + frd);
+    double pn = pd - dp;
+    st.iters += 1;
+    m.accepted = 2.0 * fabs(dp / (pn + pd)) <= tol && pn > 0.0;
+    m.pn = pn;
+    m.um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (frd - fld) * dp);
+    m.p = pn > 0.0 ? (float)pn : p;
+    return m;
+}
+
+__device__ inline RSol<float> mixed_finish(const GasConst<float> &cf, const GasConst<double> &cd,
+                                           float dl, float ul, float pl, float dr, float ur,
+                                           float pr, double tol, int max_iter, int iters32,
+                                           float restart, const MixedProbe &m, SolveStats &st) {
+    bool ok = m.valid && m.accepted;
+    double pn = m.pn, um = m.um;
+    if (m.valid && !m.accepted) {
+        const float du = ur - ul, tiny = 1e-6f * 0.5f * (pl + pr);
+        float p = restart > 0.f ? restart : m.p;
+        int it = 0;
+        for (; it < iters32; ++it) {
+            float gl, gld, gr, grd;
+            prefun(cf, p, dl, pl, m.cl, gl, gld);
+            prefun(cf, p, dr, pr, m.cr, gr, grd);
+            float q = p - (gl + gr + du) / (gld + grd);
+            if (!(q > 0.f)) q = tiny;
+            float ch = __builtin_fabsf(q - p) / (q + p);
+            p = q;
+            if (ch < 2e-6f) { ++it; break; }
+        }
+        st.iters += it;
+        const double dld = dl, pld = pl, drd = dr, prd = pr, dud = (double)ur - (double)ul;
+        const double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
+        double fl, fld, fr, frd, pd = p;
+        prefun(cd, pd, dld, pld, cld, fl, fld);
+        prefun(cd, pd, drd, prd, crd, fr, frd);
+        double dp = (fl + fr + dud) / (fld + frd);
+        pn = pd - dp;
+        st.iters += 1;
+        ok = 2.0 * fabs(dp / (pn + pd)) <= tol && pn > 0.0;
+        if (!ok) st.fallback += 1;
+        um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (frd - fld) * dp);
+    }
+    if (ok) return sample(cf, float(pn), float(um), dl, ul, pl, m.cl, dr, ur, pr, m.cr);
+    RSol<double> r = solve_fp64(cd, dl, ul, pl, dr, ur, pr, tol, max_iter, -1.0, st);
+    return RSol<float>{float(r.d), float(r.u), float(r.p), r.flag};
+}
+
+// probe at p_seed (the standard guess when <= 0), then finish
 __device__ inline RSol<float> solve_mixed(const GasConst<float> &cf,
                                           const GasConst<double> &cd, float dl,
                                           float ul, float pl, float dr,
                                           float ur, float pr, double tol,
                                           int max_iter, int iters32,
                                           float p_seed, SolveStats &st) {
-    bool ok = dl > 0.f && dr > 0.f;
-    float cl = ok ? __builtin_sqrtf(cf.g * pl / dl) : 0.f;
-    float cr = ok ? __builtin_sqrtf(cf.g * pr / dr) : 0.f;
-    float du = ur - ul;
-    ok = ok && cf.g4 * (cl + cr) > du;
-    double pn = 0.0, um = 0.0;
-    if (ok) {
-        float tiny = 1e-6f * 0.5f * (pl + pr);
-        float p = p_seed > 0.f ? p_seed : guess_p(cf, dl, ul, pl, cl, dr, ur, pr, cr, tiny);
-        const double dld = dl, pld = pl, drd = dr, prd = pr, dud = (double)ur - (double)ul;
-        const double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
-        double fl, fld, fr, frd, dp;
-        // FP64 Newton step at the start point: a start within the tolerance
-        // (smooth flow: the standard guess is exact to O(jump^2)) is accepted
-        // at once; otherwise FP32 Newton continues from that step and one
-        // more FP64 step at the FP32 root decides.
-        for (int pass = 0;; ++pass) {
-            double pd = p;
-            prefun(cd, pd, dld, pld, cld, fl, fld);
-            prefun(cd, pd, drd, prd, crd, fr, frd);
-            dp = (fl + fr + dud) / (fld + frd);
-            pn = pd - dp;
-            double change = 2.0 * fabs(dp / (pn + pd));
-            ok = change <= tol && pn > 0.0;
-            st.iters += 1;
-            if (ok || pass == 1) break;
-            if (pn > 0.0) p = (float)pn;
-            int it = 0;
-            for (; it < iters32; ++it) {
-                float gl, gld, gr, grd;
-                prefun(cf, p, dl, pl, cl, gl, gld);
-                prefun(cf, p, dr, pr, cr, gr, grd);
-                float q = p - (gl + gr + du) / (gld + grd);
-                if (!(q > 0.f)) q = tiny;
-                float ch = __builtin_fabsf(q - p) / (q + p);
-                p = q;
-                if (ch < 2e-6f) { ++it; break; }
-            }
-            st.iters += it;
-        }
-        if (!ok) st.fallback += 1;
-        um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (frd - fld) * dp);
-    }
-    if (ok) return sample(cf, float(pn), float(um), dl, ul, pl, cl, dr, ur, pr, cr);
-    RSol<double> r = solve_fp64(cd, dl, ul, pl, dr, ur, pr, tol, max_iter, -1.0, st);
-    return RSol<float>{float(r.d), float(r.u), float(r.p), r.flag};
+    MixedProbe m = mixed_probe(cf, cd, dl, ul, pl, dr, ur, pr, tol, p_seed, st);
+    return mixed_finish(cf, cd, dl, ul, pl, dr, ur, pr, tol, max_iter, iters32, -1.f, m, st);
 }
 
 }  // namespace mlh
