@@ -47,6 +47,26 @@ def face_kernel_bytes(D, tsz, k_mean):
     return own + row + write
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the counters of a
+    1 GiB copy): the line `traffic_bytes_per_launch <kernel> = <bytes>` of
+    profiles/pmc_traffic_blast128.md, or None when absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "pmc_traffic_blast128.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                line = line.strip().strip("`")
+                if line.startswith("traffic_bytes_per_launch ") and "=" in line:
+                    name, val = line[len("traffic_bytes_per_launch "):].rsplit("=", 1)
+                    if name.strip() == kernel:
+                        return float(val)
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +212,7 @@ def main():
         head = dict(rate=learned[0], tim=learned[1], diag=learned[2], l1=l1_l, linf=linf_l,
                     seed="learned", train=learned[5])
 
+    face_kernel = "k_faces<3, float, %s>" % ("true" if head["seed"] == "learned" else "false")
     k_mean = (d64["faces_solved"] / max(1, (W + K) * N))
     tface = float(head["tim"][3]) * 1e-3
     bytes_face = face_kernel_bytes(3, 4, k_mean) * N
@@ -244,8 +265,9 @@ def main():
         "seeded_rate": head["diag"]["learned_seeded"] / max(1, head["diag"]["faces_solved"]),
         "nonconverged_rate": head["diag"]["nonconverged_faces"] / max(1, head["diag"]["faces_solved"]),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_faces (faces + Riemann + flux + update)"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(face_kernel),
+                     "kernel": face_kernel,
+                     "algorithmic_bytes_per_launch": bytes_face},
         "cpu_baseline": cpu,
     }
     if "train" in head:

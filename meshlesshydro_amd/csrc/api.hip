@@ -144,6 +144,22 @@ static int make_geom(mlh_ctx *c, const double *lo, const double *hi) {
     }
     if (ncells > (1ll << 27)) return fail(MLH_ERR_GRID, "too many search cells");
     g.ncells = (int)ncells;
+    // Morton keys when their table (2^(D bits) entries) stays within 2^24
+    // entries and 8x the cell count; linear keys otherwise
+    int nmax = 1;
+    for (int a = 0; a < c->D; ++a) nmax = g.nc[a] > nmax ? g.nc[a] : nmax;
+    int mb = 0;
+    while ((1 << mb) < nmax) ++mb;
+    long long mkeys = 1ll << (c->D * mb);
+    const int maxbits = c->D == 2 ? 16 : 10;
+    if (c->p.work_order != MLH_ORDER_CALLER && mb <= maxbits && mkeys <= (1ll << 24) &&
+        mkeys <= 8 * ncells) {
+        g.mbits = mb;
+        g.nkeys = (int)mkeys;
+    } else {
+        g.mbits = 0;
+        g.nkeys = g.ncells;
+    }
     // Two points with computed d^2 < h^2 differ by less than h per axis.
     // Their computed cell indices differ by at most 1 when the cell size
     // exceeds h by more than the rounding of floor((x-min)/cs) and of the
@@ -244,8 +260,8 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     A_(&c->cell_key, 4 * N); A_(&c->sorted_idx, 4 * N); A_(&c->sorted_key, 4 * N);
     A_(&c->sort_scratch, 4 * (2 * N + radix_sort_scratch_words(n)));
     if (p->periodic) {
-        A_(&c->cstart, 4 * (size_t)c->g.ncells); A_(&c->cend, 4 * (size_t)c->g.ncells);
-        c->cell_cap = c->g.ncells;
+        A_(&c->cstart, 4 * (size_t)c->g.nkeys); A_(&c->cend, 4 * (size_t)c->g.nkeys);
+        c->cell_cap = c->g.nkeys;
     } else {
         A_(&c->lim_scratch, sizeof(double) * 3 * D * (size_t)limits_blocks(n));
         A_(&c->lim_out, sizeof(double) * 2 * D);
@@ -364,13 +380,13 @@ static int prepare_open_grid(mlh_ctx *c) {
     }
     int st = make_geom(c, lo, hi);
     if (st) return st;
-    if (c->g.ncells > c->cell_cap) {
+    if (c->g.nkeys > c->cell_cap) {
         if (c->cstart) hipFree(c->cstart);
         if (c->cend) hipFree(c->cend);
         c->cstart = c->cend = nullptr;
-        HIPCHK(hipMalloc(&c->cstart, 4 * (size_t)c->g.ncells));
-        HIPCHK(hipMalloc(&c->cend, 4 * (size_t)c->g.ncells));
-        c->cell_cap = c->g.ncells;
+        HIPCHK(hipMalloc(&c->cstart, 4 * (size_t)c->g.nkeys));
+        HIPCHK(hipMalloc(&c->cend, 4 * (size_t)c->g.nkeys));
+        c->cell_cap = c->g.nkeys;
     }
     return MLH_OK;
 }

@@ -51,16 +51,14 @@ __global__ void k_assign_cells(Geom g, const double *__restrict__ x0,
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double xs[3] = {x0[i], x1[i], D == 3 ? x2[i] : 0.0};
-    int c = 0, mul = 1;
+    int f[3] = {0, 0, 0};
     for (int a = 0; a < D; ++a) {
-        int f = (int)floor((xs[a] - g.bmin[a]) / g.cs[a]);
-        if (f == g.nc[a]) f -= 1;
-        if (f < 0) f = 0;
-        if (f > g.nc[a] - 1) f = g.nc[a] - 1;
-        c += f * mul;
-        mul *= g.nc[a];
+        f[a] = (int)floor((xs[a] - g.bmin[a]) / g.cs[a]);
+        if (f[a] == g.nc[a]) f[a] -= 1;
+        if (f[a] < 0) f[a] = 0;
+        if (f[a] > g.nc[a] - 1) f[a] = g.nc[a] - 1;
     }
-    key[i] = (uint32_t)c;
+    key[i] = cell_key<D>(g, f[0], f[1], f[2]);
     idx[i] = (uint32_t)i;
 }
 
@@ -269,9 +267,7 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
     const uint32_t io = (uint32_t)pi.w;
     uint32_t ck = a.ck_w[t];
     int cc[3];
-    cc[0] = (int)(ck % (uint32_t)g.nc[0]);
-    cc[1] = (int)((ck / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
-    cc[2] = D == 3 ? (int)(ck / (uint32_t)(g.nc[0] * g.nc[1])) : 0;
+    cell_coords<D>(g, ck, cc);
     uint32_t *row = a.nbr + t;
     int cnt = 0, err = 0;
     // real neighbours, reference stencil order
@@ -281,7 +277,7 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
             if (l < 0 || l >= g.nc[1]) continue;
             for (int m = (D == 3 ? cc[2] - 1 : 0); m <= (D == 3 ? cc[2] + 1 : 0) && !err; ++m) {
                 if (D == 3 && (m < 0 || m >= g.nc[2])) continue;
-                int c = k + l * g.nc[0] + (D == 3 ? m * g.nc[0] * g.nc[1] : 0);
+                uint32_t c = cell_key<D>(g, k, l, m);
                 int q1 = a.cend[c];
                 for (int q = a.cstart[c]; q < q1; ++q) {
                     double4 pj = a.pos_cell[q];
@@ -324,7 +320,7 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
                         wrapped |= s[ax] != 0;
                     }
                     if (!wrapped) continue;
-                    int c = r[0] + r[1] * g.nc[0] + (D == 3 ? r[2] * g.nc[0] * g.nc[1] : 0);
+                    uint32_t c = cell_key<D>(g, r[0], r[1], r[2]);
                     uint32_t code = (uint32_t)((s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1));
                     int q1 = a.cend[c];
                     for (int q = a.cstart[c]; q < q1; ++q) {
@@ -528,15 +524,18 @@ __global__ __launch_bounds__(256) void k_order_coherence(Geom g, int D,
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long hit = 0;
     if (i + 1 < N) {
-        uint32_t a = key[i], b = key[i + 1];
+        int ca[3], cb[3];
+        if (D == 2) {
+            cell_coords<2>(g, key[i], ca);
+            cell_coords<2>(g, key[i + 1], cb);
+        } else {
+            cell_coords<3>(g, key[i], ca);
+            cell_coords<3>(g, key[i + 1], cb);
+        }
         bool adj = true;
-        uint32_t div = 1;
         for (int ax = 0; ax < D; ++ax) {
-            int ca = (int)((a / div) % (uint32_t)g.nc[ax]);
-            int cb = (int)((b / div) % (uint32_t)g.nc[ax]);
-            int d = ca - cb;
+            int d = ca[ax] - cb[ax];
             adj &= d >= -1 && d <= 1;
-            div *= (uint32_t)g.nc[ax];
         }
         hit = adj ? 1ull : 0ull;
     }
@@ -558,10 +557,10 @@ void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
     // sort a copy: key[i] stays the cell of particle i
     hipMemcpyAsync(skey, key, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, s);
     int bits = 1;
-    while ((1 << bits) < g.ncells) ++bits;
+    while ((1 << bits) < g.nkeys) ++bits;
     uint32_t *k1 = scratch, *v1 = scratch + N, *rs = scratch + 2 * (size_t)N;
     radix_sort_pairs(skey, idx, k1, v1, N, bits, rs, s);
-    int nc = g.ncells;
+    int nc = g.nkeys;
     hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, s, cstart, 0, nc);
     hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, s, cend, 0, nc);
     hipLaunchKernelGGL(k_cell_ranges, dim3(nb), dim3(256), 0, s, skey, N, cstart, cend);

@@ -33,15 +33,77 @@ __host__ __device__ inline uint64_t ghost_key(uint32_t parent, uint32_t code) {
     return (uint64_t)parent * 8u + (uint64_t)(7 - code_mask(code));
 }
 
-// Search geometry (Domain::createGrid, Domain.cpp:9-54).
+// Search geometry (Domain::createGrid, Domain.cpp:9-54).  Cells are the
+// reference's; their keys (the sort key and the index of the cell-range
+// tables) follow a Morton curve when mbits > 0 (mbits bits per axis, nkeys =
+// 2^(D mbits) table entries), so that consecutive particles of the cell
+// order, and the blocks in flight on one XCD, are compact bricks of space.
+// mbits = 0: linear keys x + nx (y + ny z), nkeys = ncells.
 struct Geom {
     double bmin[3], bmax[3], cs[3];
     int nc[3];
     int ncells;
+    int mbits, nkeys;
     double h, hsq;        // kernelSize and kernelSize*kernelSize
     double sigma;         // cubic-spline normalisation for h/2
     int periodic;
 };
+
+__host__ __device__ inline uint32_t spread2(uint32_t x) {  // 16 bits -> even bits
+    x &= 0xffffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    return (x | (x << 1)) & 0x55555555u;
+}
+__host__ __device__ inline uint32_t compact2(uint32_t x) {
+    x &= 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0f0f0f0fu;
+    x = (x | (x >> 4)) & 0x00ff00ffu;
+    return (x | (x >> 8)) & 0xffffu;
+}
+__host__ __device__ inline uint32_t spread3(uint32_t x) {  // 10 bits -> every third bit
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    return (x | (x << 2)) & 0x09249249u;
+}
+__host__ __device__ inline uint32_t compact3(uint32_t x) {
+    x &= 0x09249249u;
+    x = (x | (x >> 2)) & 0x030c30c3u;
+    x = (x | (x >> 4)) & 0x0300f00fu;
+    x = (x | (x >> 8)) & 0x030000ffu;
+    return (x | (x >> 16)) & 0x3ffu;
+}
+// key of cell (cx, cy, cz)
+template <int D>
+__host__ __device__ inline uint32_t cell_key(const Geom &g, int cx, int cy, int cz) {
+    if (g.mbits) {
+        if (D == 2) return spread2((uint32_t)cx) | (spread2((uint32_t)cy) << 1);
+        return spread3((uint32_t)cx) | (spread3((uint32_t)cy) << 1) | (spread3((uint32_t)cz) << 2);
+    }
+    return (uint32_t)(cx + cy * g.nc[0] + (D == 3 ? cz * g.nc[0] * g.nc[1] : 0));
+}
+template <int D>
+__host__ __device__ inline void cell_coords(const Geom &g, uint32_t key, int *c) {
+    if (g.mbits) {
+        if (D == 2) {
+            c[0] = (int)compact2(key);
+            c[1] = (int)compact2(key >> 1);
+            c[2] = 0;
+        } else {
+            c[0] = (int)compact3(key);
+            c[1] = (int)compact3(key >> 1);
+            c[2] = (int)compact3(key >> 2);
+        }
+        return;
+    }
+    c[0] = (int)(key % (uint32_t)g.nc[0]);
+    c[1] = (int)((key / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
+    c[2] = D == 3 ? (int)(key / (uint32_t)(g.nc[0] * g.nc[1])) : 0;
+}
 
 // Periodic image coordinate (createGhostParticles, Particles.cpp:2124-2138):
 // s = +1 -> max + (x - min), s = -1 -> min - (max - x).
