@@ -70,10 +70,10 @@ typedef enum mlh_work_order {
     /* Order in which the kernels visit particles.  Results do not depend on
      * it (sums run in the reference's neighbour order, faces are solved by
      * the lower original index); only memory locality does. */
-    MLH_ORDER_AUTO = 0,    /* caller's order when consecutive particles are  *
-                            * mostly in adjacent search cells (lattices,     *
-                            * jittered lattices), else cell order            */
-    MLH_ORDER_CELL = 1,    /* particles sorted by search cell                */
+    MLH_ORDER_AUTO = 0,    /* cell order (the faster one on every input     *
+                            * measured)                                      */
+    MLH_ORDER_CELL = 1,    /* particles sorted by search cell, cells along  *
+                            * a Morton curve                                 */
     MLH_ORDER_CALLER = 2   /* the caller's particle order                    */
 } mlh_work_order;
 

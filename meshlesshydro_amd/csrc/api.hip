@@ -38,7 +38,6 @@ struct mlh_ctx {
     double *xn[3] = {}, *vn[3] = {}, *mn = nullptr, *un = nullptr;
     double4 *pos = nullptr;    // cell order
     double4 *pos_w = nullptr;  // caller order (work order = caller order)
-    unsigned long long *coh = nullptr;
     int order = 0;             // resolved work order (MLH_ORDER_CELL / _CALLER), 0 = not yet
     uint32_t *cell_key = nullptr, *sorted_idx = nullptr, *sorted_key = nullptr,
              *sort_scratch = nullptr;
@@ -256,7 +255,6 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     A_(&c->m, dsz); A_(&c->u, dsz); A_(&c->mn, dsz); A_(&c->un, dsz);
     A_(&c->pos, sizeof(double4) * N);
     A_(&c->pos_w, sizeof(double4) * N);
-    A_(&c->coh, sizeof(unsigned long long));
     A_(&c->cell_key, 4 * N); A_(&c->sorted_idx, 4 * N); A_(&c->sorted_key, 4 * N);
     A_(&c->sort_scratch, 4 * (2 * N + radix_sort_scratch_words(n)));
     if (p->periodic) {
@@ -327,7 +325,7 @@ int mlh_upload_state(mlh_ctx *c, const double *x, const double *y, const double 
     HIPCHK(hipMemcpyAsync(c->u, u, b, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     c->stage = 0;
-    if (c->p.work_order == MLH_ORDER_AUTO) c->order = 0;  // decided at the next search
+    if (c->p.work_order == MLH_ORDER_AUTO) c->order = 0;  // settled at the next search
     return MLH_OK;
 }
 
@@ -349,20 +347,11 @@ int mlh_download_state(mlh_ctx *c, double *x, double *y, double *z, double *vx, 
 }  // extern "C"
 
 // ---- stage launchers (asynchronous) ----------------------------------------
-// MLH_ORDER_AUTO: the caller's order when at least 3/4 of consecutive
-// particles sit in adjacent search cells (one device reduction and one
-// synchronisation per upload), else cell order
+// MLH_ORDER_AUTO resolves to cell order: with Morton-ordered cell keys it
+// is the faster order on every input measured (lattices in their natural
+// order included) and it does not depend on the caller's particle order
 static void resolve_order(mlh_ctx *c) {
-    if (c->p.work_order != MLH_ORDER_AUTO) {
-        c->order = c->p.work_order;
-        return;
-    }
-    unsigned long long n = 0;
-    hipMemsetAsync(c->coh, 0, sizeof(n), c->s);
-    launch_order_coherence(c->g, c->D, c->cell_key, c->N, c->coh, c->s);
-    hipMemcpyAsync(&n, c->coh, sizeof(n), hipMemcpyDeviceToHost, c->s);
-    hipStreamSynchronize(c->s);
-    c->order = (double)n >= 0.75 * (double)(c->N - 1) ? MLH_ORDER_CALLER : MLH_ORDER_CELL;
+    c->order = c->p.work_order == MLH_ORDER_AUTO ? MLH_ORDER_CELL : c->p.work_order;
 }
 
 // non-periodic: the domain limits of this step (one synchronisation), the

@@ -516,33 +516,6 @@ __global__ void k_pack_pos(const double *__restrict__ x0, const double *__restri
     pos[i] = make_double4(x0[i], x1[i], D == 3 ? x2[i] : 0.0, (double)i);
 }
 
-// number of i in [0, N-1) whose successor i+1 lies in the 3^D stencil of
-// i's search cell: the spatial coherence of the caller's order
-__global__ __launch_bounds__(256) void k_order_coherence(Geom g, int D,
-                                                         const uint32_t *__restrict__ key,
-                                                         int N, unsigned long long *count) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long hit = 0;
-    if (i + 1 < N) {
-        int ca[3], cb[3];
-        if (D == 2) {
-            cell_coords<2>(g, key[i], ca);
-            cell_coords<2>(g, key[i + 1], cb);
-        } else {
-            cell_coords<3>(g, key[i], ca);
-            cell_coords<3>(g, key[i + 1], cb);
-        }
-        bool adj = true;
-        for (int ax = 0; ax < D; ++ax) {
-            int d = ca[ax] - cb[ax];
-            adj &= d >= -1 && d <= 1;
-        }
-        hit = adj ? 1ull : 0ull;
-    }
-    for (int off = 32; off > 0; off >>= 1) hit += __shfl_xor(hit, off, 64);
-    if (lane_id() == 0 && hit) atomicAdd(count, hit);
-}
-
 template __global__ void k_neighbours<2>(NbrArgs);
 template __global__ void k_neighbours<3>(NbrArgs);
 
@@ -570,12 +543,6 @@ void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
 void launch_pack_pos(const double *x0, const double *x1, const double *x2, int D, int N,
                      double4 *pos, hipStream_t s) {
     hipLaunchKernelGGL(k_pack_pos, dim3((N + 255) / 256), dim3(256), 0, s, x0, x1, x2, D, N, pos);
-}
-
-void launch_order_coherence(const Geom &g, int D, const uint32_t *key, int N,
-                            unsigned long long *count, hipStream_t s) {
-    hipLaunchKernelGGL(k_order_coherence, dim3((N + 255) / 256), dim3(256), 0, s, g, D, key, N,
-                       count);
 }
 
 void launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {

@@ -71,8 +71,7 @@ void launch_domain_limits(const double *x0, const double *x1, const double *x2, 
                           double *scratch, double *out, hipStream_t s);
 void launch_pack_pos(const double *x0, const double *x1, const double *x2, int D, int N,
                      double4 *pos, hipStream_t s);
-void launch_order_coherence(const Geom &g, int D, const uint32_t *key, int N,
-                            unsigned long long *count, hipStream_t s);
+
 size_t radix_sort_scratch_words(int N);
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *tmp,
                         hipStream_t s);
