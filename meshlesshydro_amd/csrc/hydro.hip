@@ -459,36 +459,66 @@ __device__ inline void face_prepare(const HydroArgs &a, const View<D, T> &L,
                                     T dt, FacePrep<D, T> &fp, FaceStats &fs) {
     constexpr int NF = D + 2;
     const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
-    double g[3], gp[3];
-    for (int ax = 0; ax < D; ++ax) {
-        g[ax] = image_coord(H.x[ax], s[ax], a.g.bmin[ax], a.g.bmax[ax]);
-        gp[ax] = image_coord(L.x[ax], -s[ax], a.g.bmin[ax], a.g.bmax[ax]);
-    }
-    // effective face A = psi_H(x_L)/omega_L - psi_L(x_H)/omega_H
-    T r1 = tsqrt(T(dist2<D>(L.x, g)));
-    T r2 = tsqrt(T(dist2<D>(H.x, gp)));
-    T w1 = kernel_w<T>(r1, h2, sig) / PD_OM(L);
-    T w2 = kernel_w<T>(r2, h2, sig) / PD_OM(H);
     T *A = fp.A;
-    for (int p = 0; p < D; ++p) {
-        T p1 = T(0), p2 = T(0);
-        for (int q = 0; q < D; ++q) {
-            p1 += PD_B(L, p, q) * T(g[q] - L.x[q]) * w1;
-            p2 += PD_B(H, p, q) * T(gp[q] - H.x[q]) * w2;
-        }
-        A[p] = T(1) / PD_OM(L) * p1 - T(1) / PD_OM(H) * p2;
-    }
-    // states at the face, L perspective: WR = L, WL = H
     T xijxi[D], xijxj[D], xjxi[D];
-    for (int ax = 0; ax < D; ++ax) {
-        xjxi[ax] = T(g[ax] - L.x[ax]);
-        if (a.foqp) {
-            xijxj[ax] = T(.5 * (L.x[ax] - g[ax]));
-            xijxi[ax] = T(.5 * (g[ax] - L.x[ax]));
-        } else {
-            double xij = L.x[ax] + a.g.h / 4. * (g[ax] - L.x[ax]);
-            xijxi[ax] = T(xij - L.x[ax]);
-            xijxj[ax] = T(xij - g[ax]);
+    if constexpr (sizeof(T) == 4) {
+        // FP32: one FP64 difference d = x_H(image) - x_L, rounded once; the
+        // geometry of both sides follows from d in FP32 (x_L(image) - x_H =
+        // -d and one distance |d| for both kernel weights)
+        T d[D];
+        for (int ax = 0; ax < D; ++ax) {
+            double g = image ? image_coord(H.x[ax], s[ax], a.g.bmin[ax], a.g.bmax[ax]) : H.x[ax];
+            d[ax] = T(g - L.x[ax]);
+        }
+        T r = tsqrt(dotT<D, T>(d, d));
+        T wk = kernel_w<T>(r, h2, sig);
+        T iL = T(1) / PD_OM(L), iH = T(1) / PD_OM(H);
+        T w1 = wk * iL, w2 = wk * iH;
+        // effective face A = psi_H(x_L)/omega_L - psi_L(x_H)/omega_H
+        for (int p = 0; p < D; ++p) {
+            T p1 = T(0), p2 = T(0);
+            for (int q = 0; q < D; ++q) {
+                p1 += PD_B(L, p, q) * d[q] * w1;
+                p2 += PD_B(H, p, q) * (-d[q]) * w2;
+            }
+            A[p] = iL * p1 - iH * p2;
+        }
+        const T fl = a.foqp ? T(.5) : T(a.g.h / 4.);
+        for (int ax = 0; ax < D; ++ax) {
+            xjxi[ax] = d[ax];
+            xijxi[ax] = fl * d[ax];
+            xijxj[ax] = (fl - T(1)) * d[ax];
+        }
+    } else {
+        double g[3], gp[3];
+        for (int ax = 0; ax < D; ++ax) {
+            g[ax] = image_coord(H.x[ax], s[ax], a.g.bmin[ax], a.g.bmax[ax]);
+            gp[ax] = image_coord(L.x[ax], -s[ax], a.g.bmin[ax], a.g.bmax[ax]);
+        }
+        // effective face A = psi_H(x_L)/omega_L - psi_L(x_H)/omega_H
+        T r1 = tsqrt(T(dist2<D>(L.x, g)));
+        T r2 = tsqrt(T(dist2<D>(H.x, gp)));
+        T w1 = kernel_w<T>(r1, h2, sig) / PD_OM(L);
+        T w2 = kernel_w<T>(r2, h2, sig) / PD_OM(H);
+        for (int p = 0; p < D; ++p) {
+            T p1 = T(0), p2 = T(0);
+            for (int q = 0; q < D; ++q) {
+                p1 += PD_B(L, p, q) * T(g[q] - L.x[q]) * w1;
+                p2 += PD_B(H, p, q) * T(gp[q] - H.x[q]) * w2;
+            }
+            A[p] = T(1) / PD_OM(L) * p1 - T(1) / PD_OM(H) * p2;
+        }
+        // states at the face, L perspective: WR = L, WL = H
+        for (int ax = 0; ax < D; ++ax) {
+            xjxi[ax] = T(g[ax] - L.x[ax]);
+            if (a.foqp) {
+                xijxj[ax] = T(.5 * (L.x[ax] - g[ax]));
+                xijxi[ax] = T(.5 * (g[ax] - L.x[ax]));
+            } else {
+                double xij = L.x[ax] + a.g.h / 4. * (g[ax] - L.x[ax]);
+                xijxi[ax] = T(xij - L.x[ax]);
+                xijxj[ax] = T(xij - g[ax]);
+            }
         }
     }
     T *vF = fp.vF;

@@ -100,6 +100,41 @@ __device__ inline void prefun(const GasConst<T> &c, T p, T rk, T pk, T ck,
     }
 }
 
+// x^(1/5) in FP64 without a division: the FP32 estimate of r = x^(-1/5)
+// refined by two Newton steps r <- r (6 - x r^5) / 5 (error ~1e-7 ->
+// ~1e-14 -> round-off), then x^(1/5) = x r^4.
+__device__ inline double root5_nodiv(double x) {
+    if (!(x > 1e-30) || !(x < 1e30)) return pow(x, 0.2);
+    double r = (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)x));
+    for (int k = 0; k < 2; ++k) {
+        double r2 = r * r;
+        double r5 = r2 * r2 * r;
+        r = r * (1.2 - 0.2 * (x * r5));
+    }
+    double r2 = r * r;
+    return x * (r2 * r2);
+}
+
+// Residual of the exact solver for the FP32 mode's acceptance steps: f_K(p)
+// in FP64 (it cancels against the other side and du at the root), its
+// derivative in FP32 (it only scales the Newton correction; a 1e-7 relative
+// error of the correction is far below the change test).
+__device__ inline void prefun_res(const GasConst<double> &c, double p, double rk, double pk,
+                                  double ck, double &f, float &fd) {
+    if (p <= pk) {
+        double prat = p / pk;
+        double y = c.g53 ? root5_nodiv(prat) : pow(prat, c.g1);
+        f = c.g4 * ck * (y - 1.0);
+        fd = (1.f / ((float)rk * (float)ck)) * ((float)y / (float)prat);
+    } else {
+        double ak = c.g5 / rk;
+        double bk = c.g6 * pk;
+        double qrt = sqrt(ak / (bk + p));
+        f = (p - pk) * qrt;
+        fd = (1.f - 0.5f * (float)(p - pk) / (float)(bk + p)) * (float)qrt;
+    }
+}
+
 // Adaptive initial guess (Toro 9.5.1): PVRS, else TRRS, else TSRS.
 template <typename T>
 __device__ inline T guess_p(const GasConst<T> &c, T dl, T ul, T pl, T cl, T dr,
@@ -343,15 +378,16 @@ __device__ inline MixedProbe mixed_probe(const GasConst<float> &cf, const GasCon
     float p = p_start > 0.f ? p_start : guess_p(cf, dl, ul, pl, m.cl, dr, ur, pr, m.cr, tiny);
     const double dld = dl, pld = pl, drd = dr, prd = pr, dud = (double)ur - (double)ul;
     const double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
-    double fl, fld, fr, frd, pd = p;
-    prefun(cd, pd, dld, pld, cld, fl, fld);
-    prefun(cd, pd, drd, prd, crd, fr, frd);
-    double dp = (fl + fr + dud) / (fld + frd);
+    double fl, fr, pd = p;
+    float fld, frd;
+    prefun_res(cd, pd, dld, pld, cld, fl, fld);
+    prefun_res(cd, pd, drd, prd, crd, fr, frd);
+    double dp = (fl + fr + dud) * (double)(1.f / (fld + frd));
     double pn = pd - dp;
     st.iters += 1;
-    m.accepted = 2.0 * fabs(dp / (pn + pd)) <= tol && pn > 0.0;
+    m.accepted = 2.f * __builtin_fabsf((float)dp / (float)(pn + pd)) <= (float)tol && pn > 0.0;
     m.pn = pn;
-    m.um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (frd - fld) * dp);
+    m.um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (double)(frd - fld) * dp);
     m.p = pn > 0.0 ? (float)pn : p;
     return m;
 }
@@ -379,15 +415,16 @@ __device__ inline RSol<float> mixed_finish(const GasConst<float> &cf, const GasC
         st.iters += it;
         const double dld = dl, pld = pl, drd = dr, prd = pr, dud = (double)ur - (double)ul;
         const double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
-        double fl, fld, fr, frd, pd = p;
-        prefun(cd, pd, dld, pld, cld, fl, fld);
-        prefun(cd, pd, drd, prd, crd, fr, frd);
-        double dp = (fl + fr + dud) / (fld + frd);
+        double fl, fr, pd = p;
+        float fld, frd;
+        prefun_res(cd, pd, dld, pld, cld, fl, fld);
+        prefun_res(cd, pd, drd, prd, crd, fr, frd);
+        double dp = (fl + fr + dud) * (double)(1.f / (fld + frd));
         pn = pd - dp;
         st.iters += 1;
-        ok = 2.0 * fabs(dp / (pn + pd)) <= tol && pn > 0.0;
+        ok = 2.f * __builtin_fabsf((float)dp / (float)(pn + pd)) <= (float)tol && pn > 0.0;
         if (!ok) st.fallback += 1;
-        um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (frd - fld) * dp);
+        um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (double)(frd - fld) * dp);
     }
     if (ok) return sample(cf, float(pn), float(um), dl, ul, pl, m.cl, dr, ur, pr, m.cr);
     RSol<double> r = solve_fp64(cd, dl, ul, pl, dr, ur, pr, tol, max_iter, -1.0, st);
