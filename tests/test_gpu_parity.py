@@ -136,13 +136,20 @@ def test_stages_fp32(name, hip, oracle_mod):
     # scaling from 1 to alpha): where FP32 and FP64 land on the two sides of
     # that switch a particle's gradient differs by O(1).  That happens at the
     # limited edges of open domains; elsewhere the difference is round-off.
-    fields = [orr] + [orc.state()[k] for k in ("vx", "vy", "vz")[: p.dim]] + [oP]
+    # Open domains: the edge particles' one-sided neighbourhoods are limited
+    # hard and flip between the two sides of that switch (and so do their
+    # neighbours' fluxes); the strict bars apply to particles more than 3h
+    # inside the particles' extent, the edges only have to stay bounded.
+    st0 = orc.state()
+    inner = np.ones(len(orr), bool)
+    if not p.periodic:
+        for k in ("x", "y", "z")[: p.dim]:
+            inner &= (st0[k] > st0[k].min() + 3 * p.kernel_size) & (st0[k] < st0[k].max() - 3 * p.kernel_size)
+    fields = [orr] + [st0[k] for k in ("vx", "vy", "vz")[: p.dim]] + [oP]
     for f in range(p.dim + 2):
         scale = max(np.max(np.abs(fields[f])), 1e-300) / p.kernel_size
         err = np.max(np.abs(gG[f] - oG[f]), axis=1) / scale
-        # open domains: many limited particles along the edges
-        flips = np.mean(err > 1e-5)
-        assert flips < (0.005 if p.periodic else 0.05), (name, f, flips)
+        assert np.mean(err[inner] > 1e-5) < 0.005, (name, f, np.mean(err[inner] > 1e-5))
         assert np.max(err) < 2.0, (name, f, np.max(err))
     sim.fluxes(dt); orc.fluxes(dt)
     gm, gv, ge = sim.get_fluxes()
@@ -156,10 +163,7 @@ def test_stages_fp32(name, hip, oracle_mod):
     for a, b, scale in ((gm, om, fm), (gv, ov, fm * (vmax + cmax) + np.max(oP) * p.kernel_size ** (p.dim - 1)),
                         (ge, oe, fm * (vmax + cmax) ** 2 + np.max(oP) * (vmax + cmax) * p.kernel_size ** (p.dim - 1))):
         err = np.abs(a - b).reshape(len(gm), -1).max(axis=1) / scale
-        # the limiter switch above moves the fluxes of those particles and
-        # of their neighbours
-        flips = np.mean(err > 1e-5)
-        assert flips < (0.01 if p.periodic else 0.2), (name, flips)
+        assert np.mean(err[inner] > 1e-5) < 0.01, (name, np.mean(err[inner] > 1e-5))
         assert np.max(err) < 1.0, (name, np.max(err))
     sim.close()
 
