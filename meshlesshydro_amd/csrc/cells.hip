@@ -13,9 +13,9 @@
 //   3. per-cell [start, end) ranges of the sorted order (ascending particle
 //      index inside a cell, as the reference's push_back lists);
 //   4. positions gathered into cell order as double4 (x, y, z, original
-//      index): every later kernel of the step works in this order, so that a
-//      wavefront's particles and their neighbours are a few adjacent cells;
-//   5. one thread per particle, in cell order: the 3^D stencil of
+//      index); cell keys follow a Morton curve (device_common.h), so that a
+//      block of the cell order is a compact brick of cells;
+//   5. one thread per particle, in work order: the 3^D stencil of
 //      non-periodic cells (x outer ... z inner, Domain.cpp:83-118) with the
 //      strict test d^2 < h^2 in the reference's operation order, then the
 //      periodic images: every candidate in the window of virtual cells
@@ -25,11 +25,12 @@
 //      subset).  A periodic image within h of x_i is less than one cell size
 //      away per axis, so the window is complete: the result equals the
 //      reference's exhaustive ghost loop.
-// Neighbour entries hold cell-order positions; rows are stored transposed
-// (entry n of the particle at cell-order position t at nbr[n*N + t]) so that
-// the writes here and the reads of the per-particle kernels are coalesced.
-// Inside one cell the cell order is ascending original index (stable sort of
-// the identity), so the reference's neighbour order is kept.
+// Neighbour entries hold work positions (cell-order positions, or original
+// indices when the work order is the caller's); rows are stored transposed
+// (entry n of work position t at nbr[n*N + t]) so that the writes here and
+// the reads of the per-particle kernels are coalesced.  Inside one cell the
+// cell order is ascending original index (stable sort of the identity), so
+// the reference's neighbour order is kept.
 #include <hip/hip_runtime.h>
 #include "device_common.h"
 #include "kernels.h"
@@ -252,6 +253,16 @@ __device__ inline double dist2(const double *a, const double *b) {
     return d;
 }
 
+// distance from x to the slab of cell k along one axis (0 inside); the first
+// and last cells are open outward
+__device__ inline double box_gap(double x, int k, double bmin, double cs, int nc) {
+    double lo = bmin + (double)k * cs, hi = bmin + (double)(k + 1) * cs;
+    double gap = 0.0;
+    if (k > 0 && x < lo) gap = lo - x;
+    if (k < nc - 1 && x > hi) gap = x - hi;
+    return gap;
+}
+
 // One thread per work position t.  Candidates are scanned in cell order
 // (pos_cell); an entry stores the candidate's work position: its cell-order
 // position, or its original index when the work order is the caller's.
@@ -270,25 +281,43 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
     cell_coords<D>(g, ck, cc);
     uint32_t *row = a.nbr + t;
     int cnt = 0, err = 0;
-    // real neighbours, reference stencil order
+    // real neighbours, reference stencil order.  A stencil cell whose box is
+    // farther than h from x_i (with a relative margin far above the
+    // rounding of positions and cell indices; edge cells are open outward,
+    // where clamped particles sit) holds no neighbour and is skipped.  The
+    // candidates of a cell are read four at a time, so that their loads are
+    // in flight together.
+    const double hlim = g.hsq * (1.0 + 1e-9);
     for (int k = cc[0] - 1; k <= cc[0] + 1 && !err; ++k) {
         if (k < 0 || k >= g.nc[0]) continue;
+        double bx = box_gap(xi[0], k, g.bmin[0], g.cs[0], g.nc[0]);
         for (int l = cc[1] - 1; l <= cc[1] + 1 && !err; ++l) {
             if (l < 0 || l >= g.nc[1]) continue;
+            double by = box_gap(xi[1], l, g.bmin[1], g.cs[1], g.nc[1]);
             for (int m = (D == 3 ? cc[2] - 1 : 0); m <= (D == 3 ? cc[2] + 1 : 0) && !err; ++m) {
                 if (D == 3 && (m < 0 || m >= g.nc[2])) continue;
+                double bz = D == 3 ? box_gap(xi[2], m, g.bmin[2], g.cs[2], g.nc[2]) : 0.0;
+                if (bx * bx + by * by + bz * bz > hlim) continue;
                 uint32_t c = cell_key<D>(g, k, l, m);
-                int q1 = a.cend[c];
-                for (int q = a.cstart[c]; q < q1; ++q) {
-                    double4 pj = a.pos_cell[q];
-                    const uint32_t jo = (uint32_t)pj.w;
-                    if (jo == io) continue;
-                    double xj[3] = {pj.x, pj.y, pj.z};
-                    double d = dist2<D>(xj, xi);
-                    if (d < g.hsq) {
-                        if (cnt >= a.maxr) { err = 1; break; }
-                        row[(size_t)cnt * N] = (co ? jo : (uint32_t)q) | (kCodeReal << kIdxBits);
-                        ++cnt;
+                const int q1 = a.cend[c];
+                for (int q = a.cstart[c]; q < q1 && !err; q += 4) {
+                    double4 pj[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        pj[u] = q + u < q1 ? a.pos_cell[q + u] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (q + u >= q1 || err) continue;
+                        const uint32_t jo = (uint32_t)pj[u].w;
+                        if (jo == io) continue;
+                        double xj[3] = {pj[u].x, pj[u].y, pj[u].z};
+                        double d = dist2<D>(xj, xi);
+                        if (d < g.hsq) {
+                            if (cnt >= a.maxr) { err = 1; continue; }
+                            row[(size_t)cnt * N] =
+                                (co ? jo : (uint32_t)(q + u)) | (kCodeReal << kIdxBits);
+                            ++cnt;
+                        }
                     }
                 }
             }
