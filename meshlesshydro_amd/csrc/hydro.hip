@@ -773,8 +773,13 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     const double dtd = a.dt[0];
     const T dt = T(dtd);
     int cnt = live ? a.noi[t] + a.nog[t] : 0;
+    constexpr bool kFp32 = sizeof(T) == 4;
+    // FP32 learned: faces whose probe fails are deferred to a second loop
+    // that seeds them with the network; the FP64 learned kernel runs the
+    // network inside the face loop, over the wavefront's longest row
+    constexpr bool kInLoopNet = LEARNED && !kFp32;
     int lim = cnt;
-    if (LEARNED) {
+    if (kInLoopNet) {
         for (int off = 32; off > 0; off >>= 1) {
             int o = __shfl_xor(lim, off, 64);
             lim = o > lim ? o : lim;
@@ -782,45 +787,74 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     }
     const bool collect = a.collect && live && (i % a.collect_stride) == 0;
     double mF = 0., eF = 0., vFs[3] = {0., 0., 0.};
-    constexpr bool kFp32 = sizeof(T) == 4;
+    uint64_t defer[2] = {0ull, 0ull};
+    // the face of neighbour entry n, from the side that solves it
+    auto prepare = [&](int n, FacePrep<D, T> &fp, MixedProbe &mp, bool &self_lo) {
+        uint32_t e = a.nbr[(size_t)n * N + t];
+        int j = (int)nb_index(e);
+        uint32_t code = nb_code(e);
+        bool image = code != kCodeReal;
+        int s[3] = {0, 0, 0};
+        if (image)
+            for (int ax = 0; ax < D; ++ax) s[ax] = code_shift(code, ax);
+        // side that solves the pair: the lower original index (images:
+        // parent >= i)
+        const int jo = (int)a.pos[j].w;
+        self_lo = !a.enforce_flux_sym || (image ? jo >= i : jo > i);
+        const View<D, T> L = make_view<D, T>(a, self_lo ? t : j);
+        const View<D, T> H = make_view<D, T>(a, self_lo ? j : t);
+        int sl[3];
+        for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
+        face_prepare<D, T>(a, L, H, sl, image, dt, fp, fs);
+        if constexpr (kFp32) mp = face_probe<D>(a, fp, fs.st);
+    };
+    // network seed for this lane's face when want (all lanes call together)
+    auto net_seed = [&](bool want, const FacePrep<D, T> &fp) -> float {
+        float f[kMlpIn];
+        float pref = 1.f;
+        bool ok = want && fp.WR[0] > T(0) && fp.WL[0] > T(0) && fp.WR[1] > T(0) && fp.WL[1] > T(0);
+        if (ok)
+            pref = mlp_features_fast((float)a.gamma, (float)fp.WR[0], (float)fp.ui, (float)fp.WR[1],
+                                     (float)fp.WL[0], (float)fp.uj, (float)fp.WL[1], f);
+        else
+            for (int q = 0; q < kMlpIn; ++q) f[q] = 0.f;
+        float y = mlp_infer_wave_bf16_lean(a.mlp_frags, f, lds[threadIdx.x >> 6]);
+        fs.seeded += ok ? 1 : 0;
+        return ok ? mlp_seed_from_output(y, f, pref) : -1.f;
+    };
+    auto accumulate = [&](const FacePrep<D, T> &fp, const MixedProbe &mp, float seed, bool self_lo,
+                          int it0) {
+        T F[NF];
+        face_finish<D, T>(a, fp, seed, F, fs, kFp32 ? &mp : nullptr);
+        // training data: faces the exact solver needed >= 2 iterations for
+        if (collect && self_lo && fs.st.iters - it0 >= 2) {
+            unsigned long long slot = atomicAdd(a.collect_n, 1ull);
+            if (slot < (unsigned long long)a.collect_cap) {
+                double *c = a.collect + slot * 6;
+                c[0] = fp.WR[0]; c[1] = fp.ui; c[2] = fp.WR[1];
+                c[3] = fp.WL[0]; c[4] = fp.uj; c[5] = fp.WL[1];
+            }
+        }
+        double sgn = self_lo ? 1. : -1.;
+        mF += sgn * (double)F[0];
+        for (int ax = 0; ax < D; ++ax) vFs[ax] += sgn * (double)F[2 + ax];
+        eF += sgn * (double)F[1];
+    };
     for (int n = 0; n < lim; ++n) {
         const bool act = n < cnt;
         FacePrep<D, T> fp;
         MixedProbe mp{};
         bool self_lo = true;
         const int it0 = fs.st.iters;
-        if (act) {
-            uint32_t e = a.nbr[(size_t)n * N + t];
-            int j = (int)nb_index(e);
-            uint32_t code = nb_code(e);
-            bool image = code != kCodeReal;
-            int s[3] = {0, 0, 0};
-            if (image)
-                for (int ax = 0; ax < D; ++ax) s[ax] = code_shift(code, ax);
-            // side that solves the pair: the lower original index (images:
-            // parent >= i)
-            const int jo = (int)a.pos[j].w;
-            self_lo = !a.enforce_flux_sym || (image ? jo >= i : jo > i);
-            const View<D, T> L = make_view<D, T>(a, self_lo ? t : j);
-            const View<D, T> H = make_view<D, T>(a, self_lo ? j : t);
-            int sl[3];
-            for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
-            face_prepare<D, T>(a, L, H, sl, image, dt, fp, fs);
-            if constexpr (kFp32) mp = face_probe<D>(a, fp, fs.st);
-        }
+        if (act) prepare(n, fp, mp, self_lo);
         float seed = -1.f;
-        if (LEARNED) {
-            // FP32: the network runs for the faces whose FP64 step at the
-            // standard guess failed the change test (Newton has to iterate),
-            // on wavefronts holding at least one.  FP64: for faces with a
-            // real jump (near-uniform faces keep the standard guess, which is
-            // converged to O(jump^2) there).
+        if constexpr (kInLoopNet) {
+            // FP64: the network runs for faces with a real jump
+            // (near-uniform faces keep the standard guess, which is converged
+            // to O(jump^2) there): |dp| > k (pL + pR) or
+            // du^2 rhoR rhoL > 2 k^2 gamma (pR rhoL + pL rhoR)
             bool strong = false;
-            if (kFp32) {
-                strong = act && mp.valid && !mp.accepted;
-            } else if (act) {
-                // |dp| > k (pL + pR)  or  du^2 > 2 k^2 gamma (pR/rhoR + pL/rhoL),
-                // the latter multiplied through by rhoR rhoL (no divisions)
+            if (act) {
                 T dp = fp.WR[1] - fp.WL[1], du = fp.uj - fp.ui;
                 T psum = fp.WR[1] + fp.WL[1];
                 T rr = fp.WR[0] * fp.WL[0];
@@ -828,38 +862,40 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
                 strong = dp * dp > T(kSeedJump * kSeedJump) * psum * psum ||
                          du * du * rr > T(2 * kSeedJump * kSeedJump) * c2r;
             }
-            if (__ballot(strong)) {
-                float f[kMlpIn];
-                float pref = 1.f;
-                bool ok = strong && fp.WR[0] > T(0) && fp.WL[0] > T(0) && fp.WR[1] > T(0) &&
-                          fp.WL[1] > T(0);
-                if (ok)
-                    pref = mlp_features_fast((float)a.gamma, (float)fp.WR[0], (float)fp.ui,
-                                             (float)fp.WR[1], (float)fp.WL[0], (float)fp.uj,
-                                             (float)fp.WL[1], f);
-                else
-                    for (int q = 0; q < kMlpIn; ++q) f[q] = 0.f;
-                float y = mlp_infer_wave_bf16_lean(a.mlp_frags, f, lds[threadIdx.x >> 6]);
-                seed = ok ? mlp_seed_from_output(y, f, pref) : -1.f;
-                fs.seeded += ok ? 1 : 0;
-            }
+            if (__ballot(strong)) seed = net_seed(strong, fp);
         }
         if (act) {
-            T F[NF];
-            face_finish<D, T>(a, fp, seed, F, fs, kFp32 ? &mp : nullptr);
-            // training data: faces the exact solver needed >= 2 iterations for
-            if (collect && self_lo && fs.st.iters - it0 >= 2) {
-                unsigned long long slot = atomicAdd(a.collect_n, 1ull);
-                if (slot < (unsigned long long)a.collect_cap) {
-                    double *c = a.collect + slot * 6;
-                    c[0] = fp.WR[0]; c[1] = fp.ui; c[2] = fp.WR[1];
-                    c[3] = fp.WL[0]; c[4] = fp.uj; c[5] = fp.WL[1];
-                }
+            if (LEARNED && kFp32 && mp.valid && !mp.accepted && n < 128) {
+                defer[n >> 6] |= 1ull << (n & 63);  // Newton has to iterate
+                continue;
             }
-            double sgn = self_lo ? 1. : -1.;
-            mF += sgn * (double)F[0];
-            for (int ax = 0; ax < D; ++ax) vFs[ax] += sgn * (double)F[2 + ax];
-            eF += sgn * (double)F[1];
+            accumulate(fp, mp, seed, self_lo, it0);
+        }
+    }
+    if constexpr (LEARNED && kFp32) {
+        // the deferred faces: recomputed, seeded by the network (one batch of
+        // the wavefront's lanes per round), solved from the seed
+        while (__ballot(defer[0] != 0ull || defer[1] != 0ull)) {
+            int n = -1;
+            if (defer[0]) {
+                n = __builtin_ctzll(defer[0]);
+                defer[0] &= defer[0] - 1ull;
+            } else if (defer[1]) {
+                n = 64 + __builtin_ctzll(defer[1]);
+                defer[1] &= defer[1] - 1ull;
+            }
+            const bool act = n >= 0;
+            FacePrep<D, T> fp;
+            MixedProbe mp{};
+            bool self_lo = true;
+            if (act) {  // counted by the face loop already
+                const FaceStats keep = fs;
+                prepare(n, fp, mp, self_lo);
+                fs = keep;
+            }
+            const int it0 = fs.st.iters;
+            float seed = net_seed(act && mp.valid && !mp.accepted, fp);
+            if (act) accumulate(fp, mp, seed, self_lo, it0);
         }
     }
     if (live) {
