@@ -218,8 +218,10 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
             T r = tsqrt(T(dist2<D>(xi, xj)));
             T wt = kernel_w<T>(r, h2, sig) / om_i;
             double w = (double)wt;
+            // E is symmetric and a*b*w == b*a*w in IEEE arithmetic: the
+            // upper triangle, mirrored after the loop
             for (int p = 0; p < D; ++p)
-                for (int q = 0; q < D; ++q) E[D * p + q] += (xj[p] - xi[p]) * (xj[q] - xi[q]) * w;
+                for (int q = p; q < D; ++q) E[D * p + q] += (xj[p] - xi[p]) * (xj[q] - xi[q]) * w;
             T prj[kPrimN];
             if (kFused) {
                 load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
@@ -247,6 +249,8 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                 vsig = vs > vsig ? vs : vsig;
             }
         }
+        for (int p = 1; p < D; ++p)
+            for (int q = 0; q < p; ++q) E[D * p + q] = E[D * q + p];
         invert<D>(E);
         T B[D * D];
         for (int k = 0; k < D * D; ++k) B[k] = T(E[k]);
