@@ -146,6 +146,7 @@ def main():
     log(f"[bench] FP64 plain: {W} warmup + {K} timed steps, N={N}")
     if not args.no_learned:
         sim.set_face_collection(1 << 21, 8)
+    c64_start = sim.conserved()
     dts_w = sim.run(W)
     faces = sim.collected_faces() if not args.no_learned else None
     sim.set_face_collection(0, 0)
@@ -154,6 +155,7 @@ def main():
     t64 = time.perf_counter() - t0
     tim64 = sim.timings()
     d64 = sim.diag()
+    cons64 = (c64_start, sim.conserved())
     ref = sim.state()
     dts = np.concatenate([dts_w, dts_k])
     rate64 = N * K / t64
@@ -173,11 +175,12 @@ def main():
         s = lib.Simulation(p, state)
         if seed_params is not None:
             s.mlp_set_params(seed_params)
+        c_start = s.conserved()
         s.run(W, dts_in=dts[:W])
         t0 = time.perf_counter()
         s.run(K, dts_in=dts[W:])
         t = time.perf_counter() - t0
-        out = (N * K / t, s.timings(), s.diag(), s.state(), s.conserved())
+        out = (N * K / t, s.timings(), s.diag(), s.state(), (c_start, s.conserved()))
         s.close()
         return out
 
@@ -190,27 +193,41 @@ def main():
 
     # density error: rho = m * omega, omega from the FP64 neighbour sets of
     # each final state (one FP64 density pass)
-    def final_rho(st):
+    def final_fields(st):
+        """rho, P (one FP64 density pass over the state) and |v|"""
         p = default_params(precision=MLH_FP64, **base)
         s = lib.Simulation(p, st)
         s.neighbours()
         s.density()
-        r = s.get_density()[0]
+        r, P, _ = s.get_density()
         s.close()
-        return r
+        v = np.sqrt(st["vx"] ** 2 + st["vy"] ** 2 + st["vz"] ** 2)
+        return {"rho": r, "P": P, "v": v}
 
-    rho64 = final_rho(ref)
+    ref_f = final_fields(ref)
 
-    def rho_err(st):
-        r = final_rho(st)
-        return float(np.sum(np.abs(r - rho64)) / np.sum(np.abs(rho64))), float(np.max(np.abs(r - rho64)) / np.max(np.abs(rho64)))
+    def field_err(st):
+        """relative L1 and Linf errors of rho, P, |v| against the FP64 run"""
+        f = final_fields(st)
+        out = {}
+        for k, b in ref_f.items():
+            d = np.abs(f[k] - b)
+            out["L1_rel_" + k] = float(np.sum(d) / np.sum(np.abs(b)))
+            out["Linf_rel_" + k] = float(np.max(d) / np.max(np.abs(b)))
+        return out
 
-    l1_32, linf_32 = rho_err(st32)
-    head = dict(rate=rate32, tim=tim32, diag=d32, l1=l1_32, linf=linf_32, seed="standard")
+    def drift(cons):
+        """relative change of total mass and energy, momentum change over
+        the total |momentum| scale sum m |v| of the end state"""
+        (c0, c1) = cons
+        return {"mass": abs(c1[1] - c0[1]) / abs(c0[1]), "energy": abs(c1[2] - c0[2]) / abs(c0[2]),
+                "momentum": float(np.sqrt(sum((c1[k] - c0[k]) ** 2 for k in (3, 4, 5))))}
+
+    e32 = field_err(st32)
+    head = dict(rate=rate32, tim=tim32, diag=d32, err=e32, drift=drift(cons32), seed="standard")
     if learned is not None:
-        l1_l, linf_l = rho_err(learned[3])
-        head = dict(rate=learned[0], tim=learned[1], diag=learned[2], l1=l1_l, linf=linf_l,
-                    seed="learned", train=learned[5])
+        head = dict(rate=learned[0], tim=learned[1], diag=learned[2], err=field_err(learned[3]),
+                    drift=drift(learned[4]), seed="learned", train=learned[5])
 
     face_kernel = "k_faces<3, float, %s>" % ("true" if head["seed"] == "learned" else "false")
     k_mean = (d64["faces_solved"] / max(1, (W + K) * N))
@@ -252,13 +269,16 @@ def main():
                    "neighbours_mean": round(k_mean, 2), "seed": head["seed"],
                    "parallelism": "single GPU"},
         "fp64_plain": {"value": rate64, "ms_per_step": 1e3 * t64 / K,
-                       "kernel_ms": [round(float(x), 4) for x in tim64]},
+                       "kernel_ms": [round(float(x), 4) for x in tim64],
+                       "drift": drift(cons64)},
         "speedup_vs_fp64": head["rate"] / rate64,
-        "fp32_standard": {"value": rate32, "L1_rel_rho": l1_32, "Linf_rel_rho": linf_32,
-                          "kernel_ms": [round(float(x), 4) for x in tim32],
-                          "newton_iters_per_face": d32["newton_iterations"] / max(1, d32["faces_solved"])},
-        "L1_rel_rho": head["l1"],
-        "Linf_rel_rho": head["linf"],
+        "fp32_standard": dict(value=rate32, kernel_ms=[round(float(x), 4) for x in tim32],
+                              newton_iters_per_face=d32["newton_iterations"] / max(1, d32["faces_solved"]),
+                              drift=drift(cons32), **e32),
+        "L1_rel_rho": head["err"]["L1_rel_rho"],
+        "Linf_rel_rho": head["err"]["Linf_rel_rho"],
+        "errors_vs_fp64": head["err"],
+        "conservation_drift": head["drift"],
         "kernel_ms": [round(float(x), 4) for x in head["tim"]],
         "newton_iters_per_face": head["diag"]["newton_iterations"] / max(1, head["diag"]["faces_solved"]),
         "fallback_rate": head["diag"]["learned_fallbacks"] / max(1, head["diag"]["faces_solved"]),
