@@ -754,6 +754,117 @@ __global__ __launch_bounds__(256) void k_riemann_flux(HydroArgs a, long long n,
 }
 
 
+// The face of neighbour entry n of work position t (original index i), seen
+// from the side that solves it: the lower original index (images: parent
+// >= i).  FP32: also the first step of the mixed solver.
+template <int D, typename T>
+__device__ inline void face_of_entry(const HydroArgs &a, int t, int i, int n, T dt,
+                                     FacePrep<D, T> &fp, MixedProbe &mp, bool &self_lo,
+                                     FaceStats &fs) {
+    uint32_t e = a.nbr[(size_t)n * a.N + t];
+    int j = (int)nb_index(e);
+    uint32_t code = nb_code(e);
+    bool image = code != kCodeReal;
+    int s[3] = {0, 0, 0};
+    if (image)
+        for (int ax = 0; ax < D; ++ax) s[ax] = code_shift(code, ax);
+    const int jo = (int)a.pos[j].w;
+    self_lo = !a.enforce_flux_sym || (image ? jo >= i : jo > i);
+    const View<D, T> L = make_view<D, T>(a, self_lo ? t : j);
+    const View<D, T> H = make_view<D, T>(a, self_lo ? j : t);
+    int sl[3];
+    for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
+    face_prepare<D, T>(a, L, H, sl, image, dt, fp, fs);
+    if constexpr (sizeof(T) == 4) mp = face_probe<D>(a, fp, fs.st);
+}
+
+// Network seed for this lane's face when want; every lane of the wavefront
+// calls it together (BF16 MFMA over the wavefront's 64 faces).
+template <int D, typename T>
+__device__ inline float face_net_seed(const HydroArgs &a, bool want, const FacePrep<D, T> &fp,
+                                      uint16_t *lds, FaceStats &fs) {
+    float f[kMlpIn];
+    float pref = 1.f;
+    bool ok = want && fp.WR[0] > T(0) && fp.WL[0] > T(0) && fp.WR[1] > T(0) && fp.WL[1] > T(0);
+    if (ok)
+        pref = mlp_features_fast((float)a.gamma, (float)fp.WR[0], (float)fp.ui, (float)fp.WR[1],
+                                 (float)fp.WL[0], (float)fp.uj, (float)fp.WL[1], f);
+    else
+        for (int q = 0; q < kMlpIn; ++q) f[q] = 0.f;
+    float y = mlp_infer_wave_bf16_lean(a.mlp_frags, f, lds);
+    fs.seeded += ok ? 1 : 0;
+    return ok ? mlp_seed_from_output(y, f, pref) : -1.f;
+}
+
+struct FluxSums {
+    double m, e, v[3];
+};
+
+// solve the face, add its flux (with the sign of this side) to the sums;
+// face collection for the training data
+template <int D, typename T>
+__device__ inline void face_accumulate(const HydroArgs &a, const FacePrep<D, T> &fp,
+                                       const MixedProbe &mp, float seed, bool self_lo, int it0,
+                                       bool collect, FluxSums &acc, FaceStats &fs) {
+    T F[D + 2];
+    face_finish<D, T>(a, fp, seed, F, fs, sizeof(T) == 4 ? &mp : nullptr);
+    // training data: faces the exact solver needed >= 2 iterations for
+    if (collect && self_lo && fs.st.iters - it0 >= 2) {
+        unsigned long long slot = atomicAdd(a.collect_n, 1ull);
+        if (slot < (unsigned long long)a.collect_cap) {
+            double *c = a.collect + slot * 6;
+            c[0] = fp.WR[0]; c[1] = fp.ui; c[2] = fp.WR[1];
+            c[3] = fp.WL[0]; c[4] = fp.uj; c[5] = fp.WL[1];
+        }
+    }
+    double sgn = self_lo ? 1. : -1.;
+    acc.m += sgn * (double)F[0];
+    for (int ax = 0; ax < D; ++ax) acc.v[ax] += sgn * (double)F[2 + ax];
+    acc.e += sgn * (double)F[1];
+}
+
+// FP32 learned: the faces whose probe failed (bits of d0 / d1 = neighbour
+// entries 0..127), recomputed, seeded by the network one batch of the
+// wavefront's lanes per round, and solved from the seed.  A function of its
+// own, called only by wavefronts that hold such faces, so that the face loop
+// keeps the code and registers of the standard kernel (arguments and results
+// by value: no address of the loop's state escapes, the copies are made at
+// the call only).
+struct DeferredOut {
+    FluxSums acc;
+    FaceStats fs;
+};
+
+template <int D>
+__device__ __attribute__((noinline)) DeferredOut faces_deferred(const HydroArgs a, int t, int i,
+                                                                float dt, uint64_t d0, uint64_t d1,
+                                                                uint16_t *lds, FluxSums acc,
+                                                                FaceStats fs) {
+    while (__ballot(d0 != 0ull || d1 != 0ull)) {
+        int n = -1;
+        if (d0) {
+            n = __builtin_ctzll(d0);
+            d0 &= d0 - 1ull;
+        } else if (d1) {
+            n = 64 + __builtin_ctzll(d1);
+            d1 &= d1 - 1ull;
+        }
+        const bool act = n >= 0;
+        FacePrep<D, float> fp;
+        MixedProbe mp{};
+        bool self_lo = true;
+        if (act) {  // counted by the face loop already
+            const FaceStats keep = fs;
+            face_of_entry<D, float>(a, t, i, n, dt, fp, mp, self_lo, fs);
+            fs = keep;
+        }
+        const int it0 = fs.st.iters;
+        float seed = face_net_seed<D, float>(a, act && mp.valid && !mp.accepted, fp, lds, fs);
+        if (act) face_accumulate<D, float>(a, fp, mp, seed, self_lo, it0, false, acc, fs);
+    }
+    return DeferredOut{acc, fs};
+}
+
 // Waves per SIMD the face kernel is compiled for.  Two waves cap a lane at
 // 256 registers: the FP32 learned kernel then spills a few registers in the
 // network's branch but keeps the occupancy of the standard FP32 kernel.
@@ -790,67 +901,15 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
         }
     }
     const bool collect = a.collect && live && (i % a.collect_stride) == 0;
-    double mF = 0., eF = 0., vFs[3] = {0., 0., 0.};
+    FluxSums acc{0., 0., {0., 0., 0.}};
     uint64_t defer[2] = {0ull, 0ull};
-    // the face of neighbour entry n, from the side that solves it
-    auto prepare = [&](int n, FacePrep<D, T> &fp, MixedProbe &mp, bool &self_lo) {
-        uint32_t e = a.nbr[(size_t)n * N + t];
-        int j = (int)nb_index(e);
-        uint32_t code = nb_code(e);
-        bool image = code != kCodeReal;
-        int s[3] = {0, 0, 0};
-        if (image)
-            for (int ax = 0; ax < D; ++ax) s[ax] = code_shift(code, ax);
-        // side that solves the pair: the lower original index (images:
-        // parent >= i)
-        const int jo = (int)a.pos[j].w;
-        self_lo = !a.enforce_flux_sym || (image ? jo >= i : jo > i);
-        const View<D, T> L = make_view<D, T>(a, self_lo ? t : j);
-        const View<D, T> H = make_view<D, T>(a, self_lo ? j : t);
-        int sl[3];
-        for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
-        face_prepare<D, T>(a, L, H, sl, image, dt, fp, fs);
-        if constexpr (kFp32) mp = face_probe<D>(a, fp, fs.st);
-    };
-    // network seed for this lane's face when want (all lanes call together)
-    auto net_seed = [&](bool want, const FacePrep<D, T> &fp) -> float {
-        float f[kMlpIn];
-        float pref = 1.f;
-        bool ok = want && fp.WR[0] > T(0) && fp.WL[0] > T(0) && fp.WR[1] > T(0) && fp.WL[1] > T(0);
-        if (ok)
-            pref = mlp_features_fast((float)a.gamma, (float)fp.WR[0], (float)fp.ui, (float)fp.WR[1],
-                                     (float)fp.WL[0], (float)fp.uj, (float)fp.WL[1], f);
-        else
-            for (int q = 0; q < kMlpIn; ++q) f[q] = 0.f;
-        float y = mlp_infer_wave_bf16_lean(a.mlp_frags, f, lds[threadIdx.x >> 6]);
-        fs.seeded += ok ? 1 : 0;
-        return ok ? mlp_seed_from_output(y, f, pref) : -1.f;
-    };
-    auto accumulate = [&](const FacePrep<D, T> &fp, const MixedProbe &mp, float seed, bool self_lo,
-                          int it0) {
-        T F[NF];
-        face_finish<D, T>(a, fp, seed, F, fs, kFp32 ? &mp : nullptr);
-        // training data: faces the exact solver needed >= 2 iterations for
-        if (collect && self_lo && fs.st.iters - it0 >= 2) {
-            unsigned long long slot = atomicAdd(a.collect_n, 1ull);
-            if (slot < (unsigned long long)a.collect_cap) {
-                double *c = a.collect + slot * 6;
-                c[0] = fp.WR[0]; c[1] = fp.ui; c[2] = fp.WR[1];
-                c[3] = fp.WL[0]; c[4] = fp.uj; c[5] = fp.WL[1];
-            }
-        }
-        double sgn = self_lo ? 1. : -1.;
-        mF += sgn * (double)F[0];
-        for (int ax = 0; ax < D; ++ax) vFs[ax] += sgn * (double)F[2 + ax];
-        eF += sgn * (double)F[1];
-    };
     for (int n = 0; n < lim; ++n) {
         const bool act = n < cnt;
         FacePrep<D, T> fp;
         MixedProbe mp{};
         bool self_lo = true;
         const int it0 = fs.st.iters;
-        if (act) prepare(n, fp, mp, self_lo);
+        if (act) face_of_entry<D, T>(a, t, i, n, dt, fp, mp, self_lo, fs);
         float seed = -1.f;
         if constexpr (kInLoopNet) {
             // FP64: the network runs for faces with a real jump
@@ -866,42 +925,26 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
                 strong = dp * dp > T(kSeedJump * kSeedJump) * psum * psum ||
                          du * du * rr > T(2 * kSeedJump * kSeedJump) * c2r;
             }
-            if (__ballot(strong)) seed = net_seed(strong, fp);
+            if (__ballot(strong)) seed = face_net_seed<D, T>(a, strong, fp, lds[threadIdx.x >> 6], fs);
         }
         if (act) {
             if (LEARNED && kFp32 && mp.valid && !mp.accepted && n < 128) {
                 defer[n >> 6] |= 1ull << (n & 63);  // Newton has to iterate
                 continue;
             }
-            accumulate(fp, mp, seed, self_lo, it0);
+            face_accumulate<D, T>(a, fp, mp, seed, self_lo, it0, collect, acc, fs);
         }
     }
     if constexpr (LEARNED && kFp32) {
-        // the deferred faces: recomputed, seeded by the network (one batch of
-        // the wavefront's lanes per round), solved from the seed
-        while (__ballot(defer[0] != 0ull || defer[1] != 0ull)) {
-            int n = -1;
-            if (defer[0]) {
-                n = __builtin_ctzll(defer[0]);
-                defer[0] &= defer[0] - 1ull;
-            } else if (defer[1]) {
-                n = 64 + __builtin_ctzll(defer[1]);
-                defer[1] &= defer[1] - 1ull;
-            }
-            const bool act = n >= 0;
-            FacePrep<D, T> fp;
-            MixedProbe mp{};
-            bool self_lo = true;
-            if (act) {  // counted by the face loop already
-                const FaceStats keep = fs;
-                prepare(n, fp, mp, self_lo);
-                fs = keep;
-            }
-            const int it0 = fs.st.iters;
-            float seed = net_seed(act && mp.valid && !mp.accepted, fp);
-            if (act) accumulate(fp, mp, seed, self_lo, it0);
+        if (__ballot(defer[0] != 0ull || defer[1] != 0ull)) {
+            DeferredOut r = faces_deferred<D>(a, t, i, (float)dt, defer[0], defer[1],
+                                              lds[threadIdx.x >> 6], acc, fs);
+            acc = r.acc;
+            fs = r.fs;
         }
     }
+    const double mF = acc.m, eF = acc.e;
+    const double vFs[3] = {acc.v[0], acc.v[1], acc.v[2]};
     if (live) {
         if (a.write_fluxes) {
             a.mF[i] = mF;
