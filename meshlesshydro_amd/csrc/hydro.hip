@@ -824,12 +824,18 @@ __device__ inline void face_accumulate(const HydroArgs &a, const FacePrep<D, T> 
 }
 
 // FP32 learned: the faces whose probe failed (bits of d0 / d1 = neighbour
-// entries 0..127), recomputed, seeded by the network one batch of the
-// wavefront's lanes per round, and solved from the seed.  A function of its
-// own, called only by wavefronts that hold such faces, so that the face loop
-// keeps the code and registers of the standard kernel (arguments and results
-// by value: no address of the loop's state escapes, the copies are made at
-// the call only).
+// entries 0..127 of this lane's particle).  The wavefront pools them in an
+// LDS queue (owner lane, entry), in lane order and ascending entry per lane,
+// and solves them 64 per round with every lane busy: face recomputed from
+// the owner's side, network seed (BF16 MFMA over the round's faces), solve
+// from the seed; each owner then adds its faces' fluxes in queue order, so
+// the sums are deterministic.  Entries beyond the queue's capacity are
+// solved afterwards by their own lanes.  A function of its own, called only
+// by wavefronts that hold such faces, so that the face loop keeps the code
+// and registers of the standard kernel (arguments and results by value: no
+// address of the loop's state escapes).
+constexpr int kDeferQueue = 4096;  // entries per wavefront
+
 struct DeferredOut {
     FluxSums acc;
     FaceStats fs;
@@ -838,8 +844,64 @@ struct DeferredOut {
 template <int D>
 __device__ __attribute__((noinline)) DeferredOut faces_deferred(const HydroArgs a, int t, int i,
                                                                 float dt, uint64_t d0, uint64_t d1,
-                                                                uint16_t *lds, FluxSums acc,
+                                                                uint16_t *lds, uint16_t *queue,
+                                                                float *res, FluxSums acc,
                                                                 FaceStats fs) {
+    constexpr int NF = D + 2;
+    const int lane = __lane_id();
+    const int c = __popcll(d0) + __popcll(d1);
+    int incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    const int excl = incl - c;
+    const int total = min(__shfl(incl, 63, 64), kDeferQueue);
+    int k = excl;
+    while (d0 && k < kDeferQueue) {
+        queue[k++] = (uint16_t)((lane << 8) | __builtin_ctzll(d0));
+        d0 &= d0 - 1ull;
+    }
+    while (d1 && k < kDeferQueue) {
+        queue[k++] = (uint16_t)((lane << 8) | (64 + __builtin_ctzll(d1)));
+        d1 &= d1 - 1ull;
+    }
+    const int mine_end = k;  // this lane's queue entries: [excl, mine_end)
+    wave_lds_sync();
+    const int t0 = t - lane;
+    for (int b = 0; b < total; b += 64) {
+        const int e = b + lane;
+        const bool act = e < total;
+        FacePrep<D, float> fp;
+        MixedProbe mp{};
+        bool self_lo = true;
+        if (act) {  // counted by the face loop already
+            const uint16_t q = queue[e];
+            const int to = t0 + (q >> 8), n = q & 255;
+            const int io = a.sidx ? (int)a.sidx[to] : to;
+            const FaceStats keep = fs;
+            face_of_entry<D, float>(a, to, io, n, dt, fp, mp, self_lo, fs);
+            fs = keep;
+        }
+        float seed = face_net_seed<D, float>(a, act && mp.valid && !mp.accepted, fp, lds, fs);
+        if (act) {
+            float F[NF];
+            face_finish<D, float>(a, fp, seed, F, fs, &mp);
+            const float sg = self_lo ? 1.f : -1.f;
+            for (int c2 = 0; c2 < NF; ++c2) res[lane * NF + c2] = sg * F[c2];
+        }
+        wave_lds_sync();
+        const int q0 = excl > b ? excl : b;
+        const int q1 = mine_end < b + 64 ? mine_end : b + 64;
+        for (int q = q0; q < q1; ++q) {
+            const float *r = res + (q - b) * NF;
+            acc.m += (double)r[0];
+            for (int ax = 0; ax < D; ++ax) acc.v[ax] += (double)r[2 + ax];
+            acc.e += (double)r[1];
+        }
+        wave_lds_sync();
+    }
+    // queue overflow: the remaining entries of each lane, by that lane
     while (__ballot(d0 != 0ull || d1 != 0ull)) {
         int n = -1;
         if (d0) {
@@ -853,7 +915,7 @@ __device__ __attribute__((noinline)) DeferredOut faces_deferred(const HydroArgs 
         FacePrep<D, float> fp;
         MixedProbe mp{};
         bool self_lo = true;
-        if (act) {  // counted by the face loop already
+        if (act) {
             const FaceStats keep = fs;
             face_of_entry<D, float>(a, t, i, n, dt, fp, mp, self_lo, fs);
             fs = keep;
@@ -880,6 +942,9 @@ __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_faces(HydroArgs a) {
     constexpr int NF = D + 2;
     __shared__ uint16_t lds[LEARNED ? 4 : 1][LEARNED ? 64 * 16 : 1];
+    constexpr bool kPool = LEARNED && sizeof(T) == 4;
+    __shared__ uint16_t dqueue[kPool ? 4 : 1][kPool ? kDeferQueue : 1];
+    __shared__ float dres[kPool ? 4 : 1][kPool ? 64 * (D + 2) : 1];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     const bool live = t < a.N;
     FaceStats fs{};
@@ -938,7 +1003,8 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     if constexpr (LEARNED && kFp32) {
         if (__ballot(defer[0] != 0ull || defer[1] != 0ull)) {
             DeferredOut r = faces_deferred<D>(a, t, i, (float)dt, defer[0], defer[1],
-                                              lds[threadIdx.x >> 6], acc, fs);
+                                              lds[threadIdx.x >> 6], dqueue[threadIdx.x >> 6],
+                                              dres[threadIdx.x >> 6], acc, fs);
             acc = r.acc;
             fs = r.fs;
         }
