@@ -31,6 +31,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue: one wave64 instruction per CU per clock (4 SIMDs x 16 lanes),
+# 256 CUs at 2.4 GHz.  Full-rate FP32 and FP64 ops issue at this rate;
+# transcendentals and FP64 divides take several issue slots.
+VALU_ISSUE_PEAK = 256 * 2.4e9
 
 
 def face_kernel_bytes(D, tsz, k_mean):
@@ -47,24 +51,42 @@ def face_kernel_bytes(D, tsz, k_mean):
     return own + row + write
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the counters of a
-    1 GiB copy): the line `traffic_bytes_per_launch <kernel> = <bytes>` of
-    profiles/pmc_traffic_blast128.md, or None when absent."""
+def profiled(key, kernel):
+    """A per-launch figure of `kernel` from the committed rocprofv3 PMC
+    summary profiles/pmc_traffic_blast128.md: the line `<key> <kernel> =
+    <value>`, or None when absent.  Keys: traffic_bytes_per_launch (HBM
+    bytes, FETCH_SIZE + WRITE_SIZE each scaled by the counters of a 1 GiB
+    copy) and valu_instr_per_launch (SQ_INSTS_VALU, wave instructions)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "pmc_traffic_blast128.md")
     try:
         with open(path) as fh:
             for line in fh:
                 line = line.strip().strip("`")
-                if line.startswith("traffic_bytes_per_launch ") and "=" in line:
-                    name, val = line[len("traffic_bytes_per_launch "):].rsplit("=", 1)
+                if line.startswith(key + " ") and "=" in line:
+                    name, val = line[len(key) + 1:].rsplit("=", 1)
                     if name.strip() == kernel:
                         return float(val)
     except OSError:
         pass
     return None
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary."""
+    return profiled("traffic_bytes_per_launch", kernel)
+
+
+def valu_roof(kernel, seconds):
+    """The face kernel's VALU issue rate against the chip's: wave-level
+    VALU instructions per launch (committed PMC summary) over the launch
+    time measured in this run."""
+    instr = profiled("valu_instr_per_launch", kernel)
+    if instr is None or seconds <= 0:
+        return None
+    rate = instr / seconds
+    return {"instr_per_launch": instr, "achieved_per_s": rate, "peak_per_s": VALU_ISSUE_PEAK,
+            "frac": rate / VALU_ISSUE_PEAK}
 
 
 def main():
@@ -287,7 +309,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(face_kernel),
                      "kernel": face_kernel,
-                     "algorithmic_bytes_per_launch": bytes_face},
+                     "algorithmic_bytes_per_launch": bytes_face,
+                     "valu": valu_roof(face_kernel, tface)},
         "cpu_baseline": cpu,
     }
     if "train" in head:
