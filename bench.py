@@ -89,6 +89,110 @@ def valu_roof(kernel, seconds):
             "frac": rate / VALU_ISSUE_PEAK}
 
 
+def sedov_run(args, state, base, log):
+    """The blast run to the time t_end at which the Sedov-Taylor shock
+    radius is args.sedov_radius, with each mode's own CFL steps (clipped to
+    land on t_end), then compared with the analytic solution: shell-mean
+    density and pressure profiles, shock position, conservation drift,
+    steps and wall time.  Prints one JSON line."""
+    from meshlesshydro_amd import analysis, lib
+    from meshlesshydro_amd import learned as L
+    from meshlesshydro_amd._abi import default_params, MLH_FP32, MLH_FP64, MLH_SEED_LEARNED
+
+    N = len(state["x"])
+    # the flag set of the reference's own blast case (testcases/sedov/
+    # parameter.h: CFL .25, BETA 1, PAIRWISE_LIMITER 1); under the
+    # Kelvin-Helmholtz long-run set (BETA 4, no pairwise limiter) this blast
+    # goes non-finite within a few dozen steps, the CPU oracle's too
+    base = dict(base, cfl=0.25, beta=1.0, pairwise_limiter=1)
+    sol = analysis.SedovTaylor(E=1.0, rho0=1.0, gamma=5.0 / 3.0, nu=3)
+    t_end = sol.time_for_radius(args.sedov_radius)
+    log(f"[sedov] t_end = {t_end:.6g} (R_s = {args.sedov_radius}), N = {N}")
+
+    def fields(st):
+        s = lib.Simulation(default_params(precision=MLH_FP64, **base), st)
+        s.neighbours()
+        s.density()
+        r, P, _ = s.get_density()
+        s.close()
+        return r, P
+
+    def advance(s, n_max):
+        """step with CFL dt clipped to land on t_end; stop before the first
+        step whose state has non-finite conserved sums.  Returns (steps, t,
+        failed): the reference algorithm ignores a singular E_i (its LAPACK
+        call's INFO is unchecked, Helper.cpp:7-18), and the blast's emptied
+        centre leaves particles with too few neighbours for a fixed kernel
+        size, so a long run can end in NaN as the reference's would."""
+        t, steps = 0.0, 0
+        while t_end - t > 1e-12 * t_end and steps < n_max:
+            dt = s.step(t_now=t, t_clip=t_end)
+            if not (np.all(np.isfinite(s.conserved())) and np.isfinite(dt)):
+                return steps, t, True
+            t += dt
+            steps += 1
+        return steps, t, False
+
+    def run(prec, seed_params=None, collect=False):
+        p = default_params(precision=prec, **base)
+
+        def fresh():
+            s = lib.Simulation(p, state)
+            if seed_params is not None:
+                s.mlp_set_params(seed_params)
+            return s
+
+        if seed_params is not None:
+            p.riemann_seed = MLH_SEED_LEARNED
+        s = fresh()
+        faces = None
+        if collect:
+            s.set_face_collection(1 << 21, 8)
+            s.run(5)
+            faces = s.collected_faces()
+            s.close()
+            s = fresh()
+        c0 = s.conserved()
+        t0 = time.perf_counter()
+        steps, t, failed = advance(s, args.sedov_max_steps)
+        wall = time.perf_counter() - t0
+        if failed:
+            # the run is deterministic: replay to the last finite state
+            s.close()
+            s = fresh()
+            advance(s, steps)
+        c1 = s.conserved()
+        d = s.diag()
+        st = s.state()
+        s.close()
+        rho, P = fields(st)
+        x = np.stack([st["x"], st["y"], st["z"]], axis=1)
+        cmp = analysis.sedov_compare(x, rho, P, t, solution=sol, nbins=64)
+        m0, e0 = c0[1], c0[2]  # conserved sums: volume, mass, energy, momentum
+        out = dict(steps=steps, reached_t_end=not failed, nonfinite_after_step=steps if failed else None,
+                   wall_s=wall, particle_steps_per_s=N * steps / wall,
+                   mass_drift=abs(c1[1] - m0) / abs(m0), energy_drift=abs(c1[2] - e0) / abs(e0),
+                   newton_iters_per_face=d["newton_iterations"] / max(1, d["faces_solved"]), **cmp)
+        return out, faces
+
+    res = {}
+    log("[sedov] FP64 plain")
+    res["fp64"], faces = run(MLH_FP64, collect=True)
+    log("[sedov] FP32 standard seed")
+    res["fp32_standard"], _ = run(MLH_FP32)
+    if faces is not None and len(faces):
+        p64 = default_params(precision=MLH_FP64, **base)
+        s = lib.Simulation(p64, state)
+        mlp_params, _ = L.train_seed_from_faces(s, faces, p64.gamma, n_epochs=4, log=log)
+        s.close()
+        log("[sedov] FP32 learned seed")
+        res["fp32_learned"], _ = run(MLH_FP32, seed_params=mlp_params)
+    print(json.dumps({"workload": f"3D point blast {round(N ** (1 / 3))}^3 periodic to the Sedov-Taylor "
+                                  f"shock radius {args.sedov_radius}",
+                      "flags": "CFL .25, BETA 1, PAIRWISE_LIMITER 1 (testcases/sedov/parameter.h)",
+                      "t_end": t_end, "xi0": sol.xi0, "modes": res}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +208,12 @@ def main():
                          "to scale FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--only", choices=["fp64", "fp32", "learned"], default=None,
                     help="profile helper: run one mode for --steps steps and print its timings")
+    ap.add_argument("--sedov", action="store_true",
+                    help="run the blast to the time its Sedov-Taylor shock reaches --sedov-radius "
+                         "(own CFL steps, FP64, FP32 and FP32 + learned seed) and compare the "
+                         "final density and pressure profiles with the analytic solution")
+    ap.add_argument("--sedov-radius", type=float, default=0.07)
+    ap.add_argument("--sedov-max-steps", type=int, default=20000)
     ap.add_argument("--order", choices=["auto", "cell", "caller"], default="auto",
                     help="work order of the kernels (results are identical)")
     ap.add_argument("--shuffle", action="store_true",
@@ -159,6 +269,10 @@ def main():
         t = time.perf_counter() - t0
         print(json.dumps({"mode": args.only, "particle_steps_per_s": N * K / t,
                           "kernel_ms": [float(x) for x in s.timings()], "diag": s.diag()}))
+        return
+
+    if args.sedov:
+        sedov_run(args, state, base, log)
         return
 
     # 1. FP64 plain, own CFL steps; the warmup steps also record the exact

@@ -54,6 +54,8 @@ struct mlh_ctx {
     unsigned long long *counters = nullptr;
     int *status = nullptr, *max_counts = nullptr;
     double *sums_block = nullptr;
+    void *defer_rec = nullptr;  // FP32 learned seed: deferred particles
+    int *defer_n = nullptr;
     MlpState mlp;
     int stage = 0;  // 0 state, 1 neighbours, 2 density, 3 gradients, 4 fluxes
     int img_window = 2;
@@ -210,6 +212,8 @@ static HydroArgs hydro_args(mlh_ctx *c) {
     a.counters = c->counters;
     a.status = c->status;
     a.mlp_frags = c->mlp.frags;
+    a.defer_rec = c->defer_rec;
+    a.defer_n = c->defer_n;
     a.collect = c->mlp.collect_stride > 0 ? c->mlp.collect : nullptr;
     a.collect_n = c->mlp.collect_n;
     a.collect_cap = c->mlp.collect_cap;
@@ -275,6 +279,10 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     A_(&c->status, sizeof(int) * 2);
     A_(&c->max_counts, sizeof(int) * 2);
     A_(&c->sums_block, sizeof(double) * 6 * (size_t)dt_blocks(n));
+    if (p->riemann_seed == MLH_SEED_LEARNED && c->prec == MLH_FP32) {
+        A_(&c->defer_rec, defer_record_bytes() * N);
+        A_(&c->defer_n, sizeof(int));
+    }
 #undef A_
     hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * kNumCounters, c->s);
     hipMemsetAsync(c->status, 0, sizeof(int) * 2, c->s);

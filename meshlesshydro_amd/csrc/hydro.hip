@@ -243,9 +243,16 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                     xij[ax] = xi[ax] - xj[ax];
                     vij[ax] = vi[ax] - (kFused ? (double)prj[kPrimV + ax] : (double)pj[kPrimV + ax]);
                 }
-                double vx = dotT<D, double>(vij, xij) / sqrt(dotT<D, double>(xij, xij));
-                vx = vx < 0. ? vx : 0.;
-                double vs = ci + cj - vx;
+                // vx = min(v_ij.x_ij / |x_ij|, 0): only approaching pairs
+                // (v_ij.x_ij < 0) need the divide and square root; for the
+                // others (including NaN) the reference's value is ci + cj
+                double dv = dotT<D, double>(vij, xij);
+                double vs = ci + cj;
+                if (dv < 0.) {
+                    double vx = dv / sqrt(dotT<D, double>(xij, xij));
+                    vx = vx < 0. ? vx : 0.;
+                    vs = vs - vx;
+                }
                 vsig = vs > vsig ? vs : vsig;
             }
         }
@@ -823,108 +830,76 @@ __device__ inline void face_accumulate(const HydroArgs &a, const FacePrep<D, T> 
     acc.e += sgn * (double)F[1];
 }
 
-// FP32 learned: the faces whose probe failed (bits of d0 / d1 = neighbour
-// entries 0..127 of this lane's particle).  The wavefront pools them in an
-// LDS queue (owner lane, entry), in lane order and ascending entry per lane,
-// and solves them 64 per round with every lane busy: face recomputed from
-// the owner's side, network seed (BF16 MFMA over the round's faces), solve
-// from the seed; each owner then adds its faces' fluxes in queue order, so
-// the sums are deterministic.  Entries beyond the queue's capacity are
-// solved afterwards by their own lanes.  A function of its own, called only
-// by wavefronts that hold such faces, so that the face loop keeps the code
-// and registers of the standard kernel (arguments and results by value: no
-// address of the loop's state escapes).
-constexpr int kDeferQueue = 4096;  // entries per wavefront
-
-struct DeferredOut {
-    FluxSums acc;
-    FaceStats fs;
+// FP32 learned: a particle with faces whose probe failed (bits of d0 / d1 =
+// neighbour entries 0..127) leaves the face loop with the sums of its other
+// faces in one of these records; k_faces_deferred solves the recorded faces
+// with the network's seed, adds them in entry order and finishes the
+// particle.
+struct DeferRec {
+    int t, pad;
+    unsigned long long d0, d1;
+    double m, e, v[3];
 };
+static_assert(sizeof(DeferRec) == 64, "DeferRec is one 64-byte record");
+size_t defer_record_bytes() { return sizeof(DeferRec); }
 
+// Flux sums and updateStateAndPosition (Particles.cpp:2013-2110) of the
+// particle at work position t (original index i).
 template <int D>
-__device__ __attribute__((noinline)) DeferredOut faces_deferred(const HydroArgs a, int t, int i,
-                                                                float dt, uint64_t d0, uint64_t d1,
-                                                                uint16_t *lds, uint16_t *queue,
-                                                                float *res, FluxSums acc,
-                                                                FaceStats fs) {
-    constexpr int NF = D + 2;
-    const int lane = __lane_id();
-    const int c = __popcll(d0) + __popcll(d1);
-    int incl = c;
-    for (int o = 1; o < 64; o <<= 1) {
-        int v = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += v;
+__device__ inline void particle_finish(const HydroArgs &a, int i, const FluxSums acc) {
+    const double dtd = a.dt[0];
+    const double mF = acc.m, eF = acc.e;
+    const double vFs[3] = {acc.v[0], acc.v[1], acc.v[2]};
+    if (a.write_fluxes) {
+        a.mF[i] = mF;
+        a.eF[i] = eF;
+        for (int ax = 0; ax < D; ++ax) a.vF[(size_t)i * D + ax] = vFs[ax];
     }
-    const int excl = incl - c;
-    const int total = min(__shfl(incl, 63, 64), kDeferQueue);
-    int k = excl;
-    while (d0 && k < kDeferQueue) {
-        queue[k++] = (uint16_t)((lane << 8) | __builtin_ctzll(d0));
-        d0 &= d0 - 1ull;
+    double vo[3] = {a.v[0][i], a.v[1][i], D == 3 ? a.v[2][i] : 0.};
+    double m0 = a.m[i], u0 = a.u[i];
+    double ke = vo[0] * vo[0] + vo[1] * vo[1];
+    if (D == 3) ke += vo[2] * vo[2];
+    double Q0 = m0 * (u0 + .5 * ke);
+    double Q[3];
+    for (int ax = 0; ax < D; ++ax) Q[ax] = m0 * vo[ax];
+    double m1 = a.mfm ? m0 : m0 - dtd * mF;
+    double vn[3];
+    for (int ax = 0; ax < D; ++ax) {
+        Q[ax] -= dtd * vFs[ax];
+        vn[ax] = Q[ax] / m1;
     }
-    while (d1 && k < kDeferQueue) {
-        queue[k++] = (uint16_t)((lane << 8) | (64 + __builtin_ctzll(d1)));
-        d1 &= d1 - 1ull;
+    Q0 -= dtd * eF;
+    double kn = vn[0] * vn[0] + vn[1] * vn[1];
+    if (D == 3) kn += vn[2] * vn[2];
+    double u1 = Q0 / m1 - .5 * kn;
+    a.mn[i] = m1;
+    a.un[i] = u1;
+    for (int ax = 0; ax < D; ++ax) {
+        a.vn[ax][i] = vn[ax];
+        double x = a.x[ax][i];
+        if (a.move_particles) {
+            x += vo[ax] * dtd;
+            if (a.g.periodic) {
+                double mn = a.g.bmin[ax], mx = a.g.bmax[ax];
+                if (x < mn) x = mx - (mn - x);
+                else if (mx <= x) x = mn + (x - mx);
+            }
+        }
+        a.xn[ax][i] = x;
     }
-    const int mine_end = k;  // this lane's queue entries: [excl, mine_end)
-    wave_lds_sync();
-    const int t0 = t - lane;
-    for (int b = 0; b < total; b += 64) {
-        const int e = b + lane;
-        const bool act = e < total;
-        FacePrep<D, float> fp;
-        MixedProbe mp{};
-        bool self_lo = true;
-        if (act) {  // counted by the face loop already
-            const uint16_t q = queue[e];
-            const int to = t0 + (q >> 8), n = q & 255;
-            const int io = a.sidx ? (int)a.sidx[to] : to;
-            const FaceStats keep = fs;
-            face_of_entry<D, float>(a, to, io, n, dt, fp, mp, self_lo, fs);
-            fs = keep;
-        }
-        float seed = face_net_seed<D, float>(a, act && mp.valid && !mp.accepted, fp, lds, fs);
-        if (act) {
-            float F[NF];
-            face_finish<D, float>(a, fp, seed, F, fs, &mp);
-            const float sg = self_lo ? 1.f : -1.f;
-            for (int c2 = 0; c2 < NF; ++c2) res[lane * NF + c2] = sg * F[c2];
-        }
-        wave_lds_sync();
-        const int q0 = excl > b ? excl : b;
-        const int q1 = mine_end < b + 64 ? mine_end : b + 64;
-        for (int q = q0; q < q1; ++q) {
-            const float *r = res + (q - b) * NF;
-            acc.m += (double)r[0];
-            for (int ax = 0; ax < D; ++ax) acc.v[ax] += (double)r[2 + ax];
-            acc.e += (double)r[1];
-        }
-        wave_lds_sync();
+}
+
+// diagnostics: one atomic per wavefront and counter
+__device__ inline void add_counters(const HydroArgs &a, const FaceStats fs, int faces) {
+    unsigned long long c[kNumCounters] = {
+        (unsigned long long)fs.neg_real, (unsigned long long)fs.neg_ghost,
+        (unsigned long long)fs.vacuum, (unsigned long long)fs.st.nonconv, 0ull,
+        (unsigned long long)fs.st.fallback, (unsigned long long)fs.st.iters,
+        (unsigned long long)faces, (unsigned long long)fs.seeded};
+    for (int k = 0; k < kNumCounters; ++k) {
+        unsigned long long v = wave_sum(c[k]);
+        if (lane_id() == 0 && v) atomicAdd(&a.counters[k], v);
     }
-    // queue overflow: the remaining entries of each lane, by that lane
-    while (__ballot(d0 != 0ull || d1 != 0ull)) {
-        int n = -1;
-        if (d0) {
-            n = __builtin_ctzll(d0);
-            d0 &= d0 - 1ull;
-        } else if (d1) {
-            n = 64 + __builtin_ctzll(d1);
-            d1 &= d1 - 1ull;
-        }
-        const bool act = n >= 0;
-        FacePrep<D, float> fp;
-        MixedProbe mp{};
-        bool self_lo = true;
-        if (act) {
-            const FaceStats keep = fs;
-            face_of_entry<D, float>(a, t, i, n, dt, fp, mp, self_lo, fs);
-            fs = keep;
-        }
-        const int it0 = fs.st.iters;
-        float seed = face_net_seed<D, float>(a, act && mp.valid && !mp.accepted, fp, lds, fs);
-        if (act) face_accumulate<D, float>(a, fp, mp, seed, self_lo, it0, false, acc, fs);
-    }
-    return DeferredOut{acc, fs};
 }
 
 // Waves per SIMD the face kernel is compiled for.  Two waves cap a lane at
@@ -941,10 +916,12 @@ template <int D, typename T, bool LEARNED>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_faces(HydroArgs a) {
     constexpr int NF = D + 2;
-    __shared__ uint16_t lds[LEARNED ? 4 : 1][LEARNED ? 64 * 16 : 1];
-    constexpr bool kPool = LEARNED && sizeof(T) == 4;
-    __shared__ uint16_t dqueue[kPool ? 4 : 1][kPool ? kDeferQueue : 1];
-    __shared__ float dres[kPool ? 4 : 1][kPool ? 64 * (D + 2) : 1];
+    constexpr bool kFp32 = sizeof(T) == 4;
+    // FP32 learned: faces whose probe fails are deferred to k_faces_deferred,
+    // which seeds them with the network; the FP64 learned kernel runs the
+    // network inside the face loop, over the wavefront's longest row
+    constexpr bool kInLoopNet = LEARNED && !kFp32;
+    __shared__ uint16_t lds[kInLoopNet ? 4 : 1][kInLoopNet ? 64 * 16 : 1];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     const bool live = t < a.N;
     FaceStats fs{};
@@ -953,11 +930,6 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     const double dtd = a.dt[0];
     const T dt = T(dtd);
     int cnt = live ? a.noi[t] + a.nog[t] : 0;
-    constexpr bool kFp32 = sizeof(T) == 4;
-    // FP32 learned: faces whose probe fails are deferred to a second loop
-    // that seeds them with the network; the FP64 learned kernel runs the
-    // network inside the face loop, over the wavefront's longest row
-    constexpr bool kInLoopNet = LEARNED && !kFp32;
     int lim = cnt;
     if (kInLoopNet) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -1000,68 +972,141 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
             face_accumulate<D, T>(a, fp, mp, seed, self_lo, it0, collect, acc, fs);
         }
     }
+    bool finish = live;
     if constexpr (LEARNED && kFp32) {
-        if (__ballot(defer[0] != 0ull || defer[1] != 0ull)) {
-            DeferredOut r = faces_deferred<D>(a, t, i, (float)dt, defer[0], defer[1],
-                                              lds[threadIdx.x >> 6], dqueue[threadIdx.x >> 6],
-                                              dres[threadIdx.x >> 6], acc, fs);
-            acc = r.acc;
-            fs = r.fs;
-        }
-    }
-    const double mF = acc.m, eF = acc.e;
-    const double vFs[3] = {acc.v[0], acc.v[1], acc.v[2]};
-    if (live) {
-        if (a.write_fluxes) {
-            a.mF[i] = mF;
-            a.eF[i] = eF;
-            for (int ax = 0; ax < D; ++ax) a.vF[(size_t)i * D + ax] = vFs[ax];
-        }
-        // updateStateAndPosition, Particles.cpp:2013-2110
-        double vo[3] = {a.v[0][i], a.v[1][i], D == 3 ? a.v[2][i] : 0.};
-        double m0 = a.m[i], u0 = a.u[i];
-        double ke = vo[0] * vo[0] + vo[1] * vo[1];
-        if (D == 3) ke += vo[2] * vo[2];
-        double Q0 = m0 * (u0 + .5 * ke);
-        double Q[3];
-        for (int ax = 0; ax < D; ++ax) Q[ax] = m0 * vo[ax];
-        double m1 = a.mfm ? m0 : m0 - dtd * mF;
-        double vn[3];
-        for (int ax = 0; ax < D; ++ax) {
-            Q[ax] -= dtd * vFs[ax];
-            vn[ax] = Q[ax] / m1;
-        }
-        Q0 -= dtd * eF;
-        double kn = vn[0] * vn[0] + vn[1] * vn[1];
-        if (D == 3) kn += vn[2] * vn[2];
-        double u1 = Q0 / m1 - .5 * kn;
-        a.mn[i] = m1;
-        a.un[i] = u1;
-        for (int ax = 0; ax < D; ++ax) {
-            a.vn[ax][i] = vn[ax];
-            double x = a.x[ax][i];
-            if (a.move_particles) {
-                x += vo[ax] * dtd;
-                if (a.g.periodic) {
-                    double mn = a.g.bmin[ax], mx = a.g.bmax[ax];
-                    if (x < mn) x = mx - (mn - x);
-                    else if (mx <= x) x = mn + (x - mx);
-                }
+        // a record per particle with deferred faces: one atomic per
+        // wavefront for the slots; the particle is finished by
+        // k_faces_deferred (record order does not matter: each record is
+        // completed on its own)
+        const bool dfr = live && (defer[0] | defer[1]) != 0ull;
+        const unsigned long long bal = __ballot(dfr);
+        if (bal) {
+            const int lane = lane_id();
+            const int first = __ffsll((long long)bal) - 1;
+            int base = 0;
+            if (lane == first) base = atomicAdd(a.defer_n, __popcll(bal));
+            base = __shfl(base, first, 64);
+            if (dfr) {
+                const int slot = base + __popcll(bal & ((1ull << lane) - 1ull));
+                DeferRec *r = (DeferRec *)a.defer_rec + slot;
+                r->t = t;
+                r->pad = 0;
+                r->d0 = defer[0];
+                r->d1 = defer[1];
+                r->m = acc.m;
+                r->e = acc.e;
+                for (int ax = 0; ax < 3; ++ax) r->v[ax] = acc.v[ax];
+                finish = false;
             }
-            a.xn[ax][i] = x;
         }
     }
-    // diagnostics: one atomic per wavefront and counter
-    unsigned long long c[kNumCounters] = {
-        (unsigned long long)fs.neg_real, (unsigned long long)fs.neg_ghost,
-        (unsigned long long)fs.vacuum, (unsigned long long)fs.st.nonconv, 0ull,
-        (unsigned long long)fs.st.fallback, (unsigned long long)fs.st.iters,
-        (unsigned long long)cnt, (unsigned long long)fs.seeded};
-    for (int k = 0; k < kNumCounters; ++k) {
-        unsigned long long v = wave_sum(c[k]);
-        if (lane_id() == 0 && v) atomicAdd(&a.counters[k], v);
+    if (finish) particle_finish<D>(a, i, acc);
+    add_counters(a, fs, cnt);
+}
+
+// FP32 learned seed: the deferred faces of the particles recorded by
+// k_faces.  A wavefront takes kDeferGroup records (one per owner lane),
+// packs their deferred faces into an LDS queue (owner lane, entry) in owner
+// order and ascending entry, and solves them 64 per round with every lane
+// busy: face recomputed from the solving side, one BF16 MFMA inference for
+// the round's faces, solve from the network's seed (FP32 Newton, FP64
+// acceptance, FP64 fallback).  Each owner then adds its faces' fluxes in
+// queue order -- its entries in ascending order, after the sums of its other
+// faces -- and finishes the particle, so the result does not depend on how
+// records are grouped.  Grid-stride over the records, the count read from
+// the device.
+constexpr int kDeferGroup = 16;                    // records per wavefront
+constexpr int kDeferQueue = kDeferGroup * 128;     // entries per wavefront
+
+template <int D>
+__global__ __launch_bounds__(256) void k_faces_deferred(HydroArgs a) {
+    constexpr int NF = D + 2;
+    __shared__ uint16_t lds[4][64 * 16];
+    __shared__ uint16_t queue[4][kDeferQueue];
+    __shared__ float res[4][64 * NF];
+    __shared__ int own_t[4][kDeferGroup], own_i[4][kDeferGroup];
+    const int w = threadIdx.x >> 6;
+    const int count = *a.defer_n;
+    const int lane = lane_id();
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+    const float dt = (float)a.dt[0];
+    const DeferRec *recs = (const DeferRec *)a.defer_rec;
+    for (int base = wave * kDeferGroup; base < count; base += nwaves * kDeferGroup) {
+        const int r = base + lane;
+        const bool owner = lane < kDeferGroup && r < count;
+        int t = 0, i = 0;
+        unsigned long long d0 = 0ull, d1 = 0ull;
+        FluxSums acc{0., 0., {0., 0., 0.}};
+        if (owner) {
+            const DeferRec q = recs[r];
+            t = q.t;
+            i = a.sidx ? (int)a.sidx[t] : t;
+            d0 = q.d0;
+            d1 = q.d1;
+            acc.m = q.m;
+            acc.e = q.e;
+            for (int ax = 0; ax < 3; ++ax) acc.v[ax] = q.v[ax];
+            own_t[w][lane] = t;
+            own_i[w][lane] = i;
+        }
+        // queue positions: exclusive prefix sum of the owners' face counts
+        const int c = __popcll(d0) + __popcll(d1);
+        int incl = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        const int excl = incl - c;
+        const int total = __shfl(incl, 63, 64);  // <= kDeferGroup * 128
+        int k = excl;
+        while (d0) {
+            queue[w][k++] = (uint16_t)((lane << 8) | __builtin_ctzll(d0));
+            d0 &= d0 - 1ull;
+        }
+        while (d1) {
+            queue[w][k++] = (uint16_t)((lane << 8) | (64 + __builtin_ctzll(d1)));
+            d1 &= d1 - 1ull;
+        }
+        wave_lds_sync();
+        FaceStats fs{};
+        for (int b = 0; b < total; b += 64) {
+            const int e = b + lane;
+            const bool act = e < total;
+            FacePrep<D, float> fp;
+            MixedProbe mp{};
+            bool self_lo = true;
+            if (act) {  // the probe was counted by k_faces
+                const uint16_t q = queue[w][e];
+                const int o = q >> 8, n = q & 255;
+                const FaceStats keep = fs;
+                face_of_entry<D, float>(a, own_t[w][o], own_i[w][o], n, dt, fp, mp, self_lo, fs);
+                fs = keep;
+            }
+            float seed = face_net_seed<D, float>(a, act && mp.valid && !mp.accepted, fp, lds[w], fs);
+            if (act) {
+                float F[NF];
+                face_finish<D, float>(a, fp, seed, F, fs, &mp);
+                const float sg = self_lo ? 1.f : -1.f;
+                for (int c2 = 0; c2 < NF; ++c2) res[w][lane * NF + c2] = sg * F[c2];
+            }
+            wave_lds_sync();
+            const int q0 = excl > b ? excl : b;
+            const int q1 = k < b + 64 ? k : b + 64;
+            for (int q = q0; q < q1; ++q) {
+                const float *rr = &res[w][(q - b) * NF];
+                acc.m += (double)rr[0];
+                for (int ax = 0; ax < D; ++ax) acc.v[ax] += (double)rr[2 + ax];
+                acc.e += (double)rr[1];
+            }
+            wave_lds_sync();
+        }
+        if (owner) particle_finish<D>(a, i, acc);
+        add_counters(a, fs, 0);
     }
 }
+
+__global__ void k_defer_reset(int *n) { *n = 0; }
 
 // conserved sums (Particles.cpp:2830-2886): V, M, E, px, py, pz per block
 template <int D, typename T>
@@ -1121,9 +1166,17 @@ void launch_faces(const HydroArgs &a, int D, int precision, hipStream_t s) {
     dim3 grid((a.N + 255) / 256);
     if (a.learned) {
         if (D == 2 && precision == 0) hipLaunchKernelGGL((k_faces<2, double, true>), grid, dim3(256), 0, s, a);
-        else if (D == 2) hipLaunchKernelGGL((k_faces<2, float, true>), grid, dim3(256), 0, s, a);
         else if (precision == 0) hipLaunchKernelGGL((k_faces<3, double, true>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_faces<3, float, true>), grid, dim3(256), 0, s, a);
+        else {
+            // FP32: the face loop defers the faces that need Newton to
+            // k_faces_deferred (a small grid-stride launch over the records)
+            hipLaunchKernelGGL(k_defer_reset, dim3(1), dim3(1), 0, s, a.defer_n);
+            if (D == 2) hipLaunchKernelGGL((k_faces<2, float, true>), grid, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((k_faces<3, float, true>), grid, dim3(256), 0, s, a);
+            dim3 dgrid(grid.x < 1024 ? grid.x : 1024);
+            if (D == 2) hipLaunchKernelGGL((k_faces_deferred<2>), dgrid, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((k_faces_deferred<3>), dgrid, dim3(256), 0, s, a);
+        }
     } else {
         if (D == 2 && precision == 0) hipLaunchKernelGGL((k_faces<2, double, false>), grid, dim3(256), 0, s, a);
         else if (D == 2) hipLaunchKernelGGL((k_faces<2, float, false>), grid, dim3(256), 0, s, a);

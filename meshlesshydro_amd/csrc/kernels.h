@@ -49,6 +49,11 @@ struct HydroArgs {
     unsigned long long *counters;  // mlh_diag counters
     int *status;
     const struct MlpFrags *mlp_frags;  // learned seed network (device)
+    // FP32 learned seed: particles whose faces wait for the network
+    // (DeferRec[N], written by k_faces, consumed by k_faces_deferred) and
+    // their count
+    void *defer_rec;
+    int *defer_n;
     double *collect;             // face collection [cap][6] (or null)
     unsigned long long *collect_n;
     long long collect_cap;
@@ -83,6 +88,8 @@ int dt_blocks(int N);
 void launch_dt_finalize(const HydroArgs &a, double dt_fixed, double t_now, double t_clip,
                         const double *dt_in, double *dt_hist, int step, hipStream_t s);
 void launch_faces(const HydroArgs &a, int D, int precision, hipStream_t s);
+// bytes of one deferred-particle record (FP32 learned seed)
+size_t defer_record_bytes();
 void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s);
 void launch_riemann_flux(const HydroArgs &a, int D, long long n, const double *wl,
                          const double *wr, const double *vf, const double *aij, double *f,

@@ -34,6 +34,8 @@ def hostlib():
         L.mlh_host_config_value.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32]
         L.mlh_host_run.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(C.c_int64), _dp, _ip]
         L.mlh_h5_write_ic.argtypes = [C.c_char_p, C.c_int32, C.c_int32] + [_dp] * 4 + [_ip]
+        L.mlh_h5_write_snapshot.argtypes = ([C.c_char_p, C.c_int32, C.c_int32] + [C.c_double] * 3 +
+                                            [_dp] * 8 + [_ip])
         L.mlh_h5_dataset_shape.argtypes = [C.c_char_p, C.c_char_p, _ip, C.POINTER(C.c_int64)]
         L.mlh_h5_read_f64.argtypes = [C.c_char_p, C.c_char_p, _dp, C.c_int64]
         L.mlh_h5_read_i32.argtypes = [C.c_char_p, C.c_char_p, _ip, C.c_int64]
@@ -79,6 +81,22 @@ def write_ic(path, state, dim):
     mat = np.zeros(n, np.int32)
     d = lambda a: a.ctypes.data_as(_dp)
     if hostlib().mlh_h5_write_ic(_b(path), n, dim, d(X), d(V), d(m), d(u), mat.ctypes.data_as(_ip)) != 0:
+        raise HostError(-1, hostlib().mlh_host_last_error().decode())
+
+
+def write_snapshot(path, snap, dim):
+    """Snapshot file in the dump2file layout from a dict with `time`,
+    `totalMass`, `energy`, `momentum` (dim values) and per-particle `rho`,
+    `m`, `u`, `P`, `noi`, `x`, `v`, `rhoGrad` ([N][dim])."""
+    f = lambda k: np.ascontiguousarray(snap[k], np.float64)
+    n = len(snap["m"])
+    arrs = {k: f(k) for k in ("momentum", "rho", "m", "u", "x", "v", "rhoGrad", "P")}
+    noi = np.ascontiguousarray(snap["noi"], np.int32)
+    d = lambda k: arrs[k].ctypes.data_as(_dp)
+    if hostlib().mlh_h5_write_snapshot(_b(path), n, dim, float(snap["time"]), float(snap["totalMass"]),
+                                       float(snap["energy"]), d("momentum"), d("rho"), d("m"), d("u"),
+                                       d("x"), d("v"), d("rhoGrad"), d("P"),
+                                       noi.ctypes.data_as(_ip)) != 0:
         raise HostError(-1, hostlib().mlh_host_last_error().decode())
 
 

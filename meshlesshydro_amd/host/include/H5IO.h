@@ -50,6 +50,13 @@ extern "C" {
 const char *mlh_host_last_error(void);
 int mlh_h5_write_ic(const char *file, int32_t n, int32_t dim, const double *x, const double *v,
                     const double *m, const double *u, const int32_t *material_id);
+/* one snapshot in the dump2file layout; x, v, rho_grad are [n][dim],
+   momentum has dim entries */
+int mlh_h5_write_snapshot(const char *file, int32_t n, int32_t dim, double time,
+                          double total_mass, double energy, const double *momentum,
+                          const double *rho, const double *m, const double *u, const double *x,
+                          const double *v, const double *rho_grad, const double *P,
+                          const int32_t *noi);
 /* shape of dataset `name`: ndims <= 2 into dims[0..1] */
 int mlh_h5_dataset_shape(const char *file, const char *name, int32_t *ndims, int64_t *dims);
 int mlh_h5_read_f64(const char *file, const char *name, double *out, int64_t count);

@@ -170,6 +170,34 @@ int mlh_h5_write_ic(const char *file, int32_t n, int32_t dim, const double *x, c
     });
 }
 
+int mlh_h5_write_snapshot(const char *file, int32_t n, int32_t dim, double time,
+                          double total_mass, double energy, const double *momentum,
+                          const double *rho, const double *m, const double *u, const double *x,
+                          const double *v, const double *rho_grad, const double *P,
+                          const int32_t *noi) {
+    return guard([&] {
+        if (!file || n < 0 || (dim != 2 && dim != 3) || !momentum || !rho || !m || !u || !x ||
+            !v || !rho_grad || !P || !noi)
+            throw std::runtime_error("mlh_h5_write_snapshot: bad arguments");
+        mlh_host::Snapshot s;
+        s.N = n;
+        s.D = dim;
+        s.time = time;
+        s.totalMass = total_mass;
+        s.energy = energy;
+        for (int a = 0; a < dim; ++a) s.mom[a] = momentum[a];
+        s.rho.assign(rho, rho + n);
+        s.m.assign(m, m + n);
+        s.u.assign(u, u + n);
+        s.P.assign(P, P + n);
+        s.x.assign(x, x + (size_t)n * dim);
+        s.v.assign(v, v + (size_t)n * dim);
+        s.rhoGrad.assign(rho_grad, rho_grad + (size_t)n * dim);
+        s.noi.assign(noi, noi + n);
+        mlh_host::write_snapshot(file, s);
+    });
+}
+
 int mlh_h5_dataset_shape(const char *file, const char *name, int32_t *ndims, int64_t *dims) {
     return guard([&] {
         if (!file || !name || !ndims || !dims) throw std::runtime_error("null argument");
