@@ -268,7 +268,9 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
         A_(&c->lim_scratch, sizeof(double) * 3 * D * (size_t)limits_blocks(n));
         A_(&c->lim_out, sizeof(double) * 2 * D);
     }
-    A_(&c->nbr, 4 * N * (size_t)c->stride);
+    // four guard rows past the list: the search stores up to four hits past
+    // MAX_NUM_INTERACTIONS before it detects the overflow
+    A_(&c->nbr, 4 * N * ((size_t)c->stride + 4));
     A_(&c->noi, 4 * N); A_(&c->nog, 4 * N);
     A_(&c->prim, c->tsz * kPrimN * N);
     A_(&c->frec, c->tsz * frn * N);

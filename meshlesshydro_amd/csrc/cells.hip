@@ -265,127 +265,156 @@ __device__ inline double box_gap(double x, int k, double bmin, double cs, int nc
 
 // One thread per work position t.  Candidates are scanned in cell order
 // (pos_cell); an entry stores the candidate's work position: its cell-order
-// position, or its original index when the work order is the caller's.
-template <int D>
+// position (kCaller false), or its original index when the work order is
+// the caller's (kCaller true).
+template <int D, bool kCaller>
 __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
-    int t = xcd_block() * blockDim.x + threadIdx.x;
-    if (t >= a.N) return;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    const bool live = t < a.N;
     const Geom &g = a.g;
     const int N = a.N;
-    const bool co = a.caller_order != 0;
-    double4 pi = a.pos_w[t];
-    double xi[3] = {pi.x, pi.y, pi.z};
-    const uint32_t io = (uint32_t)pi.w;
-    uint32_t ck = a.ck_w[t];
-    int cc[3];
-    cell_coords<D>(g, ck, cc);
-    uint32_t *row = a.nbr + t;
-    int cnt = 0, err = 0;
-    // real neighbours, reference stencil order.  A stencil cell whose box is
-    // farther than h from x_i (with a relative margin far above the
-    // rounding of positions and cell indices; edge cells are open outward,
-    // where clamped particles sit) holds no neighbour and is skipped.  The
-    // candidates of a cell are read four at a time, so that their loads are
-    // in flight together.
-    const double hlim = g.hsq * (1.0 + 1e-9);
-    for (int k = cc[0] - 1; k <= cc[0] + 1 && !err; ++k) {
-        if (k < 0 || k >= g.nc[0]) continue;
-        double bx = box_gap(xi[0], k, g.bmin[0], g.cs[0], g.nc[0]);
-        for (int l = cc[1] - 1; l <= cc[1] + 1 && !err; ++l) {
-            if (l < 0 || l >= g.nc[1]) continue;
-            double by = box_gap(xi[1], l, g.bmin[1], g.cs[1], g.nc[1]);
-            for (int m = (D == 3 ? cc[2] - 1 : 0); m <= (D == 3 ? cc[2] + 1 : 0) && !err; ++m) {
-                if (D == 3 && (m < 0 || m >= g.nc[2])) continue;
-                double bz = D == 3 ? box_gap(xi[2], m, g.bmin[2], g.cs[2], g.nc[2]) : 0.0;
-                if (bx * bx + by * by + bz * bz > hlim) continue;
-                uint32_t c = cell_key<D>(g, k, l, m);
-                const int q1 = a.cend[c];
-                for (int q = a.cstart[c]; q < q1 && !err; q += 4) {
-                    double4 pj[4];
+    int cnt = 0, gcnt = 0;
+    if (live) {
+        double4 pi = a.pos_w[t];
+        double xi[3] = {pi.x, pi.y, pi.z};
+        const uint32_t io = (uint32_t)pi.w;
+        uint32_t ck = a.ck_w[t];
+        int cc[3];
+        cell_coords<D>(g, ck, cc);
+        uint32_t *row = a.nbr + t;
+        int err = 0;
+        // real neighbours, reference stencil order (i itself excluded: in
+        // cell order its candidate position is t).  A stencil cell whose box
+        // is farther than h from x_i (with a relative margin far above the
+        // rounding of positions and cell indices; edge cells are open
+        // outward, where clamped particles sit) holds no neighbour and is
+        // skipped.  Candidates are read four at a time (indices clamped to
+        // the cell, no branches around the loads); a hit is stored through
+        // a running pointer.  The capacity is checked per group of four:
+        // the list has four guard rows past its last one for the overshoot.
+        uint32_t *wp = row;
+        const char *pcb = (const char *)a.pos_cell;
+        const double hlim = g.hsq * (1.0 + 1e-9);
+        for (int k = cc[0] - 1; k <= cc[0] + 1 && !err; ++k) {
+            if (k < 0 || k >= g.nc[0]) continue;
+            double bx = box_gap(xi[0], k, g.bmin[0], g.cs[0], g.nc[0]);
+            for (int l = cc[1] - 1; l <= cc[1] + 1 && !err; ++l) {
+                if (l < 0 || l >= g.nc[1]) continue;
+                double by = box_gap(xi[1], l, g.bmin[1], g.cs[1], g.nc[1]);
+                for (int m = (D == 3 ? cc[2] - 1 : 0); m <= (D == 3 ? cc[2] + 1 : 0) && !err; ++m) {
+                    if (D == 3 && (m < 0 || m >= g.nc[2])) continue;
+                    double bz = D == 3 ? box_gap(xi[2], m, g.bmin[2], g.cs[2], g.nc[2]) : 0.0;
+                    if (bx * bx + by * by + bz * bz > hlim) continue;
+                    uint32_t c = cell_key<D>(g, k, l, m);
+                    const int q1 = a.cend[c];
+                    for (int q = a.cstart[c]; q < q1; q += 4) {
+                        double4 pj[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        pj[u] = q + u < q1 ? a.pos_cell[q + u] : make_double4(0.0, 0.0, 0.0, 0.0);
+                        for (int u = 0; u < 4; ++u) {
+                            // 32-bit byte offset from the array base (N < 2^27)
+                            const uint32_t off = (uint32_t)(q + u < q1 ? q + u : q1 - 1) << 5;
+                            pj[u] = *(const double4 *)(pcb + off);
+                        }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (q + u >= q1 || err) continue;
-                        const uint32_t jo = (uint32_t)pj[u].w;
-                        if (jo == io) continue;
-                        double xj[3] = {pj[u].x, pj[u].y, pj[u].z};
-                        double d = dist2<D>(xj, xi);
-                        if (d < g.hsq) {
-                            if (cnt >= a.maxr) { err = 1; continue; }
-                            row[(size_t)cnt * N] =
-                                (co ? jo : (uint32_t)(q + u)) | (kCodeReal << kIdxBits);
-                            ++cnt;
+                        for (int u = 0; u < 4; ++u) {
+                            const int qq = q + u;
+                            double xj[3] = {pj[u].x, pj[u].y, pj[u].z};
+                            double d = dist2<D>(xj, xi);
+                            bool other;
+                            uint32_t val;
+                            if constexpr (kCaller) {
+                                const uint32_t jo = (uint32_t)pj[u].w;
+                                other = jo != io;
+                                val = jo;
+                            } else {
+                                other = qq != t;
+                                val = (uint32_t)qq;
+                            }
+                            if (qq < q1 && other && d < g.hsq) {
+                                *wp = val | (kCodeReal << kIdxBits);
+                                wp += N;
+                                ++cnt;
+                            }
+                        }
+                        if (cnt > a.maxr) {
+                            err = 1;
+                            break;
                         }
                     }
                 }
             }
         }
-    }
-    a.noi[t] = cnt;
-    if (err) {  // rows stay valid: the entries written so far, no images
-        a.nog[t] = 0;
-        atomicOr(a.status, 1 << 1);
-        return;
-    }
-    int gcnt = 0;
-    // images can only come from virtual cells within W of the particle's
-    // cell; a particle with no such cell wrapping skips the image search
-    const int W = a.img_window;
-    bool near_edge = false;
-    for (int ax = 0; ax < D; ++ax) near_edge |= cc[ax] - W < 0 || cc[ax] + W >= g.nc[ax];
-    if (g.periodic && near_edge) {
-        uint32_t *grow = row + (size_t)cnt * N;
-        for (int vz = (D == 3 ? cc[2] - W : 0); vz <= (D == 3 ? cc[2] + W : 0) && !err; ++vz)
-            for (int vy = cc[1] - W; vy <= cc[1] + W && !err; ++vy)
-                for (int vx = cc[0] - W; vx <= cc[0] + W && !err; ++vx) {
-                    int v[3] = {vx, vy, vz};
-                    int s[3] = {0, 0, 0}, r[3] = {0, 0, 0};
-                    bool wrapped = false;
-                    for (int ax = 0; ax < D; ++ax) {
-                        s[ax] = v[ax] < 0 ? -1 : (v[ax] >= g.nc[ax] ? 1 : 0);
-                        r[ax] = v[ax] - s[ax] * g.nc[ax];
-                        wrapped |= s[ax] != 0;
-                    }
-                    if (!wrapped) continue;
-                    uint32_t c = cell_key<D>(g, r[0], r[1], r[2]);
-                    uint32_t code = (uint32_t)((s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1));
-                    int q1 = a.cend[c];
-                    for (int q = a.cstart[c]; q < q1; ++q) {
-                        double4 pj = a.pos_cell[q];
-                        double xj[3] = {pj.x, pj.y, pj.z};
-                        bool exists = true;
+        if (err) {  // rows stay valid: the first maxr entries, no images
+            cnt = a.maxr;
+            atomicOr(a.status, 1 << 1);
+        }
+        a.noi[t] = cnt;
+        // images can only come from virtual cells within W of the particle's
+        // cell; a particle with no such cell wrapping skips the image search
+        const int W = a.img_window;
+        bool near_edge = false;
+        for (int ax = 0; ax < D; ++ax) near_edge |= cc[ax] - W < 0 || cc[ax] + W >= g.nc[ax];
+        if (!err && g.periodic && near_edge) {
+            uint32_t *grow = row + (size_t)cnt * N;
+            for (int vz = (D == 3 ? cc[2] - W : 0); vz <= (D == 3 ? cc[2] + W : 0) && !err; ++vz)
+                for (int vy = cc[1] - W; vy <= cc[1] + W && !err; ++vy)
+                    for (int vx = cc[0] - W; vx <= cc[0] + W && !err; ++vx) {
+                        int v[3] = {vx, vy, vz};
+                        int s[3] = {0, 0, 0}, r[3] = {0, 0, 0};
+                        bool wrapped = false;
                         for (int ax = 0; ax < D; ++ax) {
-                            if (s[ax] == 0) continue;
-                            bool low = xj[ax] <= g.bmin[ax] + g.h;
-                            bool high = g.bmax[ax] - g.h < xj[ax];
-                            exists &= s[ax] > 0 ? low : (!low && high);
-                            xj[ax] = image_coord(xj[ax], s[ax], g.bmin[ax], g.bmax[ax]);
+                            s[ax] = v[ax] < 0 ? -1 : (v[ax] >= g.nc[ax] ? 1 : 0);
+                            r[ax] = v[ax] - s[ax] * g.nc[ax];
+                            wrapped |= s[ax] != 0;
                         }
-                        if (!exists) continue;
-                        double d = dist2<D>(xj, xi);
-                        if (!(d < g.hsq)) continue;
-                        if (gcnt >= a.maxg) { err = 1; break; }
-                        const uint32_t jo = (uint32_t)pj.w;
-                        uint64_t ke = ghost_key(jo, code);
-                        int p = gcnt;
-                        while (p > 0) {
-                            uint32_t prev = grow[(size_t)(p - 1) * N];
-                            uint32_t pp = (uint32_t)a.pos_w[nb_index(prev)].w;
-                            if (ghost_key(pp, nb_code(prev)) <= ke) break;
-                            grow[(size_t)p * N] = prev;
-                            --p;
+                        if (!wrapped) continue;
+                        uint32_t c = cell_key<D>(g, r[0], r[1], r[2]);
+                        uint32_t code = (uint32_t)((s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1));
+                        int q1 = a.cend[c];
+                        for (int q = a.cstart[c]; q < q1; ++q) {
+                            double4 pj = a.pos_cell[q];
+                            double xj[3] = {pj.x, pj.y, pj.z};
+                            bool exists = true;
+                            for (int ax = 0; ax < D; ++ax) {
+                                if (s[ax] == 0) continue;
+                                bool low = xj[ax] <= g.bmin[ax] + g.h;
+                                bool high = g.bmax[ax] - g.h < xj[ax];
+                                exists &= s[ax] > 0 ? low : (!low && high);
+                                xj[ax] = image_coord(xj[ax], s[ax], g.bmin[ax], g.bmax[ax]);
+                            }
+                            if (!exists) continue;
+                            double d = dist2<D>(xj, xi);
+                            if (!(d < g.hsq)) continue;
+                            if (gcnt >= a.maxg) { err = 1; break; }
+                            const uint32_t jo = (uint32_t)pj.w;
+                            uint64_t ke = ghost_key(jo, code);
+                            int p = gcnt;
+                            while (p > 0) {
+                                uint32_t prev = grow[(size_t)(p - 1) * N];
+                                uint32_t pp = (uint32_t)a.pos_w[nb_index(prev)].w;
+                                if (ghost_key(pp, nb_code(prev)) <= ke) break;
+                                grow[(size_t)p * N] = prev;
+                                --p;
+                            }
+                            grow[(size_t)p * N] = (kCaller ? jo : (uint32_t)q) | (code << kIdxBits);
+                            ++gcnt;
                         }
-                        grow[(size_t)p * N] = (co ? jo : (uint32_t)q) | (code << kIdxBits);
-                        ++gcnt;
                     }
-                }
-        if (err) atomicOr(a.status, 1 << 3);
+            if (err) atomicOr(a.status, 1 << 3);
+        }
+        a.nog[t] = gcnt;
     }
-    a.nog[t] = gcnt;
-    atomicMax(&a.max_counts[0], cnt);
-    atomicMax(&a.max_counts[1], gcnt);
+    // largest counts: a wavefront maximum, one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) {
+        int o = __shfl_xor(cnt, off, 64);
+        cnt = o > cnt ? o : cnt;
+        o = __shfl_xor(gcnt, off, 64);
+        gcnt = o > gcnt ? o : gcnt;
+    }
+    if (__lane_id() == 0) {
+        atomicMax(&a.max_counts[0], cnt);
+        atomicMax(&a.max_counts[1], gcnt);
+    }
 }
 
 template __global__ void k_assign_cells<2>(Geom, const double *, const double *, const double *,
@@ -545,8 +574,10 @@ __global__ void k_pack_pos(const double *__restrict__ x0, const double *__restri
     pos[i] = make_double4(x0[i], x1[i], D == 3 ? x2[i] : 0.0, (double)i);
 }
 
-template __global__ void k_neighbours<2>(NbrArgs);
-template __global__ void k_neighbours<3>(NbrArgs);
+template __global__ void k_neighbours<2, false>(NbrArgs);
+template __global__ void k_neighbours<3, false>(NbrArgs);
+template __global__ void k_neighbours<2, true>(NbrArgs);
+template __global__ void k_neighbours<3, true>(NbrArgs);
 
 void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
                   const double *x2, int N, double4 *pos, uint32_t *key, uint32_t *idx,
@@ -576,10 +607,13 @@ void launch_pack_pos(const double *x0, const double *x1, const double *x2, int D
 
 void launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {
     int nb = (a.N + 255) / 256;
-    if (D == 2)
-        hipLaunchKernelGGL(k_neighbours<2>, dim3(nb), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_neighbours<3>, dim3(nb), dim3(256), 0, s, a);
+    if (a.caller_order) {
+        if (D == 2) hipLaunchKernelGGL((k_neighbours<2, true>), dim3(nb), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_neighbours<3, true>), dim3(nb), dim3(256), 0, s, a);
+    } else {
+        if (D == 2) hipLaunchKernelGGL((k_neighbours<2, false>), dim3(nb), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_neighbours<3, false>), dim3(nb), dim3(256), 0, s, a);
+    }
 }
 
 }  // namespace mlh

@@ -20,8 +20,9 @@ def kh_random(n=48, **kw):
 
 
 def blast3d(n=16, **kw):
-    p = default_params(dim=3, kernel_size=ics.blast_kernel_size(n), max_interactions=128,
-                       max_ghost_interactions=128, **kw)
+    kw.setdefault("max_interactions", 128)
+    kw.setdefault("max_ghost_interactions", 128)
+    p = default_params(dim=3, kernel_size=ics.blast_kernel_size(n), **kw)
     return p, ics.blast_3d(n)
 
 
@@ -43,8 +44,9 @@ def jitter3d(n=12, seed=11, **kw):
     m = rho / N
     P = 1.0 + 0.1 * np.cos(2 * np.pi * Z)
     u = P / ((5.0 / 3.0 - 1.0) * rho)
-    p = default_params(dim=3, kernel_size=2.4 / n, max_interactions=128,
-                       max_ghost_interactions=128, **kw)
+    kw.setdefault("max_interactions", 128)
+    kw.setdefault("max_ghost_interactions", 128)
+    p = default_params(dim=3, kernel_size=2.4 / n, **kw)
     return p, dict(x=X, y=Y, z=Z, vx=vx, vy=vy, vz=vz, m=m, u=u)
 
 

@@ -225,3 +225,30 @@ def test_multistep_open_domain(name, hip, oracle_mod):
     assert abs(c1[1] - c0[1]) <= 1e-13 * abs(c0[1])
     assert abs(c1[2] - c0[2]) <= 1e-12 * abs(c0[2])
     sim.close()
+
+
+@pytest.mark.parametrize("case,kw,code", [
+    ("kh100", dict(max_interactions=40), 1),
+    ("jitter12", dict(max_interactions=30), 1),
+    # open domain without image rows: the search's overshoot lands in the
+    # list's guard rows
+    ("khopen40", dict(max_interactions=40, max_ghost_interactions=0), 1),
+    ("kh100", dict(max_ghost_interactions=5), 3),
+])
+def test_neighbour_overflow_status(case, kw, code, hip, oracle_mod):
+    """MAX_NUM_INTERACTIONS / MAX_NUM_GHOST_INTERACTIONS exceeded: the
+    reference's exit codes 1 / 3 (Particles.cpp:348-351, 2249-2252), on the
+    GPU as in the oracle; the largest real count reported is the limit."""
+    p, s = CASES[case](**kw)
+    print(f"\n[overflow {case} {kw}]", flush=True)
+    sim = hip.Simulation(p, s)
+    with pytest.raises(hip.MLHError) as e:
+        sim.neighbours()
+    assert e.value.status == code
+    if code == 1:
+        assert sim.diag()["max_neighbours"] == p.max_interactions
+    sim.close()
+    orc = oracle_mod.Oracle(p, s)
+    with pytest.raises(oracle_mod.OracleError) as e2:
+        orc.neighbours()
+    assert e2.value.status == code
