@@ -27,6 +27,10 @@ FLAGS = [
     # FP32 divide / sqrt as the hardware reciprocal / square root (~1 ulp);
     # FP64 is unaffected (FP32 arithmetic only runs under stated tolerances)
     "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+    # FP32 denormals flushed to zero: the FP32 divide and square root then
+    # need no range scaling around the hardware instructions.  FP64 keeps
+    # IEEE denormals (its code is unchanged by this flag).
+    "-fgpu-flush-denormals-to-zero",
     "-Wno-unused-result",
 ]
 
