@@ -193,6 +193,112 @@ def sedov_run(args, state, base, log):
                       "t_end": t_end, "xi0": sol.xi0, "modes": res}))
 
 
+def c5_run(args, log):
+    """The decaying turbulent box: glass relaxation and Mach-1 solenoidal
+    velocity (meshlesshydro_amd.turbulence), FP64 plain with its own CFL
+    steps (its first 5 steps record faces for the initial training), then
+    FP32 with the learned seed replaying that dt sequence and retraining
+    online every args.retrain_every steps.  Rates count the step loop;
+    `with_retraining` also counts the retraining.  Prints one JSON line."""
+    from meshlesshydro_amd import learned as L
+    from meshlesshydro_amd import lib, turbulence
+    from meshlesshydro_amd._abi import default_params, MLH_FP32, MLH_FP64, MLH_SEED_LEARNED
+
+    n, K = args.c5_n, args.steps
+    N = n ** 3
+    t0 = time.perf_counter()
+    st, info = turbulence.turbulent_box(n)
+    t_ic = time.perf_counter() - t0
+    log(f"[c5] {n}^3 initial state in {t_ic:.1f} s, density spread {info['density_spread']}")
+    # the reference's list limits (parameter_long_run.h: 400 / 300): Mach-1
+    # compressions raise the 64 mean neighbours several-fold
+    base = dict(dim=3, kernel_size=info["kernel_size"], max_interactions=400,
+                max_ghost_interactions=300, debug_level=0)
+
+    def fields(state):
+        s = lib.Simulation(default_params(precision=MLH_FP64, **base), state)
+        s.neighbours()
+        s.density()
+        r, P, _ = s.get_density()
+        s.close()
+        return {"rho": r, "P": P, "v": np.sqrt(state["vx"] ** 2 + state["vy"] ** 2 + state["vz"] ** 2)}
+
+    p64 = default_params(precision=MLH_FP64, **base)
+    sim = lib.Simulation(p64, st)
+    c0 = sim.conserved()
+    sim.set_face_collection(1 << 21, 8)
+    ta = time.perf_counter()
+    dts_a = sim.run(5)
+    faces = sim.collected_faces()
+    sim.set_face_collection(0, 0)
+    dts_b = sim.run(K - 5)
+    t64 = time.perf_counter() - ta
+    tim64 = sim.timings()
+    c1 = sim.conserved()
+    ref = fields(sim.state())
+    weights, train0 = L.train_seed_from_faces(sim, faces, p64.gamma, n_epochs=4, log=log)
+    sim.close()
+    dts = np.concatenate([dts_a, dts_b])
+
+    # FP32, standard seed, same dt sequence
+    p32s = default_params(precision=MLH_FP32, **base)
+    s32s = lib.Simulation(p32s, st)
+    e0 = s32s.conserved()
+    tc = time.perf_counter()
+    s32s.run(K, dts_in=dts)
+    t32s = time.perf_counter() - tc
+    std = {"particle_steps_per_s": N * K / t32s,
+           "kernel_ms": [round(float(x), 4) for x in s32s.timings()]}
+    dstd = s32s.diag()
+    std["newton_iters_per_face"] = dstd["newton_iterations"] / max(1, dstd["faces_solved"])
+    std["drift"] = {"mass": abs(s32s.conserved()[1] - e0[1]) / abs(e0[1]),
+                    "energy": abs(s32s.conserved()[2] - e0[2]) / abs(e0[2])}
+    outs = fields(s32s.state())
+    s32s.close()
+    std["errors_vs_fp64"] = {"L1_rel_" + k: float(np.sum(np.abs(outs[k] - b)) / np.sum(np.abs(b)))
+                             for k, b in ref.items()}
+
+    p32 = default_params(precision=MLH_FP32, **base)
+    p32.riemann_seed = MLH_SEED_LEARNED
+    s32 = lib.Simulation(p32, st)
+    s32.mlp_set_params(weights)
+    d0 = s32.conserved()
+    tb = time.perf_counter()
+    _, infos = turbulence.run_online(s32, K, retrain_every=args.retrain_every, stride=8, epochs=2,
+                                     dts=dts, log=log)
+    t32 = time.perf_counter() - tb
+    t_train = sum(i.get("seconds", 0.0) for i in infos)
+    d1 = s32.conserved()
+    diag = s32.diag()
+    tim32 = s32.timings()
+    out = fields(s32.state())
+    s32.close()
+    err = {}
+    for k, b in ref.items():
+        d = np.abs(out[k] - b)
+        err["L1_rel_" + k] = float(np.sum(d) / np.sum(np.abs(b)))
+        err["Linf_rel_" + k] = float(np.max(d) / np.max(np.abs(b)))
+    drift = lambda a, b: {"mass": abs(b[1] - a[1]) / abs(a[1]), "energy": abs(b[2] - a[2]) / abs(a[2])}
+    print(json.dumps({
+        "workload": f"decaying turbulent box {n}^3 (C5): glass, Mach 1 solenoidal P(k)~k^-4, k in [2,8]",
+        "steps": K, "particles": N, "retrain_every": args.retrain_every,
+        "fp32_learned_online": {"particle_steps_per_s": N * K / (t32 - t_train),
+                                "with_retraining": N * K / t32, "retraining_s": t_train,
+                                "kernel_ms_last_segment": [round(float(x), 4) for x in tim32],
+                                "newton_iters_per_face": diag["newton_iterations"] / max(1, diag["faces_solved"]),
+                                "seeded_rate": diag["learned_seeded"] / max(1, diag["faces_solved"]),
+                                "drift": drift(d0, d1)},
+        "fp64_plain": {"particle_steps_per_s": N * K / t64, "kernel_ms": [round(float(x), 4) for x in tim64],
+                       "drift": drift(c0, c1)},
+        "fp32_standard": std,
+        "speedup_vs_fp64": t64 / (t32 - t_train),
+        "errors_vs_fp64": err,
+        "initial_training": train0,
+        "retraining": infos,
+        "initial_state_s": t_ic, "density_spread": info["density_spread"],
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +320,12 @@ def main():
                          "final density and pressure profiles with the analytic solution")
     ap.add_argument("--sedov-radius", type=float, default=0.07)
     ap.add_argument("--sedov-max-steps", type=int, default=20000)
+    ap.add_argument("--c5", action="store_true",
+                    help="the decaying turbulent box (glass, Mach-1 solenoidal velocity): FP64 plain "
+                         "against FP32 + learned seed retrained online every --retrain-every steps, "
+                         "--steps steps each, at --c5-n^3 particles")
+    ap.add_argument("--c5-n", type=int, default=256)
+    ap.add_argument("--retrain-every", type=int, default=50)
     ap.add_argument("--order", choices=["auto", "cell", "caller"], default="auto",
                     help="work order of the kernels (results are identical)")
     ap.add_argument("--shuffle", action="store_true",
@@ -237,6 +349,10 @@ def main():
     from meshlesshydro_amd import ics, lib
     from meshlesshydro_amd import learned as L
     from meshlesshydro_amd._abi import default_params, MLH_FP32, MLH_FP64, MLH_SEED_LEARNED
+
+    if args.c5:
+        c5_run(args, lambda *a: print(*a, file=sys.stderr, flush=True))
+        return
 
     n = args.n
     N = n ** 3

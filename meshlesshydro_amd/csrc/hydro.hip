@@ -807,6 +807,31 @@ struct FluxSums {
     double m, e, v[3];
 };
 
+// training data: the face states of the faces the solver needed Newton
+// iterations for (left / right rho, normal velocity, P), for the lanes with
+// `want`.  Called by the active lanes of a wavefront together: one atomic
+// per wavefront for the slots, nothing once the record is full.
+template <int D, typename T>
+__device__ inline void collect_faces(const HydroArgs &a, bool want, const FacePrep<D, T> &fp) {
+    const unsigned long long b = __ballot(want);
+    if (!b) return;
+    const unsigned long long cap = (unsigned long long)a.collect_cap;
+    if (__atomic_load_n(a.collect_n, __ATOMIC_RELAXED) >= cap) return;
+    const int lane = lane_id();
+    const int leader = __ffsll((long long)b) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(a.collect_n, (unsigned long long)__popcll(b));
+    base = __shfl(base, leader, 64);
+    if (want) {
+        const unsigned long long slot = base + __popcll(b & ((1ull << lane) - 1ull));
+        if (slot < cap) {
+            double *c = a.collect + slot * 6;
+            c[0] = fp.WR[0]; c[1] = fp.ui; c[2] = fp.WR[1];
+            c[3] = fp.WL[0]; c[4] = fp.uj; c[5] = fp.WL[1];
+        }
+    }
+}
+
 // solve the face, add its flux (with the sign of this side) to the sums;
 // face collection for the training data
 template <int D, typename T>
@@ -815,15 +840,7 @@ __device__ inline void face_accumulate(const HydroArgs &a, const FacePrep<D, T> 
                                        bool collect, FluxSums &acc, FaceStats &fs) {
     T F[D + 2];
     face_finish<D, T>(a, fp, seed, F, fs, sizeof(T) == 4 ? &mp : nullptr);
-    // training data: faces the exact solver needed >= 2 iterations for
-    if (collect && self_lo && fs.st.iters - it0 >= 2) {
-        unsigned long long slot = atomicAdd(a.collect_n, 1ull);
-        if (slot < (unsigned long long)a.collect_cap) {
-            double *c = a.collect + slot * 6;
-            c[0] = fp.WR[0]; c[1] = fp.ui; c[2] = fp.WR[1];
-            c[3] = fp.WL[0]; c[4] = fp.uj; c[5] = fp.WL[1];
-        }
-    }
+    if (a.collect) collect_faces<D, T>(a, collect && self_lo && fs.st.iters - it0 >= 2, fp);
     double sgn = self_lo ? 1. : -1.;
     acc.m += sgn * (double)F[0];
     for (int ax = 0; ax < D; ++ax) acc.v[ax] += sgn * (double)F[2 + ax];
@@ -920,7 +937,11 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     // FP32 learned: faces whose probe fails are deferred to k_faces_deferred,
     // which seeds them with the network; the FP64 learned kernel runs the
     // network inside the face loop, over the wavefront's longest row
+#ifdef MLH_FP32_INLOOP
+    constexpr bool kInLoopNet = LEARNED;
+#else
     constexpr bool kInLoopNet = LEARNED && !kFp32;
+#endif
     __shared__ uint16_t lds[kInLoopNet ? 4 : 1][kInLoopNet ? 64 * 16 : 1];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     const bool live = t < a.N;
@@ -954,7 +975,9 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
             // to O(jump^2) there): |dp| > k (pL + pR) or
             // du^2 rhoR rhoL > 2 k^2 gamma (pR rhoL + pL rhoR)
             bool strong = false;
-            if (act) {
+            if constexpr (kFp32) {
+                strong = act && mp.valid && !mp.accepted;
+            } else if (act) {
                 T dp = fp.WR[1] - fp.WL[1], du = fp.uj - fp.ui;
                 T psum = fp.WR[1] + fp.WL[1];
                 T rr = fp.WR[0] * fp.WL[0];
@@ -965,8 +988,10 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
             if (__ballot(strong)) seed = face_net_seed<D, T>(a, strong, fp, lds[threadIdx.x >> 6], fs);
         }
         if (act) {
-            if (LEARNED && kFp32 && mp.valid && !mp.accepted && n < 128) {
-                defer[n >> 6] |= 1ull << (n & 63);  // Newton has to iterate
+            if (LEARNED && kFp32 && !kInLoopNet && mp.valid && !mp.accepted &&
+                mp.chg > kSeedGate && n < 128) {
+                // the standard guess was far off: Newton from the network's seed
+                defer[n >> 6] |= 1ull << (n & 63);
                 continue;
             }
             face_accumulate<D, T>(a, fp, mp, seed, self_lo, it0, collect, acc, fs);
@@ -1086,7 +1111,13 @@ __global__ __launch_bounds__(256) void k_faces_deferred(HydroArgs a) {
             float seed = face_net_seed<D, float>(a, act && mp.valid && !mp.accepted, fp, lds[w], fs);
             if (act) {
                 float F[NF];
+                const int it0 = fs.st.iters;
                 face_finish<D, float>(a, fp, seed, F, fs, &mp);
+                // training data of the owner's particle (every collect_stride-th)
+                if (a.collect)
+                    collect_faces<D, float>(a, self_lo && fs.st.iters - it0 >= 2 &&
+                                                   own_i[w][queue[w][e] >> 8] % a.collect_stride == 0,
+                                            fp);
                 const float sg = self_lo ? 1.f : -1.f;
                 for (int c2 = 0; c2 < NF; ++c2) res[w][lane * NF + c2] = sg * F[c2];
             }

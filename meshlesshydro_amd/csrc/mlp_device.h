@@ -132,6 +132,11 @@ __device__ inline float mlp_features_fast(float gamma, float dl, float ul, float
 
 // relative pressure / velocity jump above which a face uses the learned seed
 constexpr float kSeedJump = 1e-2f;
+// FP32 mode: the network seeds a face only when the FP64 Newton step from
+// the standard guess changed p by more than this fraction.  Below it the
+// step's result (quadratically closer than the guess) is the better start
+// for the remaining Newton iterations than the network's prediction.
+constexpr float kSeedGate = 5e-2f;
 
 __device__ inline float mlp_forward_scalar(const float *__restrict__ P, const float *x) {
     float h1[kMlpH];

@@ -17,6 +17,7 @@
 //                an FP64 acceptance step, solve_fp64 as the fallback.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <float.h>
 #include <math.h>
 
 namespace mlh {
@@ -360,6 +361,7 @@ __device__ inline double star_pressure_fp64(const GasConst<double> &c, double dl
 struct MixedProbe {
     float p;          // FP32 iterate to continue from
     float cl, cr;     // FP32 sound speeds
+    float chg;        // relative change of the FP64 step from the guess
     double pn, um;    // FP64 root and contact velocity once accepted
     bool valid;       // positive densities and no vacuum generated
     bool accepted;    // an FP64 step passed the change test
@@ -385,7 +387,8 @@ __device__ inline MixedProbe mixed_probe(const GasConst<float> &cf, const GasCon
     double dp = (fl + fr + dud) * (double)(1.f / (fld + frd));
     double pn = pd - dp;
     st.iters += 1;
-    m.accepted = 2.f * __builtin_fabsf((float)dp / (float)(pn + pd)) <= (float)tol && pn > 0.0;
+    m.chg = pn > 0.0 ? 2.f * __builtin_fabsf((float)dp / (float)(pn + pd)) : FLT_MAX;
+    m.accepted = m.chg <= (float)tol && pn > 0.0;
     m.pn = pn;
     m.um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (double)(frd - fld) * dp);
     m.p = pn > 0.0 ? (float)pn : p;

@@ -67,15 +67,17 @@ def newton_stats(ctx_sim, gamma, faces, precision=MLH_FP32):
     return out
 
 
-def train_seed_from_faces(sim, faces, gamma, n_epochs=4, batch=4096, lr=2e-3, seed=1, log=None):
-    """Train the seed network of Simulation `sim` on recorded faces; returns
+def train_seed_from_faces(sim, faces, gamma, n_epochs=4, batch=4096, lr=2e-3, seed=1, log=None,
+                          init=None):
+    """Train the seed network of Simulation `sim` on recorded faces, from
+    `init` (weights to continue from) or from init_params(seed); returns
     (params, info)."""
     feats, target = dataset(faces, gamma)
     n = len(target)
     if n < 256:
         raise ValueError(f"too few training faces ({n})")
     nval = max(n // 10, 64)
-    sim.mlp_set_params(init_params(seed))
+    sim.mlp_set_params(init_params(seed) if init is None else np.asarray(init, np.float32))
     loss0, _ = sim.mlp_loss_grad(feats[:min(n, 65536)], target[:min(n, 65536)])
     loss = sim.mlp_train(feats[nval:], target[nval:], epochs=n_epochs, batch=min(batch, n - nval), lr=lr, seed=seed)
     val = sim.mlp_infer(feats[:nval], bf16=True)
