@@ -193,6 +193,87 @@ def sedov_run(args, state, base, log):
                       "t_end": t_end, "xi0": sol.xi0, "modes": res}))
 
 
+def shear_run(args, log):
+    """The 2D shear layer (Kelvin-Helmholtz, reference flags).
+
+    --c1: 100x100 lattice, h = 4 dx, 100 FP64 steps on the GPU and on the CPU
+    oracle (both with their own CFL steps): largest state difference
+    relative to each field's scale, dt sequence difference, and wall times.
+    --c2: 1000x1000, h = 4 dx, 200 steps: FP64 plain against FP32 with the
+    standard seed replaying the FP64 dt sequence; rates, L1/Linf errors of
+    rho, P, |v| and the conservation drift.  Prints one JSON line."""
+    from meshlesshydro_amd import ics, lib
+    from meshlesshydro_amd._abi import default_params, MLH_FP32, MLH_FP64
+
+    n, K = (100, 100) if args.c1 else (1000, 200)
+    st = ics.kelvin_helmholtz(n)
+    p = default_params(kernel_size=4.0 / n, debug_level=0)
+    N = n * n
+
+    def fields(state):
+        s = lib.Simulation(default_params(kernel_size=4.0 / n, debug_level=0), state)
+        s.neighbours()
+        s.density()
+        r, P, _ = s.get_density()
+        s.close()
+        return {"rho": r, "P": P, "v": np.sqrt(state["vx"] ** 2 + state["vy"] ** 2)}
+
+    sim = lib.Simulation(p, st)
+    c0 = sim.conserved()
+    sim.run(1)  # first launch of each kernel outside the timed region
+    sim.close()
+    sim = lib.Simulation(p, st)
+    t0 = time.perf_counter()
+    dts = sim.run(K)
+    t64 = time.perf_counter() - t0
+    c1 = sim.conserved()
+    g64 = sim.state()
+    sim.close()
+    drift = lambda a, b: {"mass": abs(b[1] - a[1]) / abs(a[1]), "energy": abs(b[2] - a[2]) / abs(a[2]),
+                          "momentum": float(np.hypot(b[3] - a[3], b[4] - a[4]))}
+    if args.c1:
+        from oracle import oracle as O
+        o = O.Oracle(p, st, threads=args.cpu_threads)
+        t0 = time.perf_counter()
+        odt = np.array([o.step() for _ in range(K)])
+        tc = time.perf_counter() - t0
+        os_ = o.state()
+        diff = {k: float(np.max(np.abs(g64[k] - os_[k])) / max(np.max(np.abs(os_[k])), 1e-300)) for k in g64}
+        print(json.dumps({
+            "workload": "shear layer 100x100 (C1), reference flags, 100 FP64 steps",
+            "gpu_particle_steps_per_s": N * K / t64, "cpu_oracle_particle_steps_per_s": N * K / tc,
+            "cpu_threads": args.cpu_threads, "gpu_over_cpu": tc / t64,
+            "max_rel_state_difference": diff,
+            "max_rel_dt_difference": float(np.max(np.abs(dts - odt) / odt)),
+            "t_end": float(np.sum(dts)), "drift": drift(c0, c1)}))
+        return
+    ref = fields(g64)
+    p32 = default_params(kernel_size=4.0 / n, debug_level=0, precision=MLH_FP32)
+    s32 = lib.Simulation(p32, st)
+    s32.run(1, dts_in=dts[:1])
+    s32.close()
+    s32 = lib.Simulation(p32, st)
+    e0 = s32.conserved()
+    t0 = time.perf_counter()
+    s32.run(K, dts_in=dts)
+    t32 = time.perf_counter() - t0
+    e1 = s32.conserved()
+    tim32 = s32.timings()
+    out = fields(s32.state())
+    s32.close()
+    err = {}
+    for k, b in ref.items():
+        d = np.abs(out[k] - b)
+        err["L1_rel_" + k] = float(np.sum(d) / np.sum(np.abs(b)))
+        err["Linf_rel_" + k] = float(np.max(d) / np.max(np.abs(b)))
+    print(json.dumps({
+        "workload": "shear layer 1000x1000 (C2), reference flags, 200 steps",
+        "fp64_plain": {"particle_steps_per_s": N * K / t64, "drift": drift(c0, c1)},
+        "fp32_standard": {"particle_steps_per_s": N * K / t32, "kernel_ms": [round(float(x), 4) for x in tim32],
+                          "drift": drift(e0, e1)},
+        "speedup_vs_fp64": t64 / t32, "errors_vs_fp64": err}))
+
+
 def c5_run(args, log):
     """The decaying turbulent box: glass relaxation and Mach-1 solenoidal
     velocity (meshlesshydro_amd.turbulence), FP64 plain with its own CFL
@@ -325,6 +406,11 @@ def main():
                          "against FP32 + learned seed retrained online every --retrain-every steps, "
                          "--steps steps each, at --c5-n^3 particles")
     ap.add_argument("--c5-n", type=int, default=256)
+    ap.add_argument("--c1", action="store_true",
+                    help="the reference's shear layer (100x100, FP64, 100 steps) on the GPU and on "
+                         "the CPU oracle: state difference and both times")
+    ap.add_argument("--c2", action="store_true",
+                    help="the shear layer at 1000x1000: FP64 against FP32 (standard seed), 200 steps")
     ap.add_argument("--retrain-every", type=int, default=50)
     ap.add_argument("--order", choices=["auto", "cell", "caller"], default="auto",
                     help="work order of the kernels (results are identical)")
@@ -352,6 +438,9 @@ def main():
 
     if args.c5:
         c5_run(args, lambda *a: print(*a, file=sys.stderr, flush=True))
+        return
+    if args.c1 or args.c2:
+        shear_run(args, lambda *a: print(*a, file=sys.stderr, flush=True))
         return
 
     n = args.n

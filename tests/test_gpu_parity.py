@@ -9,7 +9,7 @@ the Riemann solver between the GPU's math library and glibc).
 import numpy as np
 import pytest
 
-from cases import CASES, relerr
+from cases import CASES, kh_lattice, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -252,3 +252,20 @@ def test_neighbour_overflow_status(case, kw, code, hip, oracle_mod):
     with pytest.raises(oracle_mod.OracleError) as e2:
         orc.neighbours()
     assert e2.value.status == code
+
+
+def test_reference_case_hundred_steps(hip, oracle_mod):
+    """The reference's runnable configuration (100 x 100 shear layer,
+    parameter_long_run.h flags) for its full 100 steps, each side with its
+    own CFL steps: the dt sequences and the final state agree with the
+    oracle to 1e-10 (measured: 1e-14 and 2e-12)."""
+    p, s = kh_lattice(100)
+    sim = hip.Simulation(p, s)
+    orc = oracle_mod.Oracle(p, s)
+    dts = sim.run(100)
+    odt = [orc.step() for _ in range(100)]
+    assert np.max(np.abs(dts - odt) / np.asarray(odt)) < 1e-10
+    gs, os_ = sim.state(), orc.state()
+    for k in gs:
+        assert relerr(gs[k], os_[k]) < 1e-10, k
+    sim.close()
