@@ -193,6 +193,84 @@ def sedov_run(args, state, base, log):
                       "t_end": t_end, "xi0": sol.xi0, "modes": res}))
 
 
+def c3_run(args, log):
+    """Riemann microbenchmark: n = 2^c3_log2n random face states
+    (ics.riemann_states, seed 20261202) resident in HBM; the batched exact
+    solver in FP64 and FP32 with the standard guess; the seed network trained
+    on the first n/4 states (log p*/p_guess targets from the FP64 star
+    pressure, the library's GPU training); FP32 and FP64 with the network's
+    seed.  Reports
+    faces/s from the kernel's HIP-event time, Newton iterations per face
+    (all states and the held-out 3n/4), fallback rates, and the kernel's
+    algorithmic bytes (6 doubles in, 3 doubles + flag + iterations out)
+    against HBM.  Prints one JSON line."""
+    from meshlesshydro_amd import ics, lib
+    from meshlesshydro_amd import learned as L
+    from meshlesshydro_amd._abi import (default_params, MLH_FP32, MLH_FP64, MLH_SEED_LEARNED,
+                                        MLH_SEED_STANDARD)
+    n = 1 << args.c3_log2n
+    ntrain = n // 4
+    gamma = 5.0 / 3.0
+    t0 = time.perf_counter()
+    st = ics.riemann_states(n)
+    log(f"[c3] {n} states in {time.perf_counter() - t0:.1f} s")
+    # device-resident inputs and outputs (the library's own allocations)
+    states = lib.DeviceBuffer((6, n), np.float64)
+    for k in range(6):
+        states.copy_from(st[k], offset_bytes=k * n * 8)
+    star = lib.DeviceBuffer((3, n), np.float64)
+    flag = lib.DeviceBuffer(n, np.int32)
+    iters = lib.DeviceBuffer(n, np.int32)
+    ctx = lib.Simulation(default_params(kernel_size=0.1), {k: np.array([0.5]) for k in ("x", "y", "m", "u")} |
+                         {"vx": np.zeros(1), "vy": np.zeros(1)})
+    bytes_face = 6 * 8 + 3 * 8 + 4 + 4
+    res = {}
+
+    def run(name, prec, seed):
+        lib.riemann_batch_device(gamma, states, star, flag, iters, prec, seed, ctx=ctx.h)  # warm-up
+        fb, ms = lib.riemann_batch_device(gamma, states, star, flag, iters, prec, seed, ctx=ctx.h)
+        it = iters.numpy()
+        r = {"faces_per_s": n / (ms * 1e-3), "kernel_ms": ms,
+             "newton_iters_per_face": float(it.mean()),
+             "newton_iters_per_face_heldout": float(it[ntrain:].mean()),
+             "fallback_rate": fb / n,
+             "hbm_gbs_algorithmic": n * bytes_face / (ms * 1e-3) / 1e9}
+        log(f"[c3] {name}: {r}")
+        res[name] = r
+
+    run("fp64_standard", MLH_FP64, MLH_SEED_STANDARD)
+    run("fp32_standard", MLH_FP32, MLH_SEED_STANDARD)
+    # training set: the first n/4 states, targets log(p*/p_guess) from the
+    # FP64 star-region pressure (learned.dataset; trivially converged faces
+    # subsampled)
+    t0 = time.perf_counter()
+    feats, target = L.dataset(np.stack([a[:ntrain] for a in st], axis=1), gamma)
+    ctx.mlp_set_params(L.init_params(1))
+    loss0, _ = ctx.mlp_loss_grad(feats[:65536], target[:65536])
+    nval = 1 << 16
+    loss = ctx.mlp_train(feats[nval:], target[nval:], epochs=4, batch=4096, lr=2e-3, seed=1)
+    val = ctx.mlp_infer(feats[:nval], bf16=True)
+    train = {"samples": int(len(target) - nval), "seconds": time.perf_counter() - t0,
+             "loss_init": loss0, "loss_final": loss,
+             "val_mse_log_pstar": float(np.mean((val - target[:nval]) ** 2)),
+             "val_mse_standard_guess": float(np.mean(target[:nval] ** 2))}
+    log(f"[c3] training: {train}")
+    run("fp32_learned", MLH_FP32, MLH_SEED_LEARNED)
+    run("fp64_learned", MLH_FP64, MLH_SEED_LEARNED)
+    ctx.close()
+    for b in (states, star, flag, iters):
+        b.free()
+    print(json.dumps({
+        "workload": f"Riemann microbenchmark: 2^{args.c3_log2n} random face states (C3), gamma 5/3",
+        "faces": n, "training": train, "modes": res,
+        "iterations_saved_fp32": res["fp32_standard"]["newton_iters_per_face_heldout"]
+        - res["fp32_learned"]["newton_iters_per_face_heldout"],
+        "iterations_saved_fp64": res["fp64_standard"]["newton_iters_per_face_heldout"]
+        - res["fp64_learned"]["newton_iters_per_face_heldout"],
+        "speedup_fp32_learned_vs_fp64": res["fp32_learned"]["faces_per_s"] / res["fp64_standard"]["faces_per_s"],
+        "speedup_fp64_learned_vs_fp64": res["fp64_learned"]["faces_per_s"] / res["fp64_standard"]["faces_per_s"]}))
+
+
 def shear_run(args, log):
     """The 2D shear layer (Kelvin-Helmholtz, reference flags).
 
@@ -406,6 +484,10 @@ def main():
                          "against FP32 + learned seed retrained online every --retrain-every steps, "
                          "--steps steps each, at --c5-n^3 particles")
     ap.add_argument("--c5-n", type=int, default=256)
+    ap.add_argument("--c3", action="store_true",
+                    help="Riemann microbenchmark: 2^26 random face states resident on the GPU, FP64 and "
+                         "FP32 with the standard guess and with the network trained on 2^24 solver outputs")
+    ap.add_argument("--c3-log2n", type=int, default=26)
     ap.add_argument("--c1", action="store_true",
                     help="the reference's shear layer (100x100, FP64, 100 steps) on the GPU and on "
                          "the CPU oracle: state difference and both times")
@@ -438,6 +520,9 @@ def main():
 
     if args.c5:
         c5_run(args, lambda *a: print(*a, file=sys.stderr, flush=True))
+        return
+    if args.c3:
+        c3_run(args, lambda *a: print(*a, file=sys.stderr, flush=True))
         return
     if args.c1 or args.c2:
         shear_run(args, lambda *a: print(*a, file=sys.stderr, flush=True))

@@ -216,6 +216,33 @@ int mlh_riemann_exact_batch(mlh_ctx *ctx, int64_t n, double gamma,
                             int32_t *flag, int32_t *iters, int32_t precision,
                             int32_t seed, double tol);
 
+/* ---- device buffers ------------------------------------------------------
+ * Memory of the library's own HIP runtime for the *_device entry points.
+ * No reference counterpart: a caller that holds device data in another HIP
+ * runtime (a second libamdhip64 in the process, e.g. a framework's bundled
+ * one) must not pass those pointers here; it copies into these buffers.
+ * kind: 1 host -> device, 2 device -> host, 3 device -> device; the copy is
+ * complete on return. */
+int mlh_device_alloc(int64_t bytes, void **ptr);
+int mlh_device_free(void *ptr);
+int mlh_device_copy(void *dst, const void *src, int64_t bytes, int32_t kind);
+
+/* The same solve on device-resident arrays, for callers whose face states
+ * already live on the GPU (no host copies; the Riemann microbenchmark
+ * measures the kernel this way).  states: device [6][n] = rhoL, uL, PL,
+ * rhoR, uR, PR; star: device [3][n] = rho*, u*, P* sampled at S = 0; flag,
+ * iters: device [n] (may be NULL).  Runs on ctx's stream (default stream
+ * when ctx is NULL) and returns when the kernel has finished.  fallbacks
+ * (host, may be NULL): faces the FP64 solver finished; kernel_ms (host,
+ * may be NULL): the kernel's duration from HIP events.  Same interface
+ * reference as mlh_riemann_exact_batch (RiemannSolver::solve, call site
+ * Riemann.cpp:84,93-94). */
+int mlh_riemann_exact_batch_device(mlh_ctx *ctx, int64_t n, double gamma,
+                                   const double *states, double *star,
+                                   int32_t *flag, int32_t *iters, int32_t precision,
+                                   int32_t seed, double tol, int64_t *fallbacks,
+                                   float *kernel_ms);
+
 /* ---- one face flux, batched (Riemann surface) ---------------------------
  * Riemann::Riemann(WR, WL, vFrame, Aij, i) + Riemann::exact(Fij, gamma)
  * (Riemann.h:19,26; Riemann.cpp:7-229) for n given faces, FP64, with ctx's

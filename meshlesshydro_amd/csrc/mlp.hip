@@ -484,6 +484,72 @@ int mlh_riemann_exact_batch(mlh_ctx *c, int64_t n, double gamma, const double *r
     return MLH_OK;
 }
 
+int mlh_device_alloc(int64_t bytes, void **ptr) {
+    if (!ptr || bytes < 0) return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
+    *ptr = nullptr;
+    if (bytes == 0) return MLH_OK;
+    CHK(hipMalloc(ptr, (size_t)bytes));
+    return MLH_OK;
+}
+
+int mlh_device_free(void *ptr) {
+    if (ptr) CHK(hipFree(ptr));
+    return MLH_OK;
+}
+
+int mlh_device_copy(void *dst, const void *src, int64_t bytes, int32_t kind) {
+    if (bytes < 0 || (bytes > 0 && (!dst || !src)) || kind < 1 || kind > 3)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (bytes == 0) return MLH_OK;
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                          : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    CHK(hipMemcpy(dst, src, (size_t)bytes, k));
+    return MLH_OK;
+}
+
+int mlh_riemann_exact_batch_device(mlh_ctx *c, int64_t n, double gamma, const double *states,
+                                   double *star, int32_t *flag, int32_t *iters, int32_t precision,
+                                   int32_t seed, double tol, int64_t *fallbacks,
+                                   float *kernel_ms) {
+    if (n < 0 || (n > 0 && (!states || !star)))
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    if (precision != MLH_FP64 && precision != MLH_FP32)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad precision");
+    const MlpFrags *F = nullptr;
+    if (seed == MLH_SEED_LEARNED) {
+        if (!c || !ctx_mlp(c).params) return ctx_fail(MLH_ERR_NOT_TRAINED, "no network parameters");
+        F = ctx_mlp(c).frags;
+    }
+    if (fallbacks) *fallbacks = 0;
+    if (kernel_ms) *kernel_ms = 0.f;
+    if (n == 0) return MLH_OK;
+    hipStream_t s = c ? ctx_stream(c) : nullptr;
+    unsigned long long *fb = nullptr;
+    CHK(dmalloc(&fb, sizeof(unsigned long long)));
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipMemsetAsync(fb, 0, sizeof(unsigned long long), s);
+    hipEventRecord(e0, s);
+    hipLaunchKernelGGL(k_riemann_batch, dim3((n + 255) / 256), dim3(256), 0, s, n, gamma, states,
+                       states + n, states + 2 * n, states + 3 * n, states + 4 * n, states + 5 * n,
+                       star, star + n, star + 2 * n, flag, iters, precision, F, tol, 50, 3, fb,
+                       nullptr);
+    hipEventRecord(e1, s);
+    unsigned long long h_fb = 0;
+    hipMemcpyAsync(&h_fb, fb, sizeof(h_fb), hipMemcpyDeviceToHost, s);
+    hipError_t e = hipStreamSynchronize(s);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(fb);
+    if (e != hipSuccess) return ctx_fail(MLH_ERR_DEVICE, hipGetErrorString(e));
+    if (fallbacks) *fallbacks = (int64_t)h_fb;
+    if (kernel_ms) *kernel_ms = ms;
+    return MLH_OK;
+}
+
 int mlh_riemann_pstar_batch(int64_t n, double gamma, const double *rhoL, const double *uL,
                             const double *PL, const double *rhoR, const double *uR,
                             const double *PR, double tol, double *p_star, int32_t *iters) {

@@ -46,6 +46,12 @@ SIGNATURES = {
     "mlh_get_timings": (C.c_int, [_vp, _fp]),
     "mlh_riemann_exact_batch": (C.c_int, [_vp, C.c_int64, C.c_double] + [_dp] * 9 + [_ip, _ip, C.c_int32, C.c_int32, C.c_double]),
     "mlh_riemann_flux_batch": (C.c_int, [_vp, C.c_int64] + [_dp] * 5 + [_ip]),
+    "mlh_device_alloc": (C.c_int, [C.c_int64, C.POINTER(C.c_void_p)]),
+    "mlh_device_free": (C.c_int, [C.c_void_p]),
+    "mlh_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]),
+    "mlh_riemann_exact_batch_device": (C.c_int, [_vp, C.c_int64, C.c_double, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                                 C.c_double, C.POINTER(C.c_int64), _fp]),
     "mlh_mlp_features": (C.c_int, [C.c_int64, C.c_double] + [_dp] * 6 + [_fp, _dp]),
     "mlh_mlp_set_params": (C.c_int, [_vp, _fp]),
     "mlh_mlp_get_params": (C.c_int, [_vp, _fp]),
@@ -277,6 +283,70 @@ def riemann_batch(gamma, rl, ul, pl, rr, ur, pr, precision=MLH_FP64, seed=MLH_SE
     flag, it = np.zeros(n, np.int32), np.zeros(n, np.int32)
     _chk(L.mlh_riemann_exact_batch(ctx, n, gamma, *[_d(a) for a in arrs], _d(rs), _d(us), _d(ps), _i(flag), _i(it), precision, seed, tol), "mlh_riemann_exact_batch")
     return rs, us, ps, flag, it
+
+
+class DeviceBuffer:
+    """A device allocation of the library's HIP runtime (mlh_device_alloc)
+    holding an array of `shape` and numpy `dtype`."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(x) for x in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _chk(lib().mlh_device_alloc(self.nbytes, C.byref(p)), "mlh_device_alloc")
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.shape, a.dtype)
+        _chk(lib().mlh_device_copy(b.ptr, a.ctypes.data_as(C.c_void_p), b.nbytes, 1), "mlh_device_copy")
+        return b
+
+    def copy_from(self, a, offset_bytes=0):
+        a = np.ascontiguousarray(a)
+        if offset_bytes < 0 or offset_bytes + a.nbytes > self.nbytes:
+            raise ValueError("copy past the buffer")
+        dst = C.c_void_p(self.ptr.value + offset_bytes)
+        _chk(lib().mlh_device_copy(dst, a.ctypes.data_as(C.c_void_p), a.nbytes, 1), "mlh_device_copy")
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _chk(lib().mlh_device_copy(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, 2), "mlh_device_copy")
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().mlh_device_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def riemann_batch_device(gamma, states, star, flag=None, iters=None, precision=MLH_FP64,
+                         seed=MLH_SEED_STANDARD, tol=1e-6, ctx=None):
+    """mlh_riemann_exact_batch_device on DeviceBuffers: float64 states
+    [6, n] and star [3, n], int32 flag / iters [n] or None.  Returns
+    (fallbacks, kernel milliseconds)."""
+    n = states.shape[1]
+    if states.dtype != np.float64 or star.dtype != np.float64 or states.shape != (6, n) \
+            or star.shape != (3, n):
+        raise ValueError("states [6, n] and star [3, n] must be float64 device buffers")
+    for b in (flag, iters):
+        if b is not None and (b.dtype != np.int32 or b.shape != (n,)):
+            raise ValueError("flag / iters must be int32 device buffers of n entries")
+    fb = C.c_int64()
+    ms = (C.c_float * 1)()
+    ptr = lambda b: b.ptr if b is not None else None
+    _chk(lib().mlh_riemann_exact_batch_device(ctx, n, gamma, states.ptr, star.ptr, ptr(flag), ptr(iters),
+                                              precision, seed, tol, C.byref(fb), ms),
+         "mlh_riemann_exact_batch_device")
+    return fb.value, float(ms[0])
 
 
 def riemann_pstar(gamma, rl, ul, pl, rr, ur, pr, tol=1e-6):

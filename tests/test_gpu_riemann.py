@@ -81,3 +81,24 @@ def test_face_flux_batch_vs_oracle(hip, oracle_mod, dim):
     for c in range(dim + 2):
         err = np.abs(F[:, c] - Fo[:, c]) / scale[c]
         assert np.max(err) < 1e-10, (c, np.max(err))
+
+
+@pytest.mark.parametrize("prec", [MLH_FP64, MLH_FP32])
+def test_device_batch_matches_host_batch(hip, prec):
+    """mlh_riemann_exact_batch_device on device-resident states gives the
+    host batch's results bitwise (same kernel), and reports a positive
+    kernel time."""
+    n = 200_000
+    rl, ul, pl, rr, ur, pr = ics.riemann_states(n, seed=7)
+    rs, us, ps, fl, it = hip.riemann_batch(5.0 / 3.0, rl, ul, pl, rr, ur, pr, precision=prec)
+    states = hip.DeviceBuffer.from_array(np.stack([rl, ul, pl, rr, ur, pr]))
+    star = hip.DeviceBuffer((3, n), np.float64)
+    flag = hip.DeviceBuffer(n, np.int32)
+    iters = hip.DeviceBuffer(n, np.int32)
+    fb, ms = hip.riemann_batch_device(5.0 / 3.0, states, star, flag, iters, precision=prec)
+    out = star.numpy()
+    assert np.array_equal(out[0], rs) and np.array_equal(out[1], us) and np.array_equal(out[2], ps)
+    assert np.array_equal(flag.numpy(), fl) and np.array_equal(iters.numpy(), it)
+    assert ms > 0.0 and fb >= 0
+    for b in (states, star, flag, iters):
+        b.free()
