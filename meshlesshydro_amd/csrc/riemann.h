@@ -430,7 +430,11 @@ __device__ inline RSol<float> mixed_finish(const GasConst<float> &cf, const GasC
         um = 0.5 * ((double)ul + (double)ur + (fr - fl) - (double)(frd - fld) * dp);
     }
     if (ok) return sample(cf, float(pn), float(um), dl, ul, pl, m.cl, dr, ur, pr, m.cr);
-    RSol<double> r = solve_fp64(cd, dl, ul, pl, dr, ur, pr, tol, max_iter, -1.0, st);
+    // FP64 fallback, from the last FP64 step when it is positive (the FP32
+    // root refined once: far closer than the standard guess), else from the
+    // standard guess
+    RSol<double> r = solve_fp64(cd, dl, ul, pl, dr, ur, pr, tol, max_iter,
+                                m.valid && pn > 0.0 ? pn : -1.0, st);
     return RSol<float>{float(r.d), float(r.u), float(r.p), r.flag};
 }
 
