@@ -937,11 +937,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     // FP32 learned: faces whose probe fails are deferred to k_faces_deferred,
     // which seeds them with the network; the FP64 learned kernel runs the
     // network inside the face loop, over the wavefront's longest row
-#ifdef MLH_FP32_INLOOP
-    constexpr bool kInLoopNet = LEARNED;
-#else
     constexpr bool kInLoopNet = LEARNED && !kFp32;
-#endif
     __shared__ uint16_t lds[kInLoopNet ? 4 : 1][kInLoopNet ? 64 * 16 : 1];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     const bool live = t < a.N;
@@ -975,9 +971,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
             // to O(jump^2) there): |dp| > k (pL + pR) or
             // du^2 rhoR rhoL > 2 k^2 gamma (pR rhoL + pL rhoR)
             bool strong = false;
-            if constexpr (kFp32) {
-                strong = act && mp.valid && !mp.accepted;
-            } else if (act) {
+            if (act) {
                 T dp = fp.WR[1] - fp.WL[1], du = fp.uj - fp.ui;
                 T psum = fp.WR[1] + fp.WL[1];
                 T rr = fp.WR[0] * fp.WL[0];
@@ -988,7 +982,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
             if (__ballot(strong)) seed = face_net_seed<D, T>(a, strong, fp, lds[threadIdx.x >> 6], fs);
         }
         if (act) {
-            if (LEARNED && kFp32 && !kInLoopNet && mp.valid && !mp.accepted &&
+            if (LEARNED && kFp32 && mp.valid && !mp.accepted &&
                 mp.chg > kSeedGate && n < 128) {
                 // the standard guess was far off: Newton from the network's seed
                 defer[n >> 6] |= 1ull << (n & 63);
