@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmlh_hip.so")
 
-SOURCES = ["api.hip", "cells.hip", "hydro.hip", "mlp.hip"]
+SOURCES = ["api.hip", "cells.hip", "hydro.hip", "hydro_fp32.hip", "mlp.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

@@ -88,6 +88,10 @@ int dt_blocks(int N);
 void launch_dt_finalize(const HydroArgs &a, double dt_fixed, double t_now, double t_clip,
                         const double *dt_in, double *dt_hist, int step, hipStream_t s);
 void launch_faces(const HydroArgs &a, int D, int precision, hipStream_t s);
+// the FP32 instantiations (hydro_fp32.hip)
+void launch_density_fp32(const HydroArgs &a, int D, hipStream_t s);
+void launch_gradients_fp32(const HydroArgs &a, int D, hipStream_t s);
+void launch_faces_fp32(const HydroArgs &a, int D, hipStream_t s);
 // bytes of one deferred-particle record (FP32 learned seed)
 size_t defer_record_bytes();
 void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s);
