@@ -41,6 +41,7 @@ __device__ inline float root5(float x) {
 template <typename T>
 struct GasConst {
     T g, g1, g2, g3, g4, g5, g6, g7, ginv;
+    T sg5;     // sqrt(g5)
     bool g53;  // gamma == 5/3: g1 = 1/5, 1/gamma = 3/5, g3 = 5, g4 = 3
     __host__ __device__ explicit GasConst(double gam) {
         g53 = gam == 5.0 / 3.0;
@@ -53,6 +54,7 @@ struct GasConst {
         g6 = T((gam - 1.0) / (gam + 1.0));
         g7 = T((gam - 1.0) / 2.0);
         ginv = T(1.0 / gam);
+        sg5 = T(sqrt(2.0 / (gam + 1.0)));
     }
 };
 
@@ -116,21 +118,41 @@ __device__ inline double root5_nodiv(double x) {
     return x * (r2 * r2);
 }
 
+// FP64 reciprocal and reciprocal square root without the divide / square
+// root expansions: the FP32 hardware estimate (~1 ulp of FP32) and two FP64
+// Newton steps (error ~(1e-7)^4, i.e. rounding-limited).  Outside the FP32
+// normal range the exact operations.
+__device__ inline double rcp_nodiv(double x) {
+    if (!(fabs(x) > 1e-30) || !(fabs(x) < 1e30)) return 1.0 / x;
+    double r = (double)__builtin_amdgcn_rcpf((float)x);
+    r = r * (2.0 - x * r);
+    r = r * (2.0 - x * r);
+    return r;
+}
+__device__ inline double rsqrt_nodiv(double x) {
+    if (!(x > 1e-30) || !(x < 1e30)) return 1.0 / sqrt(x);
+    double r = (double)__builtin_amdgcn_rsqf((float)x);
+    r = r * (1.5 - 0.5 * x * (r * r));
+    r = r * (1.5 - 0.5 * x * (r * r));
+    return r;
+}
+
 // Residual of the exact solver for the FP32 mode's acceptance steps: f_K(p)
 // in FP64 (it cancels against the other side and du at the root), its
 // derivative in FP32 (it only scales the Newton correction; a 1e-7 relative
 // error of the correction is far below the change test).
 __device__ inline void prefun_res(const GasConst<double> &c, double p, double rk, double pk,
                                   double ck, double &f, float &fd) {
+    // divide-free: p / p_K = p * (1 / p_K), sqrt(A_K / (B_K + p)) =
+    // sqrt(g5) / sqrt(rho_K (B_K + p)), both to FP64 rounding
     if (p <= pk) {
-        double prat = p / pk;
+        double prat = p * rcp_nodiv(pk);
         double y = c.g53 ? root5_nodiv(prat) : pow(prat, c.g1);
         f = c.g4 * ck * (y - 1.0);
         fd = (1.f / ((float)rk * (float)ck)) * ((float)y / (float)prat);
     } else {
-        double ak = c.g5 / rk;
         double bk = c.g6 * pk;
-        double qrt = sqrt(ak / (bk + p));
+        double qrt = c.sg5 * rsqrt_nodiv(rk * (bk + p));
         f = (p - pk) * qrt;
         fd = (1.f - 0.5f * (float)(p - pk) / (float)(bk + p)) * (float)qrt;
     }
