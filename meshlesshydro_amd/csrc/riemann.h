@@ -140,19 +140,23 @@ __device__ inline double rsqrt_nodiv(double x) {
 // Residual of the exact solver for the FP32 mode's acceptance steps: f_K(p)
 // in FP64 (it cancels against the other side and du at the root), its
 // derivative in FP32 (it only scales the Newton correction; a 1e-7 relative
-// error of the correction is far below the change test).
+// error of the correction is far below the change test).  ckf is the FP32
+// sound speed.  Divide-free, to FP64 rounding: p / p_K = p * (1 / p_K);
+// both branches need one reciprocal square root, taken outside the branch:
+//   rarefaction: c_K = sqrt(g p_K / rho_K) = g p_K / sqrt(g p_K rho_K),
+//   shock:       sqrt(A_K / (B_K + p)) = sqrt(g5) / sqrt(rho_K (B_K + p)).
 __device__ inline void prefun_res(const GasConst<double> &c, double p, double rk, double pk,
-                                  double ck, double &f, float &fd) {
-    // divide-free: p / p_K = p * (1 / p_K), sqrt(A_K / (B_K + p)) =
-    // sqrt(g5) / sqrt(rho_K (B_K + p)), both to FP64 rounding
-    if (p <= pk) {
+                                  float ckf, double &f, float &fd) {
+    const bool rare = p <= pk;
+    const double gp = c.g * pk, bk = c.g6 * pk;
+    const double r = rsqrt_nodiv(rare ? gp * rk : rk * (bk + p));
+    if (rare) {
         double prat = p * rcp_nodiv(pk);
         double y = c.g53 ? root5_nodiv(prat) : pow(prat, c.g1);
-        f = c.g4 * ck * (y - 1.0);
-        fd = (1.f / ((float)rk * (float)ck)) * ((float)y / (float)prat);
+        f = c.g4 * (gp * r) * (y - 1.0);
+        fd = (1.f / ((float)rk * ckf)) * ((float)y / (float)prat);
     } else {
-        double bk = c.g6 * pk;
-        double qrt = c.sg5 * rsqrt_nodiv(rk * (bk + p));
+        double qrt = c.sg5 * r;
         f = (p - pk) * qrt;
         fd = (1.f - 0.5f * (float)(p - pk) / (float)(bk + p)) * (float)qrt;
     }
@@ -401,11 +405,10 @@ __device__ inline MixedProbe mixed_probe(const GasConst<float> &cf, const GasCon
     float tiny = 1e-6f * 0.5f * (pl + pr);
     float p = p_start > 0.f ? p_start : guess_p(cf, dl, ul, pl, m.cl, dr, ur, pr, m.cr, tiny);
     const double dld = dl, pld = pl, drd = dr, prd = pr, dud = (double)ur - (double)ul;
-    const double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
     double fl, fr, pd = p;
     float fld, frd;
-    prefun_res(cd, pd, dld, pld, cld, fl, fld);
-    prefun_res(cd, pd, drd, prd, crd, fr, frd);
+    prefun_res(cd, pd, dld, pld, m.cl, fl, fld);
+    prefun_res(cd, pd, drd, prd, m.cr, fr, frd);
     double dp = (fl + fr + dud) * (double)(1.f / (fld + frd));
     double pn = pd - dp;
     st.iters += 1;
@@ -439,11 +442,10 @@ __device__ inline RSol<float> mixed_finish(const GasConst<float> &cf, const GasC
         }
         st.iters += it;
         const double dld = dl, pld = pl, drd = dr, prd = pr, dud = (double)ur - (double)ul;
-        const double cld = sqrt(cd.g * pld / dld), crd = sqrt(cd.g * prd / drd);
         double fl, fr, pd = p;
         float fld, frd;
-        prefun_res(cd, pd, dld, pld, cld, fl, fld);
-        prefun_res(cd, pd, drd, prd, crd, fr, frd);
+        prefun_res(cd, pd, dld, pld, m.cl, fl, fld);
+        prefun_res(cd, pd, drd, prd, m.cr, fr, frd);
         double dp = (fl + fr + dud) * (double)(1.f / (fld + frd));
         pn = pd - dp;
         st.iters += 1;
