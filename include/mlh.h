@@ -296,7 +296,10 @@ int mlh_mlp_infer(mlh_ctx *ctx, int64_t n, const float *feats, float *out,
  * (particle_stride <= 0 switches recording off).  Training data for the
  * learned seed, including online retraining during a run. */
 int mlh_set_face_collection(mlh_ctx *ctx, int64_t cap, int32_t particle_stride);
-/* Copy the recorded faces (n x 6 doubles) out and restart the recording. */
+/* Copy the recorded faces (n x 6 doubles; states6 holds at least cap rows)
+ * out and restart the recording.  The record is filled in slices, one per
+ * counter, to spread the atomics; faces come out slice by slice, not in
+ * the order they were solved. */
 int mlh_get_collected_faces(mlh_ctx *ctx, double *states6, int64_t *n_out);
 
 #ifdef __cplusplus

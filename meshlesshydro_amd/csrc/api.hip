@@ -216,7 +216,8 @@ static HydroArgs hydro_args(mlh_ctx *c) {
     a.defer_n = c->defer_n;
     a.collect = c->mlp.collect_stride > 0 ? c->mlp.collect : nullptr;
     a.collect_n = c->mlp.collect_n;
-    a.collect_cap = c->mlp.collect_cap;
+    a.collect_scap = c->mlp.collect_scap;
+    a.collect_smask = c->mlp.collect_stripes > 0 ? c->mlp.collect_stripes - 1 : 0;
     a.collect_stride = c->mlp.collect_stride > 0 ? c->mlp.collect_stride : 1;
     return a;
 }
@@ -817,13 +818,20 @@ int mlh_set_face_collection(mlh_ctx *c, int64_t cap, int32_t particle_stride) {
     if (!c || cap < 0) return fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
     MlpState &m = c->mlp;
     if (m.collect) { hipFree(m.collect); m.collect = nullptr; }
-    if (!m.collect_n) HIPCHK(hipMalloc(&m.collect_n, sizeof(unsigned long long)));
-    HIPCHK(hipMemset(m.collect_n, 0, sizeof(unsigned long long)));
-    m.collect_cap = 0;
+    if (!m.collect_n) HIPCHK(hipMalloc(&m.collect_n, sizeof(unsigned long long) * kCollectStripes));
+    HIPCHK(hipMemset(m.collect_n, 0, sizeof(unsigned long long) * kCollectStripes));
+    m.collect_cap = m.collect_scap = 0;
+    m.collect_stripes = 0;
     m.collect_stride = 0;
     if (particle_stride <= 0 || cap == 0) return MLH_OK;
     HIPCHK(hipMalloc(&m.collect, sizeof(double) * 6 * (size_t)cap));
+    // as many stripes as fit (a power of two, at most kCollectStripes), each
+    // of cap / stripes rows; the rows past stripes * scap stay unused
+    int stripes = 1;
+    while (stripes * 2 <= kCollectStripes && stripes * 2 <= cap) stripes *= 2;
     m.collect_cap = cap;
+    m.collect_stripes = stripes;
+    m.collect_scap = cap / stripes;
     m.collect_stride = particle_stride;
     return MLH_OK;
 }
@@ -833,14 +841,21 @@ int mlh_get_collected_faces(mlh_ctx *c, double *states6, int64_t *n_out) {
     MlpState &m = c->mlp;
     *n_out = 0;
     if (!m.collect || !m.collect_n) return MLH_OK;
-    unsigned long long n = 0;
+    unsigned long long n[kCollectStripes] = {};
     HIPCHK(hipStreamSynchronize(c->s));
-    HIPCHK(hipMemcpy(&n, m.collect_n, sizeof(n), hipMemcpyDeviceToHost));
-    if (n > (unsigned long long)m.collect_cap) n = (unsigned long long)m.collect_cap;
-    if (states6 && n)
-        HIPCHK(hipMemcpy(states6, m.collect, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemset(m.collect_n, 0, sizeof(unsigned long long)));
-    *n_out = (int64_t)n;
+    HIPCHK(hipMemcpy(n, m.collect_n, sizeof(unsigned long long) * m.collect_stripes,
+                     hipMemcpyDeviceToHost));
+    // each stripe's filled prefix, stripe after stripe
+    int64_t out = 0;
+    for (int s = 0; s < m.collect_stripes; ++s) {
+        const int64_t k = n[s] < (unsigned long long)m.collect_scap ? (int64_t)n[s] : m.collect_scap;
+        if (states6 && k)
+            HIPCHK(hipMemcpy(states6 + 6 * out, m.collect + 6 * (size_t)s * m.collect_scap,
+                             sizeof(double) * 6 * k, hipMemcpyDeviceToHost));
+        out += k;
+    }
+    HIPCHK(hipMemset(m.collect_n, 0, sizeof(unsigned long long) * kCollectStripes));
+    *n_out = out;
     return MLH_OK;
 }
 

@@ -826,22 +826,27 @@ struct FluxSums {
 // training data: the face states of the faces the solver needed Newton
 // iterations for (left / right rho, normal velocity, P), for the lanes with
 // `want`.  Called by the active lanes of a wavefront together: one atomic
-// per wavefront for the slots, nothing once the record is full.
+// per wavefront for the slots, nothing once its stripe is full.  The
+// wavefront writes to stripe (wavefront index) % stripes, each with its own
+// counter: with one counter, the atomics of all wavefronts serialise on one
+// address while the record fills.
 template <int D, typename T>
 __device__ inline void collect_faces(const HydroArgs &a, bool want, const FacePrep<D, T> &fp) {
     const unsigned long long b = __ballot(want);
     if (!b) return;
-    const unsigned long long cap = (unsigned long long)a.collect_cap;
-    if (__atomic_load_n(a.collect_n, __ATOMIC_RELAXED) >= cap) return;
+    const int s = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) & a.collect_smask;
+    unsigned long long *cnt = a.collect_n + s;
+    const unsigned long long cap = (unsigned long long)a.collect_scap;
+    if (__atomic_load_n(cnt, __ATOMIC_RELAXED) >= cap) return;
     const int lane = lane_id();
     const int leader = __ffsll((long long)b) - 1;
     unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(a.collect_n, (unsigned long long)__popcll(b));
+    if (lane == leader) base = atomicAdd(cnt, (unsigned long long)__popcll(b));
     base = __shfl(base, leader, 64);
     if (want) {
         const unsigned long long slot = base + __popcll(b & ((1ull << lane) - 1ull));
         if (slot < cap) {
-            double *c = a.collect + slot * 6;
+            double *c = a.collect + ((unsigned long long)s * cap + slot) * 6;
             c[0] = fp.WR[0]; c[1] = fp.ui; c[2] = fp.WR[1];
             c[3] = fp.WL[0]; c[4] = fp.uj; c[5] = fp.WL[1];
         }

@@ -54,9 +54,10 @@ struct HydroArgs {
     // their count
     void *defer_rec;
     int *defer_n;
-    double *collect;             // face collection [cap][6] (or null)
-    unsigned long long *collect_n;
-    long long collect_cap;
+    double *collect;             // face collection [stripes][scap][6] (or null)
+    unsigned long long *collect_n;  // [stripes] fill counters
+    long long collect_scap;      // rows per stripe
+    int collect_smask;           // stripes - 1 (a power of two)
     int collect_stride;
 };
 

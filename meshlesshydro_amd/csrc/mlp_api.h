@@ -23,12 +23,19 @@ struct MlpState {
     MlpFrags *frags = nullptr;  // packed BF16 fragments (device)
     float *adam_m = nullptr, *adam_v = nullptr, *grad = nullptr;
     int64_t adam_t = 0;
-    // face collection for (online) training
+    // face collection for (online) training: the record is split into
+    // collect_stripes slices of collect_scap rows, each filled through its
+    // own counter (a wavefront writes to slice wave % stripes)
     double *collect = nullptr;          // [cap][6] rhoL,uL,PL,rhoR,uR,PR
-    unsigned long long *collect_n = nullptr;
+    unsigned long long *collect_n = nullptr;  // [kCollectStripes] fill counters
     int64_t collect_cap = 0;
+    int64_t collect_scap = 0;
+    int collect_stripes = 0;
     int collect_stride = 0;
 };
+
+// most stripes of the face record
+constexpr int kCollectStripes = 64;
 
 void mlp_free(MlpState &m);
 
