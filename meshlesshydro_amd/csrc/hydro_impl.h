@@ -259,7 +259,13 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                 // others (including NaN) the reference's value is ci + cj
                 double dv = dotT<D, double>(vij, xij);
                 double vs = ci + cj;
-                if (dv < 0.) {
+                if constexpr (kFused) {
+                    // FP32 mode: 1/|x_ij| as a refined reciprocal square
+                    // root (FP64 rounding, no divide) for every pair, then
+                    // the same selection, without a divergent branch
+                    const double vx = dv * rsqrt_nodiv(dotT<D, double>(xij, xij));
+                    vs = dv < 0. ? vs - (vx < 0. ? vx : 0.) : vs;
+                } else if (dv < 0.) {
                     double vx = dv / sqrt(dotT<D, double>(xij, xij));
                     vx = vx < 0. ? vx : 0.;
                     vs = vs - vx;
