@@ -226,18 +226,25 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
             double xj[3];
             int s[3];
             int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
-            T r = tsqrt(T(dist2<D>(xi, xj)));
+            // one set of differences dd = x_j - x_i serves the distance, E,
+            // the FP32 gradient sums and the signal speed; (-a)(-a) = a a and
+            // -(a + b) = (-a) + (-b) exactly, so |dd|^2 is dist2(x_i, x_j)
+            // and -(v_ij . dd) is v_ij . x_ij bit for bit
+            double dd[3] = {0., 0., 0.};
+            for (int p = 0; p < D; ++p) dd[p] = xj[p] - xi[p];
+            const double d2 = dotT<D, double>(dd, dd);
+            T r = tsqrt(T(d2));
             T wt = kernel_w<T>(r, h2, sig) / om_i;
             double w = (double)wt;
             // E is symmetric and a*b*w == b*a*w in IEEE arithmetic: the
             // upper triangle, mirrored after the loop
             for (int p = 0; p < D; ++p)
-                for (int q = p; q < D; ++q) E[D * p + q] += (xj[p] - xi[p]) * (xj[q] - xi[q]) * w;
+                for (int q = p; q < D; ++q) E[D * p + q] += dd[p] * dd[q] * w;
             T prj[kPrimN];
             if (kFused) {
                 load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
                 T dx[D];
-                for (int p = 0; p < D; ++p) dx[p] = T(xj[p] - xi[p]) * wt;
+                for (int p = 0; p < D; ++p) dx[p] = T(dd[p]) * wt;
                 for (int f = 0; f < (kFused ? NF : 1); ++f) {
                     T fj = prj[field_slot<D>(f)];
                     T df = fj - fi[f];
@@ -249,24 +256,23 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
             if (n < nr) {
                 const T *pj = prim + (size_t)j * kPrimN;
                 double cj = kFused ? (double)prj[kPrimC] : (double)pj[kPrimC];
-                double xij[3], vij[3];
-                for (int ax = 0; ax < D; ++ax) {
-                    xij[ax] = xi[ax] - xj[ax];
+                double vij[3];
+                for (int ax = 0; ax < D; ++ax)
                     vij[ax] = vi[ax] - (kFused ? (double)prj[kPrimV + ax] : (double)pj[kPrimV + ax]);
-                }
-                // vx = min(v_ij.x_ij / |x_ij|, 0): only approaching pairs
-                // (v_ij.x_ij < 0) need the divide and square root; for the
-                // others (including NaN) the reference's value is ci + cj
-                double dv = dotT<D, double>(vij, xij);
+                // vx = min(v_ij.x_ij / |x_ij|, 0) with x_ij = x_i - x_j = -dd:
+                // only approaching pairs (v_ij.x_ij < 0) need the divide and
+                // square root; for the others (including NaN) the reference's
+                // value is ci + cj
+                double dv = -dotT<D, double>(vij, dd);
                 double vs = ci + cj;
                 if constexpr (kFused) {
                     // FP32 mode: 1/|x_ij| as a refined reciprocal square
                     // root (FP64 rounding, no divide) for every pair, then
                     // the same selection, without a divergent branch
-                    const double vx = dv * rsqrt_nodiv(dotT<D, double>(xij, xij));
+                    const double vx = dv * rsqrt_nodiv(d2);
                     vs = dv < 0. ? vs - (vx < 0. ? vx : 0.) : vs;
                 } else if (dv < 0.) {
-                    double vx = dv / sqrt(dotT<D, double>(xij, xij));
+                    double vx = dv / sqrt(d2);
                     vx = vx < 0. ? vx : 0.;
                     vs = vs - vx;
                 }
