@@ -126,6 +126,10 @@ def test_stages_fp32(name, hip, oracle_mod):
     orr, oP, ow = orc.get_density()
     assert relerr(gw, ow) < 1e-6 and relerr(gr, orr) < 1e-6 and relerr(gP, oP) < 1e-6
     dt = orc.timestep()
+    # the FP32 signal speeds (FP32 sound speeds and velocities, 1/|x_ij| from
+    # a refined reciprocal square root) give dt to ~1e-7
+    dt32 = sim.timestep()
+    assert abs(dt32 - dt) <= 1e-6 * dt, (name, dt32, dt)
     sim.gradients(); orc.gradients()
     gG, gB = sim.get_gradients()
     oG, oB = orc.get_gradients()
