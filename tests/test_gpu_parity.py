@@ -94,6 +94,14 @@ def test_multistep_and_conservation(hip, oracle_mod):
     gs, os_ = sim.state(), orc.state()
     for k in gs:
         assert relerr(gs[k], os_[k]) < 1e-9, k
+    # the device sums (sumMass / sumEnergy / sumMomentum*) against numpy sums
+    # of the same state: sum m, sum m (u + v^2 / 2), sum m v
+    m = gs["m"]
+    v2 = gs["vx"] ** 2 + gs["vy"] ** 2
+    assert abs(c1[1] - np.sum(m)) <= 1e-13 * np.sum(m)
+    assert abs(c1[2] - np.sum(m * (gs["u"] + 0.5 * v2))) <= 1e-12 * abs(c1[2])
+    assert abs(c1[3] - np.sum(m * gs["vx"])) <= 1e-13 * np.sum(m)
+    assert abs(c1[4] - np.sum(m * gs["vy"])) <= 1e-13 * np.sum(m)
     sim.close()
 
 
