@@ -65,3 +65,28 @@ def test_create_without_device_fails_cleanly():
         assert e.status == _abi.MLH_ERR_DEVICE
     else:
         sim.close()
+
+
+def test_create_rejects_bad_arguments_before_any_device_call():
+    """mlh_create validates its arguments before it touches the device, so
+    these run anywhere: each bad argument gives MLH_ERR_INVALID_ARGUMENT
+    with its own message and leaves the context pointer null."""
+    from meshlesshydro_amd import lib, _abi
+    L = lib.lib()
+    cases = [
+        (dict(), 0, "n <= 0"),
+        (dict(), -5, "n <= 0"),
+        (dict(dim=4), 100, "dim must be 2 or 3"),
+        (dict(), 1 << 27, "n exceeds 2^27-1"),
+        (dict(max_interactions=0), 100, "bad interaction capacities"),
+        (dict(max_ghost_interactions=-1), 100, "bad interaction capacities"),
+        (dict(precision=7), 100, "bad precision"),
+        (dict(work_order=9), 100, "bad work_order"),
+    ]
+    for kw, n, msg in cases:
+        p = _abi.default_params(kernel_size=0.1, **kw)
+        h = ctypes.c_void_p()
+        st = L.mlh_create(ctypes.byref(p), n, ctypes.byref(h))
+        assert st == _abi.MLH_ERR_INVALID_ARGUMENT, (kw, n, st)
+        assert msg in L.mlh_last_error().decode(), (kw, n)
+        assert not h.value
