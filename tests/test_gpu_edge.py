@@ -8,6 +8,10 @@
   state after three steps. The isolated particles only drift: m is
   unchanged, u and v to a few ulps (the update divides m v by m), and x
   moves by v times the elapsed time. FP32 mode leaves them the same way.
+* Error paths: negative face pressures abort at the reference's debug
+  levels with status 6 (against the oracle); stages called out of order,
+  a learned seed without a network and a too-coarse periodic grid report
+  their own status codes.
 """
 import numpy as np
 import pytest
@@ -80,3 +84,66 @@ def test_isolated_particles(hip, oracle_mod):
     assert np.array_equal(o32["m"][:3], s["m"][:3])
     for k in ("vx", "vy", "u"):
         assert np.allclose(o32[k][:3], s[k][:3], rtol=1e-14, atol=0), k
+
+
+def test_negative_face_pressure_status(hip, oracle_mod):
+    """Slope limiting off on the 16^3 blast: linear reconstruction next to
+    the energy spike gives negative face pressures on real faces in the
+    first step (none across images). The reference aborts with exit code 6
+    on real faces only above DEBUG_LVL 1 (Particles.cpp:1824-1827, ghost
+    faces :1878-1881 from level 1): level 2 fails on the first step like the
+    oracle; level 1 runs and counts the same faces as the oracle."""
+    from cases import CASES
+    from meshlesshydro_amd._abi import MLH_ERR_NEGATIVE_FACE_PRESSURE
+    p, s = CASES["blast16"](debug_level=2, slope_limiting=0)
+    sim = hip.Simulation(p, s)
+    with pytest.raises(hip.MLHError) as e:
+        sim.step()
+    assert e.value.status == MLH_ERR_NEGATIVE_FACE_PRESSURE
+    sim.close()
+    orc = oracle_mod.Oracle(p, s)
+    with pytest.raises(oracle_mod.OracleError) as eo:
+        orc.step()
+    assert eo.value.status == MLH_ERR_NEGATIVE_FACE_PRESSURE
+    p1, _ = CASES["blast16"](debug_level=1, slope_limiting=0)
+    sim = hip.Simulation(p1, s)
+    orc = oracle_mod.Oracle(p1, s)
+    sim.step()
+    orc.step()
+    dg, do = sim.diag(), orc.diag()
+    sim.close()
+    assert dg["neg_face_pressure_real"] == do["neg_face_pressure_real"] > 0
+    assert dg["neg_face_pressure_ghost"] == do["neg_face_pressure_ghost"]
+
+
+def test_call_order_and_configuration_errors(hip):
+    """Stages called before their inputs exist, a learned seed without a
+    network, and a periodic box with fewer than 3 search cells per axis are
+    reported with their own status codes; nothing runs."""
+    from cases import CASES
+    from meshlesshydro_amd import ics
+    from meshlesshydro_amd._abi import (MLH_ERR_CALL_ORDER, MLH_ERR_GRID, MLH_ERR_NOT_TRAINED,
+                                        MLH_SEED_LEARNED)
+    p, s = CASES["jitter12"]()
+    sim = hip.Simulation(p, s)
+    for call in (sim.density, sim.gradients, lambda: sim.fluxes(1e-3), sim.get_density):
+        with pytest.raises(hip.MLHError) as e:
+            call()
+        assert e.value.status == MLH_ERR_CALL_ORDER
+    sim.neighbours()
+    for call in (sim.gradients, lambda: sim.fluxes(1e-3), sim.get_gradients):
+        with pytest.raises(hip.MLHError) as e:
+            call()
+        assert e.value.status == MLH_ERR_CALL_ORDER
+    sim.close()
+    p32, _ = CASES["jitter12"](precision=MLH_FP32)
+    p32.riemann_seed = MLH_SEED_LEARNED
+    sim = hip.Simulation(p32, s)
+    with pytest.raises(hip.MLHError) as e:
+        sim.step()
+    assert e.value.status == MLH_ERR_NOT_TRAINED
+    sim.close()
+    # kernel size 0.4 in the unit box: floor(1 / 0.4) = 2 cells per axis
+    with pytest.raises(hip.MLHError) as e:
+        hip.Simulation(default_params(kernel_size=0.4), ics.kelvin_helmholtz(16))
+    assert e.value.status == MLH_ERR_GRID
