@@ -196,6 +196,11 @@ __device__ inline void invert(double *A) {
 }
 
 // ---------------------------------------------------------------- gradients
+#ifndef MLH_GRAD_BATCH
+#define MLH_GRAD_BATCH 4
+#endif
+// real neighbours per gather batch of the FP32 gradient pass
+constexpr int kGradBatch = MLH_GRAD_BATCH;
 // compPsijTilde (Particles.cpp:1186-1254, ghosts :2292-2455): E_i in FP64,
 // B_i = E_i^-1 in FP64; gradient (:1257-1270, :2457-2502) for rho, v, P;
 // slopeLimiter (:1313-1444); compGlobalTimestep (:1446-1485) per particle.
@@ -239,10 +244,9 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
         }
         for (int f = 0; f < (kFused ? NF : 1); ++f)
             for (int ax = 0; ax < D; ++ax) G[f][ax] = T(0);
-        for (int n = 0; n < cnt; ++n) {
-            double xj[3];
-            int s[3];
-            int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
+        // one neighbour of pass 1: n = entry, j = its work position, xj its
+        // position (image applied), prj its record (FP32 mode)
+        auto visit = [&](int n, int j, const double *xj, const T *prj) {
             // one set of differences dd = x_j - x_i serves the distance, E,
             // the FP32 gradient sums and the signal speed; (-a)(-a) = a a and
             // -(a + b) = (-a) + (-b) exactly, so |dd|^2 is dist2(x_i, x_j)
@@ -257,9 +261,7 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
             // upper triangle, mirrored after the loop
             for (int p = 0; p < D; ++p)
                 for (int q = p; q < D; ++q) E[D * p + q] += dd[p] * dd[q] * w;
-            T prj[kPrimN];
             if (kFused) {
-                load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
                 T dx[D];
                 for (int p = 0; p < D; ++p) dx[p] = T(dd[p]) * wt;
                 for (int f = 0; f < (kFused ? NF : 1); ++f) {
@@ -295,6 +297,39 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                 }
                 vsig = vs > vsig ? vs : vsig;
             }
+        };
+        int n = 0;
+        if constexpr (kFused) {
+            // real neighbours (no image transform) in batches: the batch's
+            // row entries, then its positions and records, are in flight
+            // together; the batch is visited in entry order, so E and the
+            // sums keep the reference order
+            constexpr int BG = kGradBatch;
+            for (; n + BG <= nr; n += BG) {
+                int jb[BG];
+#pragma unroll
+                for (int u = 0; u < BG; ++u) jb[u] = (int)nb_index(a.nbr[(size_t)(n + u) * N + t]);
+                double4 pb[BG];
+                T rb[BG][kPrimN];
+#pragma unroll
+                for (int u = 0; u < BG; ++u) {
+                    pb[u] = a.pos[jb[u]];
+                    load_vec<T, kPrimN>(rb[u], prim + (size_t)jb[u] * kPrimN);
+                }
+#pragma unroll
+                for (int u = 0; u < BG; ++u) {
+                    const double xj[3] = {pb[u].x, pb[u].y, pb[u].z};
+                    visit(n + u, jb[u], xj, rb[u]);
+                }
+            }
+        }
+        for (; n < cnt; ++n) {
+            double xj[3];
+            int s[3];
+            int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
+            T prj[kPrimN];
+            if (kFused) load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
+            visit(n, j, xj, prj);
         }
         for (int p = 1; p < D; ++p)
             for (int q = 0; q < p; ++q) E[D * p + q] = E[D * q + p];
