@@ -845,17 +845,15 @@ __global__ __launch_bounds__(256) void k_riemann_flux(HydroArgs a, long long n,
 // from the side that solves it: the lower original index (images: parent
 // >= i).  FP32: also the first step of the mixed solver.
 template <int D, typename T>
-__device__ inline void face_of_entry(const HydroArgs &a, int t, int i, int n, T dt,
+__device__ inline void face_of_entry(const HydroArgs &a, int t, int i, uint32_t e, int jo, T dt,
                                      FacePrep<D, T> &fp, MixedProbe &mp, bool &self_lo,
                                      FaceStats &fs) {
-    uint32_t e = a.nbr[(size_t)n * a.N + t];
     int j = (int)nb_index(e);
     uint32_t code = nb_code(e);
     bool image = code != kCodeReal;
     int s[3] = {0, 0, 0};
     if (image)
         for (int ax = 0; ax < D; ++ax) s[ax] = code_shift(code, ax);
-    const int jo = (int)a.pos[j].w;
     self_lo = !a.enforce_flux_sym || (image ? jo >= i : jo > i);
     const View<D, T> L = make_view<D, T>(a, self_lo ? t : j);
     const View<D, T> H = make_view<D, T>(a, self_lo ? j : t);
@@ -863,6 +861,16 @@ __device__ inline void face_of_entry(const HydroArgs &a, int t, int i, int n, T 
     for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
     face_prepare<D, T>(a, L, H, sl, image, dt, fp, fs);
     if constexpr (sizeof(T) == 4) mp = face_probe<D>(a, fp, fs.st);
+}
+
+// the face of row entry n of work position t: its entry and the neighbour's
+// original index (which decides the solving side) read here
+template <int D, typename T>
+__device__ inline void face_of_entry(const HydroArgs &a, int t, int i, int n, T dt,
+                                     FacePrep<D, T> &fp, MixedProbe &mp, bool &self_lo,
+                                     FaceStats &fs) {
+    const uint32_t e = a.nbr[(size_t)n * a.N + t];
+    face_of_entry<D, T>(a, t, i, e, (int)a.pos[nb_index(e)].w, dt, fp, mp, self_lo, fs);
 }
 
 // Network seed for this lane's face when want; every lane of the wavefront
@@ -1042,13 +1050,26 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     const bool collect = a.collect && live && (i % a.collect_stride) == 0;
     FluxSums acc{0., 0., {0., 0., 0.}};
     uint64_t defer[2] = {0ull, 0ull};
+    // software pipeline of the face's first two loads: row entries two
+    // faces ahead, the neighbour's original index one face ahead, so a face
+    // waits only on its records
+    uint32_t e0 = 0, e1 = 0;
+    int jo0 = 0;
+    if (cnt > 0) e0 = a.nbr[t];
+    if (cnt > 1) e1 = a.nbr[(size_t)N + t];
+    if (cnt > 0) jo0 = (int)a.pos[nb_index(e0)].w;
     for (int n = 0; n < lim; ++n) {
         const bool act = n < cnt;
+        const uint32_t e = e0;
+        const int jo = jo0;
+        if (n + 1 < cnt) jo0 = (int)a.pos[nb_index(e1)].w;
+        e0 = e1;
+        if (n + 2 < cnt) e1 = a.nbr[(size_t)(n + 2) * N + t];
         FacePrep<D, T> fp;
         MixedProbe mp{};
         bool self_lo = true;
         const int it0 = fs.st.iters;
-        if (act) face_of_entry<D, T>(a, t, i, n, dt, fp, mp, self_lo, fs);
+        if (act) face_of_entry<D, T>(a, t, i, e, jo, dt, fp, mp, self_lo, fs);
         float seed = -1.f;
         if constexpr (kInLoopNet) {
             // FP64: the network runs for faces with a real jump
