@@ -107,19 +107,19 @@ __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
     const T h2 = T(a.g.h / 2.), sig = T(a.g.sigma);
     int nr = a.noi[t], ng = a.nog[t];
     T omg = T(0);
-    // real neighbours (no image transform), four entries at a time: the
-    // four row entries, then the four positions, are in flight together;
+    // real neighbours (no image transform), eight entries at a time: the
+    // eight row entries, then the eight positions, are in flight together;
     // the sum keeps the entry order
     int n = 0;
-    for (; n + 4 <= nr; n += 4) {
-        uint32_t e[4];
+    for (; n + 8 <= nr; n += 8) {
+        uint32_t e[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) e[u] = a.nbr[(size_t)(n + u) * N + t];
-        double4 pj[4];
+        for (int u = 0; u < 8; ++u) e[u] = a.nbr[(size_t)(n + u) * N + t];
+        double4 pj[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pj[u] = a.pos[nb_index(e[u])];
+        for (int u = 0; u < 8; ++u) pj[u] = a.pos[nb_index(e[u])];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const double xj[3] = {pj[u].x, pj[u].y, pj[u].z};
             T r = tsqrt(T(dist2<D>(xi, xj)));
             omg += kernel_w<T>(r, h2, sig);
