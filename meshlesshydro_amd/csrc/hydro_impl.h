@@ -80,6 +80,17 @@ __device__ inline int nb_pos(const HydroArgs &a, uint32_t e, double *xj, int *s)
     return j;
 }
 
+// position of row entry e from its loaded cell-order record p (the periodic
+// image applied for an image entry), as nb_pos gives it
+template <int D>
+__device__ inline void entry_pos(const HydroArgs &a, uint32_t e, const double4 &p, double *xj) {
+    xj[0] = p.x; xj[1] = p.y; xj[2] = p.z;
+    const uint32_t code = nb_code(e);
+    if (code != kCodeReal)
+        for (int ax = 0; ax < D; ++ax)
+            xj[ax] = image_coord(xj[ax], code_shift(code, ax), a.g.bmin[ax], a.g.bmax[ax]);
+}
+
 template <typename T, int N>
 __device__ inline void load_vec(T *dst, const T *__restrict__ src) {
     static_assert((N * sizeof(T)) % 16 == 0, "record size");
@@ -133,8 +144,25 @@ __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
     }
     T om = omg + kernel_w<T>(T(0), h2, sig);
     if (a.g.periodic) {
+        // images, gathered eight at a time like the real neighbours
         T og = om;
-        for (int n = nr; n < nr + ng; ++n) {
+        int n = nr;
+        for (; n + 8 <= nr + ng; n += 8) {
+            uint32_t e[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) e[u] = a.nbr[(size_t)(n + u) * N + t];
+            double4 pj[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) pj[u] = a.pos[nb_index(e[u])];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                double xj[3];
+                entry_pos<D>(a, e[u], pj[u], xj);
+                T r = tsqrt(T(dist2<D>(xi, xj)));
+                og += kernel_w<T>(r, h2, sig);
+            }
+        }
+        for (; n < nr + ng; ++n) {
             double xj[3];
             int s[3];
             nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
@@ -300,26 +328,28 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
         };
         int n = 0;
         if constexpr (kFused) {
-            // real neighbours (no image transform) in batches: the batch's
-            // row entries, then its positions and records, are in flight
+            // neighbours (real, then images) in batches: the batch's row
+            // entries, then its positions and records, are in flight
             // together; the batch is visited in entry order, so E and the
             // sums keep the reference order
             constexpr int BG = kGradBatch;
-            for (; n + BG <= nr; n += BG) {
-                int jb[BG];
+            for (; n + BG <= cnt; n += BG) {
+                uint32_t eb[BG];
 #pragma unroll
-                for (int u = 0; u < BG; ++u) jb[u] = (int)nb_index(a.nbr[(size_t)(n + u) * N + t]);
+                for (int u = 0; u < BG; ++u) eb[u] = a.nbr[(size_t)(n + u) * N + t];
                 double4 pb[BG];
                 T rb[BG][kPrimN];
 #pragma unroll
                 for (int u = 0; u < BG; ++u) {
-                    pb[u] = a.pos[jb[u]];
-                    load_vec<T, kPrimN>(rb[u], prim + (size_t)jb[u] * kPrimN);
+                    const int j = (int)nb_index(eb[u]);
+                    pb[u] = a.pos[j];
+                    load_vec<T, kPrimN>(rb[u], prim + (size_t)j * kPrimN);
                 }
 #pragma unroll
                 for (int u = 0; u < BG; ++u) {
-                    const double xj[3] = {pb[u].x, pb[u].y, pb[u].z};
-                    visit(n + u, jb[u], xj, rb[u]);
+                    double xj[3];
+                    entry_pos<D>(a, eb[u], pb[u], xj);
+                    visit(n + u, (int)nb_index(eb[u]), xj, rb[u]);
                 }
             }
         }
