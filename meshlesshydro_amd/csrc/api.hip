@@ -56,6 +56,8 @@ struct mlh_ctx {
     double *sums_block = nullptr;
     void *defer_rec = nullptr;  // FP32 learned seed: deferred particles
     int *defer_n = nullptr;
+    unsigned long long *defer_hi = nullptr;  // their masks past entry 127
+    int defer_words = 0;
     MlpState mlp;
     int stage = 0;  // 0 state, 1 neighbours, 2 density, 3 gradients, 4 fluxes
     int img_window = 2;
@@ -214,6 +216,8 @@ static HydroArgs hydro_args(mlh_ctx *c) {
     a.mlp_frags = c->mlp.frags;
     a.defer_rec = c->defer_rec;
     a.defer_n = c->defer_n;
+    a.defer_hi = c->defer_hi;
+    a.defer_words = c->defer_words;
     a.collect = c->mlp.collect_stride > 0 ? c->mlp.collect : nullptr;
     a.collect_n = c->mlp.collect_n;
     a.collect_scap = c->mlp.collect_scap;
@@ -234,6 +238,11 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
         return fail(MLH_ERR_INVALID_ARGUMENT, "bad precision");
     if (p->work_order < MLH_ORDER_AUTO || p->work_order > MLH_ORDER_CALLER)
         return fail(MLH_ERR_INVALID_ARGUMENT, "bad work_order");
+    if (p->riemann_seed == MLH_SEED_LEARNED && p->precision == MLH_FP32 &&
+        (int64_t)p->max_interactions + p->max_ghost_interactions > kDeferMaxEntries)
+        return fail(MLH_ERR_INVALID_ARGUMENT,
+                    "FP32 with the learned seed: max_interactions + max_ghost_interactions "
+                    "must not exceed " + std::to_string(kDeferMaxEntries));
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(MLH_ERR_DEVICE, "no HIP device");
@@ -285,10 +294,17 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     if (p->riemann_seed == MLH_SEED_LEARNED && c->prec == MLH_FP32) {
         A_(&c->defer_rec, defer_record_bytes() * N);
         A_(&c->defer_n, sizeof(int));
+        // mask words for entries 128 and up of rows longer than 128
+        c->defer_words = c->stride > 128 ? (c->stride + 63) / 64 - 2 : 0;
+        if (c->defer_words) {
+            A_(&c->defer_hi, sizeof(unsigned long long) * N * (size_t)c->defer_words);
+            hipMemsetAsync(c->defer_hi, 0, sizeof(unsigned long long) * N * c->defer_words, c->s);
+        }
     }
 #undef A_
     hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * kNumCounters, c->s);
-    hipMemsetAsync(c->status, 0, sizeof(int) * 2, c->s);
+    hipMemsetAsync(c->status, 0, sizeof(int), c->s);
+    hipMemsetAsync(c->status + 1, 0xff, sizeof(int), c->s);
     hipMemsetAsync(c->max_counts, 0, sizeof(int) * 2, c->s);
     hipMemsetAsync(c->noi, 0, 4 * N, c->s);
     hipMemsetAsync(c->nog, 0, 4 * N, c->s);
@@ -436,14 +452,28 @@ static int read_order(mlh_ctx *c, std::vector<uint32_t> &perm) {
     return MLH_OK;
 }
 
+// status words of a step or run: cleared here, then error bits are sticky
+// until the next call (a run does not lose an overflow of an early step)
+static int clear_status(mlh_ctx *c) {
+    HIPCHK(hipMemsetAsync(c->status, 0, sizeof(int), c->s));
+    HIPCHK(hipMemsetAsync(c->status + 1, 0xff, sizeof(int), c->s));
+    return MLH_OK;
+}
+
 static int check_status(mlh_ctx *c) {
     int st[2];
     HIPCHK(hipMemcpyAsync(st, c->status, sizeof(st), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipGetLastError());
-    if (st[0] & (1 << 1)) return fail(MLH_ERR_NEIGHBOUR_OVERFLOW, "MAX_NUM_INTERACTIONS exceeded");
+    // a run names the step (the state is the one at its start: the face
+    // kernel of that step and of every later one keeps the state unchanged)
+    const std::string at =
+        st[1] >= 0 ? " in step " + std::to_string(st[1]) + " of the run (state kept from its start)"
+                   : "";
+    if (st[0] & (1 << 1))
+        return fail(MLH_ERR_NEIGHBOUR_OVERFLOW, "MAX_NUM_INTERACTIONS exceeded" + at);
     if (st[0] & (1 << 3))
-        return fail(MLH_ERR_GHOST_OVERFLOW, "MAX_NUM_GHOST_INTERACTIONS exceeded");
+        return fail(MLH_ERR_GHOST_OVERFLOW, "MAX_NUM_GHOST_INTERACTIONS exceeded" + at);
     return MLH_OK;
 }
 
@@ -492,7 +522,6 @@ static int ensure_events(mlh_ctx *c, int nsteps) {
 
 static int enqueue_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip,
                          const double *dt_in, double *dt_hist, int step, hipEvent_t *ev) {
-    hipMemsetAsync(c->status, 0, sizeof(int), c->s);
     if (ev) hipEventRecord(ev[0], c->s);
     int st = enqueue_neighbours(c);
     if (st) return st;
@@ -532,6 +561,7 @@ int mlh_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip, double *d
     unsigned long long before[kNumCounters], after[kNumCounters];
     if ((st = read_counters(c, before))) return st;
     if ((st = ensure_events(c, 1))) return st;
+    if ((st = clear_status(c))) return st;
     if ((st = enqueue_step(c, dt_fixed, t_now, t_clip, nullptr, nullptr, 0, c->evpool.data())))
         return st;
     double dtv[3];
@@ -565,10 +595,11 @@ int mlh_run(mlh_ctx *c, int32_t n_steps, double dt_fixed, const double *dts_in,
     if ((st = ensure_events(c, n_steps))) return st;
     unsigned long long before[kNumCounters], after[kNumCounters];
     if ((st = read_counters(c, before))) return st;
+    if ((st = clear_status(c))) return st;
     for (int k = 0; k < n_steps; ++k)
         if ((st = enqueue_step(c, dt_fixed, -1.0, -1.0, dev_in, c->dt_hist, k, &c->evpool[5 * k])))
             return st;
-    if ((st = check_status(c))) return st;
+    if ((st = check_status(c))) { c->stage = 0; return st; }
     if (n_steps > 0) collect_timings(c, n_steps);
     if (dts_out && n_steps > 0)
         HIPCHK(hipMemcpy(dts_out, c->dt_hist, sizeof(double) * n_steps, hipMemcpyDeviceToHost));
@@ -580,8 +611,8 @@ int mlh_run(mlh_ctx *c, int32_t n_steps, double dt_fixed, const double *dts_in,
 
 int mlh_neighbours(mlh_ctx *c) {
     if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
-    HIPCHK(hipMemsetAsync(c->status, 0, sizeof(int), c->s));
-    int st = enqueue_neighbours(c);
+    int st = clear_status(c);
+    if (!st) st = enqueue_neighbours(c);
     if (!st) st = check_status(c);
     c->stage = st ? 0 : 1;
     return st;

@@ -33,7 +33,7 @@ void launch_gradients(const HydroArgs &a, int D, int precision, hipStream_t s) {
 void launch_dt_finalize(const HydroArgs &a, double dt_fixed, double t_now, double t_clip,
                         const double *dt_in, double *dt_hist, int step, hipStream_t s) {
     hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, s, a.dt_block, dt_blocks(a.N),
-                       dt_fixed, t_now, t_clip, dt_in, a.dt, dt_hist, step);
+                       dt_fixed, t_now, t_clip, dt_in, a.dt, dt_hist, step, a.status);
 }
 void launch_faces(const HydroArgs &a, int D, int precision, hipStream_t s) {
     dim3 grid((a.N + 255) / 256);

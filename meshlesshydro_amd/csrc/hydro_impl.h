@@ -467,7 +467,8 @@ __global__ __launch_bounds__(256) void k_dt_finalize(const double *__restrict__ 
                                                      double t_now, double t_clip,
                                                      const double *__restrict__ dt_in,
                                                      double *__restrict__ dt,
-                                                     double *__restrict__ dt_hist, int step) {
+                                                     double *__restrict__ dt_hist, int step,
+                                                     int *__restrict__ status) {
     __shared__ double sh[256];
     double v = DBL_MAX;
     for (int k = threadIdx.x; k < nblocks; k += 256) {
@@ -496,6 +497,8 @@ __global__ __launch_bounds__(256) void k_dt_finalize(const double *__restrict__ 
         dt[0] = d;
         dt[2] = clipped;
         if (dt_hist) dt_hist[step] = d;
+        // the first step of the run whose neighbour search overflowed
+        if ((status[0] & kStatusErrMask) && status[1] < 0) status[1] = step;
     }
 }
 #endif
@@ -970,18 +973,33 @@ __device__ inline void face_accumulate(const HydroArgs &a, const FacePrep<D, T> 
     acc.e += sgn * (double)F[1];
 }
 
-// FP32 learned: a particle with faces whose probe failed (bits of d0 / d1 =
-// neighbour entries 0..127) leaves the face loop with the sums of its other
-// faces in one of these records; k_faces_deferred solves the recorded faces
-// with the network's seed, adds them in entry order and finishes the
-// particle.
+// FP32 learned: a particle with faces whose probe failed leaves the face
+// loop with the sums of its other faces in one of these records; bits of
+// d0 / d1 are its deferred neighbour entries 0..127, and hi = 1 when entries
+// from 128 on are deferred too (their bits are in HydroArgs::defer_hi).
+// k_faces_deferred solves the recorded faces with the network's seed, adds
+// them in entry order and finishes the particle.  Whether a face is
+// deferred depends on the face alone (the probe runs on the solving side's
+// operands), so both sides of a face defer it, whatever its entry number.
 struct DeferRec {
-    int t, pad;
+    int t, hi;
     unsigned long long d0, d1;
     double m, e, v[3];
 };
 static_assert(sizeof(DeferRec) == 64, "DeferRec is one 64-byte record");
 
+
+// the state of particle i carried over unchanged (a step whose neighbour
+// search overflowed)
+template <int D>
+__device__ inline void particle_keep(const HydroArgs &a, int i) {
+    a.mn[i] = a.m[i];
+    a.un[i] = a.u[i];
+    for (int ax = 0; ax < D; ++ax) {
+        a.vn[ax][i] = a.v[ax][i];
+        a.xn[ax][i] = a.x[ax][i];
+    }
+}
 
 // Flux sums and updateStateAndPosition (Particles.cpp:2013-2110) of the
 // particle at work position t (original index i).
@@ -1067,6 +1085,12 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     FaceStats fs{};
     const int N = a.N;
     const int i = live ? (a.sidx ? (int)a.sidx[t] : t) : 0;
+    if (a.status[0] & kStatusErrMask) {
+        // this step's neighbour search overflowed (the reference exits in
+        // gridNNS / ghostNNS): the state is carried over unchanged
+        if (live) particle_keep<D>(a, i);
+        return;
+    }
     const double dtd = a.dt[0];
     const T dt = T(dtd);
     int cnt = live ? a.noi[t] + a.nog[t] : 0;
@@ -1080,6 +1104,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     const bool collect = a.collect && live && (i % a.collect_stride) == 0;
     FluxSums acc{0., 0., {0., 0., 0.}};
     uint64_t defer[2] = {0ull, 0ull};
+    bool defer_hi = false;
     // software pipeline of the face's first two loads: row entries two
     // faces ahead, the neighbour's original index one face ahead, so a face
     // waits only on its records
@@ -1118,10 +1143,16 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
             if (__ballot(strong)) seed = face_net_seed<D, T>(a, strong, fp, lds[threadIdx.x >> 6], fs);
         }
         if (act) {
-            if (LEARNED && kFp32 && mp.valid && !mp.accepted &&
-                mp.chg > kSeedGate && n < 128) {
-                // the standard guess was far off: Newton from the network's seed
-                defer[n >> 6] |= 1ull << (n & 63);
+            if (LEARNED && kFp32 && mp.valid && !mp.accepted && mp.chg > kSeedGate) {
+                // the standard guess was far off: Newton from the network's
+                // seed.  Entries 0..127 are marked in registers, later ones
+                // (rows longer than 128) in this particle's defer_hi words
+                if (n < 128) {
+                    defer[n >> 6] |= 1ull << (n & 63);
+                } else {
+                    a.defer_hi[(size_t)((n >> 6) - 2) * N + t] |= 1ull << (n & 63);
+                    defer_hi = true;
+                }
                 continue;
             }
             face_accumulate<D, T>(a, fp, mp, seed, self_lo, it0, collect, acc, fs);
@@ -1133,7 +1164,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
         // wavefront for the slots; the particle is finished by
         // k_faces_deferred (record order does not matter: each record is
         // completed on its own)
-        const bool dfr = live && (defer[0] | defer[1]) != 0ull;
+        const bool dfr = live && ((defer[0] | defer[1]) != 0ull || defer_hi);
         const unsigned long long bal = __ballot(dfr);
         if (bal) {
             const int lane = lane_id();
@@ -1145,7 +1176,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
                 const int slot = base + __popcll(bal & ((1ull << lane) - 1ull));
                 DeferRec *r = (DeferRec *)a.defer_rec + slot;
                 r->t = t;
-                r->pad = 0;
+                r->hi = defer_hi ? 1 : 0;
                 r->d0 = defer[0];
                 r->d1 = defer[1];
                 r->m = acc.m;
@@ -1161,23 +1192,26 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
 
 // FP32 learned seed: the deferred faces of the particles recorded by
 // k_faces.  A wavefront takes kDeferGroup records (one per owner lane),
-// packs their deferred faces into an LDS queue (owner lane, entry) in owner
-// order and ascending entry, and solves them 64 per round with every lane
-// busy: face recomputed from the solving side, one BF16 MFMA inference for
-// the round's faces, solve from the network's seed (FP32 Newton, FP64
-// acceptance, FP64 fallback).  Each owner then adds its faces' fluxes in
-// queue order -- its entries in ascending order, after the sums of its other
-// faces -- and finishes the particle, so the result does not depend on how
-// records are grouped.  Grid-stride over the records, the count read from
-// the device.
-constexpr int kDeferGroup = 16;                    // records per wavefront
-constexpr int kDeferQueue = kDeferGroup * 128;     // entries per wavefront
+// lays their deferral masks out in LDS, numbers their deferred faces in
+// owner order and ascending entry (a prefix sum of the owners' counts), and
+// solves them 64 per round with every lane busy: lane e of a round decodes
+// queue position e to (owner, entry) -- the owner from the prefix sums, the
+// entry as the k-th set bit of the owner's mask -- recomputes the face from
+// the solving side, and solves it from the network's seed (one BF16 MFMA
+// inference for the round's faces; FP32 Newton, FP64 acceptance, FP64
+// fallback).  Each owner then adds its faces' fluxes in queue order -- its
+// entries in ascending order, after the sums of its other faces -- and
+// finishes the particle, so the result does not depend on how records are
+// grouped.  Grid-stride over the records, the count read from the device.
+constexpr int kDeferGroup = 16;                     // records per wavefront
+constexpr int kDeferWords = kDeferMaxEntries / 64;  // mask words per owner
 
 template <int D>
 __global__ __launch_bounds__(256) void k_faces_deferred(HydroArgs a) {
     constexpr int NF = D + 2;
     __shared__ uint16_t lds[4][64 * 16];
-    __shared__ uint16_t queue[4][kDeferQueue];
+    __shared__ unsigned long long msk[4][kDeferGroup][kDeferWords];
+    __shared__ int qoff[4][kDeferGroup];
     __shared__ float res[4][64 * NF];
     __shared__ int own_t[4][kDeferGroup], own_i[4][kDeferGroup];
     const int w = threadIdx.x >> 6;
@@ -1186,43 +1220,48 @@ __global__ __launch_bounds__(256) void k_faces_deferred(HydroArgs a) {
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
     const float dt = (float)a.dt[0];
+    const int nwords = 2 + a.defer_words;  // <= kDeferWords (checked at create)
     const DeferRec *recs = (const DeferRec *)a.defer_rec;
     for (int base = wave * kDeferGroup; base < count; base += nwaves * kDeferGroup) {
         const int r = base + lane;
         const bool owner = lane < kDeferGroup && r < count;
-        int t = 0, i = 0;
-        unsigned long long d0 = 0ull, d1 = 0ull;
+        int i = 0, c = 0;
         FluxSums acc{0., 0., {0., 0., 0.}};
         if (owner) {
             const DeferRec q = recs[r];
-            t = q.t;
+            const int t = q.t;
             i = a.sidx ? (int)a.sidx[t] : t;
-            d0 = q.d0;
-            d1 = q.d1;
             acc.m = q.m;
             acc.e = q.e;
             for (int ax = 0; ax < 3; ++ax) acc.v[ax] = q.v[ax];
+            msk[w][lane][0] = q.d0;
+            msk[w][lane][1] = q.d1;
+            c = __popcll(q.d0) + __popcll(q.d1);
+            for (int k = 2; k < nwords; ++k) {
+                unsigned long long b = 0ull;
+                if (q.hi) {
+                    // read and cleared: the words are zero again for the
+                    // next step's face loop
+                    unsigned long long *p = a.defer_hi + (size_t)(k - 2) * a.N + t;
+                    b = *p;
+                    *p = 0ull;
+                }
+                msk[w][lane][k] = b;
+                c += __popcll(b);
+            }
             own_t[w][lane] = t;
             own_i[w][lane] = i;
         }
         // queue positions: exclusive prefix sum of the owners' face counts
-        const int c = __popcll(d0) + __popcll(d1);
+        // (lanes without a record count 0, so their offset is the total)
         int incl = c;
         for (int o = 1; o < 64; o <<= 1) {
             int v = __shfl_up(incl, o, 64);
             if (lane >= o) incl += v;
         }
         const int excl = incl - c;
-        const int total = __shfl(incl, 63, 64);  // <= kDeferGroup * 128
-        int k = excl;
-        while (d0) {
-            queue[w][k++] = (uint16_t)((lane << 8) | __builtin_ctzll(d0));
-            d0 &= d0 - 1ull;
-        }
-        while (d1) {
-            queue[w][k++] = (uint16_t)((lane << 8) | (64 + __builtin_ctzll(d1)));
-            d1 &= d1 - 1ull;
-        }
+        const int total = __shfl(incl, 63, 64);
+        if (lane < kDeferGroup) qoff[w][lane] = excl;
         wave_lds_sync();
         FaceStats fs{};
         for (int b = 0; b < total; b += 64) {
@@ -1231,9 +1270,22 @@ __global__ __launch_bounds__(256) void k_faces_deferred(HydroArgs a) {
             FacePrep<D, float> fp;
             MixedProbe mp{};
             bool self_lo = true;
-            if (act) {  // the probe was counted by k_faces
-                const uint16_t q = queue[w][e];
-                const int o = q >> 8, n = q & 255;
+            int o = 0;
+            if (act) {
+                // owner: the last one whose offset is <= e (it has a face
+                // at position e, since the next offset is > e)
+                while (o + 1 < kDeferGroup && qoff[w][o + 1] <= e) ++o;
+                // entry: the (e - offset)-th set bit of the owner's mask
+                int k = e - qoff[w][o];
+                int wd = 0;
+                unsigned long long m = msk[w][o][0];
+                for (int pc = __popcll(m); k >= pc; pc = __popcll(m)) {
+                    k -= pc;
+                    m = msk[w][o][++wd];
+                }
+                for (; k > 0; --k) m &= m - 1ull;
+                const int n = wd * 64 + __builtin_ctzll(m);
+                // the probe was counted by k_faces
                 const FaceStats keep = fs;
                 face_of_entry<D, float>(a, own_t[w][o], own_i[w][o], n, dt, fp, mp, self_lo, fs);
                 fs = keep;
@@ -1246,14 +1298,14 @@ __global__ __launch_bounds__(256) void k_faces_deferred(HydroArgs a) {
                 // training data of the owner's particle (every collect_stride-th)
                 if (a.collect)
                     collect_faces<D, float>(a, self_lo && fs.st.iters - it0 >= 2 &&
-                                                   own_i[w][queue[w][e] >> 8] % a.collect_stride == 0,
+                                                   own_i[w][o] % a.collect_stride == 0,
                                             fp);
                 const float sg = self_lo ? 1.f : -1.f;
                 for (int c2 = 0; c2 < NF; ++c2) res[w][lane * NF + c2] = sg * F[c2];
             }
             wave_lds_sync();
             const int q0 = excl > b ? excl : b;
-            const int q1 = k < b + 64 ? k : b + 64;
+            const int q1 = excl + c < b + 64 ? excl + c : b + 64;
             for (int q = q0; q < q1; ++q) {
                 const float *rr = &res[w][(q - b) * NF];
                 acc.m += (double)rr[0];
@@ -1264,6 +1316,8 @@ __global__ __launch_bounds__(256) void k_faces_deferred(HydroArgs a) {
         }
         if (owner) particle_finish<D>(a, i, acc);
         add_counters(a, fs, 0);
+        // the next group's masks and offsets overwrite this group's
+        wave_lds_sync();
     }
 }
 

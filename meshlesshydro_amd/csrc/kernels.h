@@ -17,7 +17,7 @@ struct NbrArgs {
     const int *cstart, *cend;    // cell -> [start, end) of the cell order
     uint32_t *nbr;               // [maxr+maxg][N], work-order rows and entries
     int *noi, *nog;              // per work position
-    int *status;                 // bit 1: real overflow, bit 3: image overflow
+    int *status;                 // [0] bit 1: real overflow, bit 3: image overflow
     int *max_counts;             // [2]
     int img_window;              // virtual-cell reach of the image search
 };
@@ -47,6 +47,10 @@ struct HydroArgs {
     double *dt;                  // [0] step used, device resident
     double *mF, *vF, *eF;        // optional flux sums (write_fluxes)
     unsigned long long *counters;  // mlh_diag counters
+    // [0]: error bits of the neighbour search (kStatusErrMask), sticky over
+    // the steps of one run; [1]: first step of the run that set one (-1:
+    // none).  With an error bit set the face kernel leaves the state as it
+    // is, so a run stops where the reference exits (Particles.cpp:351,2252).
     int *status;
     const struct MlpFrags *mlp_frags;  // learned seed network (device)
     // FP32 learned seed: particles whose faces wait for the network
@@ -54,12 +58,23 @@ struct HydroArgs {
     // their count
     void *defer_rec;
     int *defer_n;
+    // entries 128.. of the deferred faces' bit masks: [defer_words][N] words
+    // by work position (entries 0..127 travel in the record); zero between
+    // steps (k_faces_deferred clears what it reads)
+    unsigned long long *defer_hi;
+    int defer_words;
     double *collect;             // face collection [stripes][scap][6] (or null)
     unsigned long long *collect_n;  // [stripes] fill counters
     long long collect_scap;      // rows per stripe
     int collect_smask;           // stripes - 1 (a power of two)
     int collect_stride;
 };
+
+// error bits of HydroArgs::status[0] (the reference's exit codes 1 and 3)
+constexpr int kStatusErrMask = (1 << 1) | (1 << 3);
+// longest neighbour row (maxr + maxg) the FP32 learned seed's deferral masks
+// cover
+constexpr int kDeferMaxEntries = 2048;
 
 enum CounterIdx {
     kCntNegReal = 0, kCntNegGhost, kCntVacuum, kCntNonconv, kCntMissing,

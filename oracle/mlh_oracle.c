@@ -901,6 +901,30 @@ static void face_states(const orc *o, int i, int j, const double *xj,
     }
 }
 
+/* Test access for the oracle's own pins (tests/test_oracle_mfv.py): the
+ * face states of the pair (i, j), with xj the neighbour's position and
+ * rho / P [N], grads [(D+2) fields][N][D] installed in place of the density
+ * and gradient stages' results.  Velocities and positions are the state. */
+int orc_test_face_states(orc *o, int i, int j, const double *xj, int is_ghost, double dt,
+                         const double *rho, const double *P, const double *grads,
+                         double *WR, double *WL, double *vFrame) {
+    if (!o || i < 0 || j < 0 || i >= o->N || j >= o->N) return MLH_ERR_INVALID_ARGUMENT;
+    memcpy(o->rho, rho, sizeof(double) * o->N);
+    memcpy(o->P, P, sizeof(double) * o->N);
+    memcpy(o->grad, grads, sizeof(double) * (o->D + 2) * o->N * o->D);
+    face_states(o, i, j, xj, is_ghost, dt, WR, WL, vFrame);
+    return 0;
+}
+
+/* pairwiseLimiter alone, with the psi / abs() settings of p */
+double orc_test_pairwise_limiter(const mlh_params *p, double phi0, double phi_i, double phi_j,
+                                 double xijxi_abs, double xjxi_abs) {
+    orc tmp;
+    memset(&tmp, 0, sizeof(tmp));
+    tmp.p = *p;
+    return pairwise_limiter(&tmp, phi0, phi_i, phi_j, xijxi_abs, xjxi_abs);
+}
+
 /* Riemann::Riemann + Riemann::exact + rotateAndProjectFluxes2D/3D
  * (Riemann.cpp:7-229).  Wi = i-state (the solver's left), Wj = j-state. */
 static int riemann_flux(orc *o, const double *Wi_in, const double *Wj_in,

@@ -80,6 +80,15 @@ void orc_riemann_f(double gamma, double rhoL, double uL, double PL,
                    double rhoR, double uR, double PR, double p, double *f,
                    double *df);
 
+/* Test access (the oracle's own pins): compRiemannStatesLR for one pair
+ * (i, j) with given rho, P [N] and gradients [(D+2)][N][D]; WR / WL
+ * [D+2] {rho, P, v...}, vFrame [D].  And pairwiseLimiter alone. */
+int orc_test_face_states(orc *o, int i, int j, const double *xj, int is_ghost, double dt,
+                         const double *rho, const double *P, const double *grads,
+                         double *WR, double *WL, double *vFrame);
+double orc_test_pairwise_limiter(const mlh_params *p, double phi0, double phi_i, double phi_j,
+                                 double xijxi_abs, double xjxi_abs);
+
 /* cubic spline kernel W(r, h) (Particles.cpp:7-24) */
 double orc_kernel(int dim, double r, double h);
 

@@ -58,6 +58,11 @@ def lib():
         L.orc_riemann_pstar.restype = C.c_double
         L.orc_riemann_guess.argtypes = [C.c_double] * 7
         L.orc_riemann_f.argtypes = [C.c_double] * 8 + [_dp, _dp]
+        L.orc_test_face_states.restype = C.c_int
+        L.orc_test_face_states.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_double,
+                                           _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_test_pairwise_limiter.restype = C.c_double
+        L.orc_test_pairwise_limiter.argtypes = [C.POINTER(Params)] + [C.c_double] * 5
         L.orc_kernel.restype = C.c_double
         L.orc_kernel.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_mlp_loss_grad.restype = C.c_double
@@ -192,6 +197,21 @@ class Oracle:
         self.L.orc_get_diag(self.h, C.byref(d))
         return d.as_dict()
 
+    def face_states(self, i, j, xj, dt, rho, P, grads, ghost=False):
+        """compRiemannStatesLR for the pair (i, j) with the given rho, P [N]
+        and gradients [D+2][N][D] in place of the stages' results: returns
+        WR (i's state), WL (j's state), vFrame."""
+        D = self.D
+        xj = np.ascontiguousarray(xj, np.float64)
+        rho = np.ascontiguousarray(rho, np.float64)
+        P = np.ascontiguousarray(P, np.float64)
+        grads = np.ascontiguousarray(grads, np.float64)
+        assert grads.shape == (D + 2, self.N, D)
+        WR, WL, vF = np.zeros(D + 2), np.zeros(D + 2), np.zeros(D)
+        self._chk(self.L.orc_test_face_states(self.h, i, j, _d(xj), int(ghost), dt, _d(rho), _d(P),
+                                              _d(grads), _d(WR), _d(WL), _d(vF)), "face_states")
+        return WR, WL, vF
+
     def set_bruteforce_ghosts(self, on=True):
         self.L.orc_set_bruteforce_ghosts(self.h, int(on))
 
@@ -220,6 +240,12 @@ def riemann_f(gamma, rl, ul, pl, rr, ur, pr, p):
     f, df = C.c_double(), C.c_double()
     lib().orc_riemann_f(gamma, rl, ul, pl, rr, ur, pr, p, C.byref(f), C.byref(df))
     return f.value, df.value
+
+
+def pairwise_limiter(params, phi0, phi_i, phi_j, xijxi_abs, xjxi_abs):
+    """pairwiseLimiter (Particles.cpp:1735-1785) with params' PSI_1, PSI_2
+    and abs() mode."""
+    return lib().orc_test_pairwise_limiter(C.byref(params), phi0, phi_i, phi_j, xijxi_abs, xjxi_abs)
 
 
 def kernel(dim, r, h):

@@ -1,0 +1,178 @@
+"""The learned p* seed inside the MFV step, against the oracle.
+
+The step's faces are solved by solveRiemannProblems (Particles.cpp:1787-1911)
+with Riemann::exact (Riemann.cpp:83-141).  With the learned seed, Newton
+starts from the network's p* instead of the standard guess and is accepted by
+the same FP64 relative-change test, so the step must still match the oracle,
+which solves every face from the standard guess:
+
+* FP64 with the seed (k_faces<D, double, true>: the network runs inside the
+  face loop for faces with a real jump) within the FP64 bars;
+* FP32 with the seed (k_faces<D, float, true> + k_faces_deferred: faces whose
+  probe at the standard guess fails are deferred and solved from the
+  network's seed) within the FP32 bars;
+* with rows longer than 128 entries, where deferred faces sit at entries
+  past the register masks, conservation to round-off (both sides of a face
+  must defer it and solve it identically).
+
+The network is trained on the blast's own FP64 faces, and every test asserts
+that faces were actually seeded.
+"""
+import numpy as np
+import pytest
+
+from cases import assert_fp32_fluxes, blast3d, interior, relerr
+from meshlesshydro_amd import learned as L
+from meshlesshydro_amd._abi import MLH_FP32, MLH_FP64, MLH_SEED_LEARNED
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 3
+
+
+def _trained(hip, p64, s):
+    """Seed network trained on the faces of the blast's first two FP64 steps."""
+    faces = L.collect_faces(p64, s, steps=2, stride=1)
+    sim = hip.Simulation(p64, s)
+    w, info = L.train_seed_from_faces(sim, faces, p64.gamma, n_epochs=4)
+    sim.close()
+    print(f"[learned] {len(faces)} faces, val MSE {info['val_mse_log_pstar']:.3e} "
+          f"vs standard {info['val_mse_standard_guess']:.3e}", flush=True)
+    return w
+
+
+@pytest.fixture(scope="module")
+def blast(hip):
+    p64, s = blast3d(16)
+    return p64, s, _trained(hip, p64, s)
+
+
+def _oracle_run(oracle_mod, p, s, steps):
+    orc = oracle_mod.Oracle(p, s)
+    dts = [orc.step() for _ in range(steps)]
+    return orc, np.array(dts)
+
+
+def test_fp64_learned_steps_vs_oracle(blast, hip, oracle_mod):
+    """k_faces<3, double, true>: three steps of the 16^3 blast with the
+    oracle's dt sequence, state within 1e-9 of the oracle, mass conserved to
+    round-off, and faces seeded by the network."""
+    p64, s, w = blast
+    orc, dts = _oracle_run(oracle_mod, p64, s, STEPS)
+    pl, _ = blast3d(16)
+    pl.riemann_seed = MLH_SEED_LEARNED
+    print("\n[learned FP64] blast16, 3 steps", flush=True)
+    sim = hip.Simulation(pl, s)
+    sim.mlp_set_params(w)
+    c0 = sim.conserved()
+    sim.run(STEPS, dts_in=dts)
+    d = sim.diag()
+    print(f"[learned FP64] seeded {d['learned_seeded']} of {d['faces_solved']} faces, "
+          f"fallbacks {d['learned_fallbacks']}", flush=True)
+    assert d["learned_seeded"] > 0
+    gs, os_ = sim.state(), orc.state()
+    for k in gs:
+        assert relerr(gs[k], os_[k]) < 1e-9, (k, relerr(gs[k], os_[k]))
+    c1 = sim.conserved()
+    assert abs(c1[1] - c0[1]) <= 1e-13 * abs(c0[1])
+    sim.close()
+
+
+def test_fp32_learned_fluxes_vs_oracle(blast, hip, oracle_mod):
+    """k_faces<3, float, true> + k_faces_deferred through the stage entry
+    points: one step's flux sums against the oracle's within the FP32 bars,
+    with deferred faces seeded by the network."""
+    p64, s, w = blast
+    p32, _ = blast3d(16, precision=MLH_FP32)
+    p32.riemann_seed = MLH_SEED_LEARNED
+    print("\n[learned FP32] blast16 fluxes", flush=True)
+    sim = hip.Simulation(p32, s)
+    sim.mlp_set_params(w)
+    orc = oracle_mod.Oracle(p64, s)
+    sim.neighbours(); orc.neighbours()
+    sim.density(); orc.density()
+    dt = orc.timestep()
+    sim.gradients(); orc.gradients()
+    sim.fluxes(dt); orc.fluxes(dt)
+    d = sim.diag()
+    print(f"[learned FP32] seeded {d['learned_seeded']} of {d['faces_solved']} faces", flush=True)
+    assert d["learned_seeded"] > 0
+    assert_fp32_fluxes(p64, orc, sim, interior(p64, s), "blast16 learned")
+    sim.close()
+
+
+def test_fp32_learned_steps_vs_oracle(blast, hip, oracle_mod):
+    """Three FP32 steps with the learned seed and the oracle's dt sequence:
+    relative L1 of m, u and v against the oracle below 1e-5 (the FP32 step
+    bar), the same state as FP32 with the standard guess to the solver
+    tolerance, mass conserved to round-off."""
+    p64, s, w = blast
+    orc, dts = _oracle_run(oracle_mod, p64, s, STEPS)
+    want = orc.state()
+    out = {}
+    for name, seeded in (("standard", False), ("learned", True)):
+        p32, _ = blast3d(16, precision=MLH_FP32)
+        if seeded:
+            p32.riemann_seed = MLH_SEED_LEARNED
+        print(f"\n[FP32 {name}] blast16, 3 steps", flush=True)
+        sim = hip.Simulation(p32, s)
+        if seeded:
+            sim.mlp_set_params(w)
+        c0 = sim.conserved()
+        sim.run(STEPS, dts_in=dts)
+        c1 = sim.conserved()
+        assert abs(c1[1] - c0[1]) <= 1e-13 * abs(c0[1]), name
+        out[name] = (sim.state(), sim.diag())
+        sim.close()
+    st, d = out["learned"]
+    assert d["learned_seeded"] > 0
+    for k in ("m", "u", "vx", "vy", "vz"):
+        l1 = np.sum(np.abs(st[k] - want[k])) / np.sum(np.abs(want[k]))
+        assert l1 < 1e-5, (k, l1)
+        l1s = np.sum(np.abs(st[k] - out["standard"][0][k])) / np.sum(np.abs(want[k]))
+        assert l1s < 1e-6, (k, l1s)
+
+
+def test_fp32_learned_long_rows_conserve(hip, oracle_mod):
+    """Rows of ~165 neighbours (h = 3.4 dx): deferred faces lie past entry
+    127 of some rows.  Deferral depends on the face only, so both sides defer
+    it and solve it from the same seed: F_ji = -F_ij, and mass is conserved
+    to round-off over three steps; the state matches the oracle within the
+    FP32 step bar."""
+    kw = dict(kernel_size=3.4 / 16, max_interactions=256, max_ghost_interactions=256)
+    p64, s = blast3d(16, **kw)
+    # faces of the hot centre (pressure ratio > 1e3) at entry 128 or later
+    sim = hip.Simulation(p64, s)
+    sim.neighbours()
+    sim.density()
+    noi, nog, nbr = sim.get_neighbours()
+    _, P, _ = sim.get_density()
+    sim.close()
+    idx = nbr & ((1 << 27) - 1)
+    n = np.arange(nbr.shape[1])[None, :]
+    live = n < (noi + nog)[:, None]
+    ratio = np.maximum(P[:, None] / P[idx], P[idx] / P[:, None])
+    far = live & (n >= 128) & (ratio > 1e3)
+    print(f"\n[long rows] max row {int((noi + nog).max())}, strong faces past entry 127: "
+          f"{int(far.sum())}", flush=True)
+    assert far.sum() > 0
+    w = _trained(hip, p64, s)
+    orc, dts = _oracle_run(oracle_mod, p64, s, STEPS)
+    want = orc.state()
+    p32, _ = blast3d(16, precision=MLH_FP32, **kw)
+    p32.riemann_seed = MLH_SEED_LEARNED
+    sim = hip.Simulation(p32, s)
+    sim.mlp_set_params(w)
+    c0 = sim.conserved()
+    sim.run(STEPS, dts_in=dts)
+    c1 = sim.conserved()
+    d = sim.diag()
+    print(f"[long rows] seeded {d['learned_seeded']} of {d['faces_solved']} faces, "
+          f"mass drift {abs(c1[1] - c0[1]) / c0[1]:.2e}", flush=True)
+    assert d["learned_seeded"] > 0
+    assert abs(c1[1] - c0[1]) <= 1e-13 * abs(c0[1])
+    st = sim.state()
+    for k in ("m", "u", "vx"):
+        l1 = np.sum(np.abs(st[k] - want[k])) / np.sum(np.abs(want[k]))
+        assert l1 < 1e-5, (k, l1)
+    sim.close()

@@ -9,7 +9,7 @@ the Riemann solver between the GPU's math library and glibc).
 import numpy as np
 import pytest
 
-from cases import CASES, kh_lattice, relerr
+from cases import CASES, assert_fp32_fluxes, kh_lattice, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -165,20 +165,7 @@ def test_stages_fp32(name, hip, oracle_mod):
             assert np.mean(err[inner] > 1e-5) < 0.005, (name, f, np.mean(err[inner] > 1e-5))
         assert np.max(err) < 2.0, (name, f, np.max(err))
     sim.fluxes(dt); orc.fluxes(dt)
-    gm, gv, ge = sim.get_fluxes()
-    om, ov, oe = orc.get_fluxes()
-    # flux sums cancel (a steady region has net flux ~0), so the FP32 error
-    # is measured against the size of one face's flux: rho (|v| + c) h^(D-1)
-    st = orc.state()
-    vmax = max(np.max(np.abs(st[k])) for k in ("vx", "vy", "vz")[: p.dim])
-    cmax = np.max(np.sqrt(p.gamma * oP / orr))
-    fm = np.max(orr) * (vmax + cmax) * p.kernel_size ** (p.dim - 1)
-    for a, b, scale in ((gm, om, fm), (gv, ov, fm * (vmax + cmax) + np.max(oP) * p.kernel_size ** (p.dim - 1)),
-                        (ge, oe, fm * (vmax + cmax) ** 2 + np.max(oP) * (vmax + cmax) * p.kernel_size ** (p.dim - 1))):
-        err = np.abs(a - b).reshape(len(gm), -1).max(axis=1) / scale
-        if inner.sum() >= 50:
-            assert np.mean(err[inner] > 1e-5) < 0.01, (name, np.mean(err[inner] > 1e-5))
-        assert np.max(err) < 1.0, (name, np.max(err))
+    assert_fp32_fluxes(p, orc, sim, inner, name)
     sim.close()
 
 
@@ -280,4 +267,38 @@ def test_reference_case_hundred_steps(hip, oracle_mod):
     gs, os_ = sim.state(), orc.state()
     for k in gs:
         assert relerr(gs[k], os_[k]) < 1e-10, k
+    sim.close()
+
+
+def test_overflow_in_a_later_step_of_a_run(hip, oracle_mod):
+    """An overflow of MAX_NUM_INTERACTIONS that first happens in the third
+    step of a run (a converging flow raises the counts 48 -> 50 -> 52 with
+    a limit of 51) ends the run with the reference's exit code 1
+    (Particles.cpp:348-351), names that step, and leaves the state of the
+    start of that step -- the oracle's state after two steps -- as the
+    reference, which exits inside gridNNS, does."""
+    p, s = kh_lattice(40, max_interactions=51)
+    s = dict(s)
+    s["vx"] = -0.8 * np.sin(2 * np.pi * s["x"])
+    s["vy"] = np.zeros_like(s["vx"])
+    orc = oracle_mod.Oracle(p, s)
+    orc.step()
+    orc.step()
+    want = orc.state()
+    with pytest.raises(oracle_mod.OracleError) as e0:
+        orc.step()
+    assert e0.value.status == 1
+    print("\n[overflow in step 2 of a 5-step run]", flush=True)
+    sim = hip.Simulation(p, s)
+    with pytest.raises(hip.MLHError) as e:
+        sim.run(5)
+    assert e.value.status == 1
+    assert "in step 2 of the run" in str(e.value), str(e.value)
+    got = sim.state()
+    for k in got:
+        assert relerr(got[k], want[k]) < 1e-10, k
+    assert sim.diag()["max_neighbours"] == p.max_interactions
+    # the context is usable again: a state within the limits steps cleanly
+    sim.upload(kh_lattice(40)[1])
+    sim.run(2)
     sim.close()

@@ -12,18 +12,58 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "toro_kats.json")
 
 
+def _sample_at_zero(g, left, right, star):
+    """The exact solution at x/t = 0 from the star values (Toro 3rd ed.,
+    section 4.5: the sampled side is the one of the contact u*; on it, a
+    shock (p* > p_K) or a rarefaction head / tail decides between the
+    unshocked, the fan and the star state)."""
+    (dl, ul, pl), (dr, ur, pr) = left, right
+    ps, us = star["p_star"], star["u_star"]
+    if 0.0 <= us:  # left of the contact
+        d, u, p, sgn, dstar = dl, ul, pl, 1.0, star["rho_star_L"]
+    else:
+        d, u, p, sgn, dstar = dr, ur, pr, -1.0, star["rho_star_R"]
+    c = np.sqrt(g * p / d)
+    if ps > p:  # shock: speed u -+ c sqrt((g+1)/(2g) p*/p + (g-1)/(2g))
+        s = u - sgn * c * np.sqrt((g + 1) / (2 * g) * ps / p + (g - 1) / (2 * g))
+        return (d, u, p) if sgn * s >= 0.0 else (dstar, us, ps)
+    head = u - sgn * c
+    tail = us - sgn * c * (ps / p) ** ((g - 1) / (2 * g))
+    if sgn * head >= 0.0:
+        return d, u, p
+    if sgn * tail <= 0.0:
+        return dstar, us, ps
+    # inside the fan
+    f = 2 / (g + 1) + sgn * (g - 1) / ((g + 1) * c) * u
+    dd = d * f ** (2 / (g - 1))
+    return dd, 2 / (g + 1) * (sgn * c + (g - 1) / 2 * u), p * f ** (2 * g / (g - 1))
+
+
 def test_toro_kats_gpu(hip):
+    """Toro's tests 1-5 (star values re-derived by an independent
+    bisection): the GPU's FP64 star pressure, and the state sampled at
+    x/t = 0 by the batch solver in FP64 and in FP32 (mixed) mode, for every
+    test.  Test 5 samples the unshocked left state (its left shock moves
+    right); tests 1-4 sample a star state."""
     with open(GOLD) as fh:
         kats = json.load(fh)
     g = kats["gamma"]
+    names = []
     for t in kats["tests"]:
         (dl, ul, pl), (dr, ur, pr) = t["left"], t["right"]
         b = t["bisection"]
-        for prec in (MLH_FP64, MLH_FP32):
-            rs, us, ps, fl, it = hip.riemann_batch(g, [dl], [ul], [pl], [dr], [ur], [pr], precision=prec, tol=1e-6)
-            tol = 1e-6 if prec == MLH_FP64 else 2e-5
-            if t["name"] in ("T1", "T2"):
-                assert abs(ps[0] - b["p_star"]) <= tol * b["p_star"], (t["name"], prec)
+        ps, _ = hip.riemann_pstar(g, [dl], [ul], [pl], [dr], [ur], [pr], tol=1e-12)
+        assert abs(ps[0] - b["p_star"]) <= 1e-10 * b["p_star"], (t["name"], ps[0])
+        want = _sample_at_zero(g, t["left"], t["right"], b)
+        cs = max(np.sqrt(g * pl / dl), np.sqrt(g * pr / dr))
+        for prec, tol in ((MLH_FP64, 1e-6), (MLH_FP32, 2e-5)):
+            rs, us, Ps, fl, it = hip.riemann_batch(g, [dl], [ul], [pl], [dr], [ur], [pr],
+                                                   precision=prec, tol=1e-6)
+            assert abs(Ps[0] - want[2]) <= tol * want[2], (t["name"], prec, Ps[0], want)
+            assert abs(rs[0] - want[0]) <= tol * want[0], (t["name"], prec, rs[0], want)
+            assert abs(us[0] - want[1]) <= tol * (abs(want[1]) + cs), (t["name"], prec, us[0], want)
+        names.append(t["name"])
+    assert names == ["T1", "T2", "T3", "T4", "T5"]
 
 
 def test_random_states_vs_oracle(hip, oracle_mod):
