@@ -22,8 +22,34 @@ def kh_random(n=48, **kw):
 def blast3d(n=16, **kw):
     kw.setdefault("max_interactions", 128)
     kw.setdefault("max_ghost_interactions", 128)
-    p = default_params(dim=3, kernel_size=ics.blast_kernel_size(n), **kw)
+    kw.setdefault("kernel_size", ics.blast_kernel_size(n))
+    p = default_params(dim=3, **kw)
     return p, ics.blast_3d(n)
+
+
+def first_seeded_state(hip, p64, p32, state, weights, max_steps=4):
+    """The first state along the FP64 run from `state` (at most max_steps
+    CFL steps in) whose FP32 learned-seed flux stage seeds faces with the
+    network.  At the blast's first step every face's standard guess passes
+    the probe, so nothing is seeded there.  Returns (state, seeded faces,
+    steps taken)."""
+    st = state
+    for k in range(max_steps + 1):
+        s32 = hip.Simulation(p32, st)
+        s32.mlp_set_params(weights)
+        s32.neighbours()
+        s32.density()
+        dt = s32.timestep()
+        s32.gradients()
+        s32.fluxes(dt)
+        seeded = s32.diag()["learned_seeded"]
+        s32.close()
+        if seeded > 0 or k == max_steps:
+            return st, seeded, k
+        s64 = hip.Simulation(p64, st)
+        s64.step()
+        st = s64.state()
+        s64.close()
 
 
 def jitter3d(n=12, seed=11, **kw):

@@ -3,7 +3,8 @@ the shear layer at its full 1000^2, and the decaying turbulent box at 20^3.
 
 * 128^3 blast (2,097,152 particles, 96 + 96 list limits): one FP64 step
   stage by stage (neighbour lists bit-exact, gridNNS Particles.cpp:324-365
-  and the image search; the other stages to the FP64 bars), one FP32 step's
+  and the image search; the other stages to the FP64 bars), then, on the
+  first state of the run whose faces the network seeds, one FP32 step's
   flux sums with the standard and with the learned seed within the FP32
   bars (the seed trained on the blast's own FP64 faces, seeding asserted).
 * 1000^2 shear layer (10^6 particles, the reference's list limits): lists
@@ -16,7 +17,7 @@ the shear layer at its full 1000^2, and the decaying turbulent box at 20^3.
 import numpy as np
 import pytest
 
-from cases import assert_fp32_fluxes, interior, relerr
+from cases import assert_fp32_fluxes, first_seeded_state, interior, relerr
 from meshlesshydro_amd import ics
 from meshlesshydro_amd import learned as L
 from meshlesshydro_amd import turbulence as T
@@ -91,11 +92,29 @@ def test_blast_128_full_size(hip, oracle_mod):
     sim.fluxes(dt); orc.fluxes(dt)
     for a, b in zip(sim.get_fluxes(), orc.get_fluxes()):
         assert relerr(a, b) < FP64_TOL
-    # FP32 with the standard and the learned seed, the same dt
-    inner = interior(p, s)
+    sim.update(); orc.update(dt)
+    gs, os_ = sim.state(), orc.state()
+    for k in gs:
+        assert relerr(gs[k], os_[k]) < 1e-12, k
+    sim.close()
+    del orc
+    # FP32 with the standard and the learned seed on the first state of the
+    # run whose faces the network seeds (none at the first step: every
+    # standard guess passes the probe there), the oracle's dt from that state
+    p32 = default_params(precision=MLH_FP32, riemann_seed=MLH_SEED_LEARNED, **kw)
+    st, seeded, k = first_seeded_state(hip, p, p32, s, weights)
+    print(f"[blast128] FP32 fluxes after {k} FP64 steps ({seeded} faces seeded)", flush=True)
+    assert seeded > 0
+    orc = oracle_mod.Oracle(p, st)
+    orc.neighbours()
+    orc.density()
+    dt = orc.timestep()
+    orc.gradients()
+    orc.fluxes(dt)
+    inner = interior(p, st)
     for name, w in (("standard", None), ("learned", weights)):
         print(f"[blast128] FP32 {name} fluxes", flush=True)
-        s32 = _fp32_sim(kw, s, w)
+        s32 = _fp32_sim(kw, st, w)
         _stages_to_fluxes(s32, dt)
         assert_fp32_fluxes(p, orc, s32, inner, "blast128 " + name)
         if w is not None:
@@ -103,11 +122,6 @@ def test_blast_128_full_size(hip, oracle_mod):
             print(f"[blast128] seeded {d['learned_seeded']} of {d['faces_solved']} faces", flush=True)
             assert d["learned_seeded"] > 0
         s32.close()
-    sim.update(); orc.update(dt)
-    gs, os_ = sim.state(), orc.state()
-    for k in gs:
-        assert relerr(gs[k], os_[k]) < 1e-12, k
-    sim.close()
 
 
 def test_shear_layer_1000_full_size(hip, oracle_mod):

@@ -21,7 +21,7 @@ that faces were actually seeded.
 import numpy as np
 import pytest
 
-from cases import assert_fp32_fluxes, blast3d, interior, relerr
+from cases import assert_fp32_fluxes, blast3d, first_seeded_state, interior, relerr
 from meshlesshydro_amd import learned as L
 from meshlesshydro_amd._abi import MLH_FP32, MLH_FP64, MLH_SEED_LEARNED
 
@@ -81,11 +81,17 @@ def test_fp64_learned_steps_vs_oracle(blast, hip, oracle_mod):
 def test_fp32_learned_fluxes_vs_oracle(blast, hip, oracle_mod):
     """k_faces<3, float, true> + k_faces_deferred through the stage entry
     points: one step's flux sums against the oracle's within the FP32 bars,
-    with deferred faces seeded by the network."""
-    p64, s, w = blast
+    with deferred faces seeded by the network.  The step is the first one of
+    the blast whose faces get seeded (at the first step every face's
+    standard guess passes the probe); oracle and GPU start from the same
+    state."""
+    p64, s0, w = blast
     p32, _ = blast3d(16, precision=MLH_FP32)
     p32.riemann_seed = MLH_SEED_LEARNED
-    print("\n[learned FP32] blast16 fluxes", flush=True)
+    s, seeded, k = first_seeded_state(hip, p64, p32, s0, w)
+    print(f"\n[learned FP32] blast16 fluxes after {k} FP64 steps ({seeded} faces seeded)",
+          flush=True)
+    assert seeded > 0
     sim = hip.Simulation(p32, s)
     sim.mlp_set_params(w)
     orc = oracle_mod.Oracle(p64, s)
