@@ -243,6 +243,23 @@ int mlh_riemann_exact_batch_device(mlh_ctx *ctx, int64_t n, double gamma,
                                    int32_t seed, double tol, int64_t *fallbacks,
                                    float *kernel_ms);
 
+/* Allocation-free variant of mlh_riemann_exact_batch_device.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as mlh_riemann_exact_batch_device
+ * except that it reports no kernel time.  It calls no hipMalloc / hipFree:
+ * its scratch (one fallback count per 256 faces and their total, 2 MiB) is
+ * a static device array that each GPU holds until the process ends, so n
+ * above MLH_NOALLOC_MAX_FACES is rejected with MLH_ERR_INVALID_ARGUMENT.
+ * Every call on one GPU uses that GPU's one array: calls that overlap on one
+ * GPU (from several host threads or streams) race. */
+#define MLH_NOALLOC_MAX_FACES (1 << 26)
+int mlh_riemann_exact_batch_device_noalloc(mlh_ctx *ctx, int64_t n, double gamma,
+                                           const double *states, double *star,
+                                           int32_t *flag, int32_t *iters,
+                                           int32_t precision, int32_t seed,
+                                           double tol, int64_t *fallbacks);
+
 /* ---- one face flux, batched (Riemann surface) ---------------------------
  * Riemann::Riemann(WR, WL, vFrame, Aij, i) + Riemann::exact(Fij, gamma)
  * (Riemann.h:19,26; Riemann.cpp:7-229) for n given faces, FP64, with ctx's

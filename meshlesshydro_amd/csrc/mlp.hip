@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
+#include <mutex>
 #include <vector>
 #include "../../include/mlh.h"
 #include "device_common.h"
@@ -195,13 +196,18 @@ __global__ __launch_bounds__(256) void k_mlp_infer_bf16(const MlpFrags *__restri
 }
 
 // ---- batched exact Riemann solver -------------------------------------------
+// Faces the FP64 solver finished: added to fallbacks[0] (atomics), or, with
+// block_counts, block b's count stored in block_counts[b] (every block
+// stores its slot, zero included, so the slots need no clearing).
 __global__ __launch_bounds__(256) void k_riemann_batch(
     int64_t n, double gamma, const double *__restrict__ dl, const double *__restrict__ ul,
     const double *__restrict__ pl, const double *__restrict__ dr, const double *__restrict__ ur,
     const double *__restrict__ pr, double *__restrict__ ds, double *__restrict__ us,
     double *__restrict__ ps, int *__restrict__ flag, int *__restrict__ iters, int precision,
     const MlpFrags *__restrict__ F, double tol, int max_iter, int iters32,
-    unsigned long long *__restrict__ fallbacks, double *__restrict__ pstar) {
+    unsigned long long *__restrict__ fallbacks, double *__restrict__ pstar,
+    unsigned long long *__restrict__ block_counts) {
+    __shared__ unsigned long long wcount[4];
     __shared__ uint16_t lds[4][64 * 16];
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool active = k < n;
@@ -222,31 +228,58 @@ __global__ __launch_bounds__(256) void k_riemann_batch(
         float y = mlp_infer_wave_bf16(R, f, lds[threadIdx.x >> 6]);
         seed = ok ? mlp_seed_from_output(y, f, pref) : -1.f;
     }
-    if (!active) return;
     SolveStats st{};
-    double rd, ru, rp;
-    int fl;
     if (pstar) {
+        if (!active) return;
         GasConst<double> gc(gamma);
         pstar[k] = star_pressure_fp64(gc, a0, a1, a2, b0, b1, b2, tol, max_iter, st);
         if (iters) iters[k] = st.iters;
         return;
     }
-    if (precision == MLH_FP64) {
-        GasConst<double> gc(gamma);
-        RSol<double> r = solve_fp64(gc, a0, a1, a2, b0, b1, b2, tol, max_iter, (double)seed, st);
-        rd = r.d; ru = r.u; rp = r.p; fl = r.flag;
-    } else {
-        GasConst<float> gf(gamma);
-        GasConst<double> gc(gamma);
-        RSol<float> r = solve_mixed(gf, gc, (float)a0, (float)a1, (float)a2, (float)b0,
-                                    (float)b1, (float)b2, tol, max_iter, iters32, seed, st);
-        rd = r.d; ru = r.u; rp = r.p; fl = r.flag;
+    if (active) {
+        double rd, ru, rp;
+        int fl;
+        if (precision == MLH_FP64) {
+            GasConst<double> gc(gamma);
+            RSol<double> r = solve_fp64(gc, a0, a1, a2, b0, b1, b2, tol, max_iter, (double)seed, st);
+            rd = r.d; ru = r.u; rp = r.p; fl = r.flag;
+        } else {
+            GasConst<float> gf(gamma);
+            GasConst<double> gc(gamma);
+            RSol<float> r = solve_mixed(gf, gc, (float)a0, (float)a1, (float)a2, (float)b0,
+                                        (float)b1, (float)b2, tol, max_iter, iters32, seed, st);
+            rd = r.d; ru = r.u; rp = r.p; fl = r.flag;
+        }
+        ds[k] = rd; us[k] = ru; ps[k] = rp;
+        if (flag) flag[k] = fl;
+        if (iters) iters[k] = st.iters;
     }
-    ds[k] = rd; us[k] = ru; ps[k] = rp;
-    if (flag) flag[k] = fl;
-    if (iters) iters[k] = st.iters;
-    if (st.fallback) atomicAdd(fallbacks, (unsigned long long)st.fallback);
+    if (!block_counts) {
+        if (st.fallback) atomicAdd(fallbacks, (unsigned long long)st.fallback);
+        return;
+    }
+    // every thread of the block is here: wavefront sums, then the block's
+    unsigned long long v = (unsigned long long)st.fallback;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) wcount[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        block_counts[blockIdx.x] = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+}
+
+// out[0] = sum of in[0..n) (one block of 256 threads)
+__global__ __launch_bounds__(256) void k_sum_u64(const unsigned long long *__restrict__ in,
+                                                  int n, unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long sh[256];
+    unsigned long long v = 0;
+    for (int k = threadIdx.x; k < n; k += 256) v += in[k];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
 }
 
 }  // namespace mlh
@@ -470,7 +503,7 @@ int mlh_riemann_exact_batch(mlh_ctx *c, int64_t n, double gamma, const double *r
     hipLaunchKernelGGL(k_riemann_batch, dim3((n + 255) / 256), dim3(256), 0, s, n, gamma, buf,
                        buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n, buf + 5 * n, buf + 6 * n,
                        buf + 7 * n, buf + 8 * n, ib, ib + n, precision, F, tol, max_iter, 3, fb,
-                       nullptr);
+                       nullptr, nullptr);
     hipMemcpyAsync(rho_s, buf + 6 * n, b, hipMemcpyDeviceToHost, s);
     hipMemcpyAsync(u_s, buf + 7 * n, b, hipMemcpyDeviceToHost, s);
     hipMemcpyAsync(P_s, buf + 8 * n, b, hipMemcpyDeviceToHost, s);
@@ -534,7 +567,7 @@ int mlh_riemann_exact_batch_device(mlh_ctx *c, int64_t n, double gamma, const do
     hipLaunchKernelGGL(k_riemann_batch, dim3((n + 255) / 256), dim3(256), 0, s, n, gamma, states,
                        states + n, states + 2 * n, states + 3 * n, states + 4 * n, states + 5 * n,
                        star, star + n, star + 2 * n, flag, iters, precision, F, tol, 50, 3, fb,
-                       nullptr);
+                       nullptr, nullptr);
     hipEventRecord(e1, s);
     unsigned long long h_fb = 0;
     hipMemcpyAsync(&h_fb, fb, sizeof(h_fb), hipMemcpyDeviceToHost, s);
@@ -547,6 +580,76 @@ int mlh_riemann_exact_batch_device(mlh_ctx *c, int64_t n, double gamma, const do
     if (e != hipSuccess) return ctx_fail(MLH_ERR_DEVICE, hipGetErrorString(e));
     if (fallbacks) *fallbacks = (int64_t)h_fb;
     if (kernel_ms) *kernel_ms = ms;
+    return MLH_OK;
+}
+
+}  // extern "C"
+
+// ---- the allocation-free batch solve ---------------------------------------
+// Scratch of mlh_riemann_exact_batch_device_noalloc: one fallback count per
+// block of 256 faces, then their total.  A static device array: each GPU
+// the library runs on holds its own copy until the process ends.
+constexpr int kNoAllocBlocks = MLH_NOALLOC_MAX_FACES / 256;
+static __device__ __align__(256) unsigned long long g_batch_scratch[kNoAllocBlocks + 32];
+
+// The array's device address on each GPU, looked up once per device (with
+// that device current) on the entry point's first call there.  In host code
+// the array's name is not a device address.
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned long long *g_scratch_ptr[kMaxDevices];
+
+static int batch_scratch(unsigned long long **out) {
+    int dev = -1;
+    CHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return ctx_fail(MLH_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        CHK(hipGetSymbolAddress(&p, HIP_SYMBOL(g_batch_scratch)));
+        if (!p) return ctx_fail(MLH_ERR_DEVICE, "no device address for the batch scratch");
+        g_scratch_ptr[dev] = (unsigned long long *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MLH_OK;
+}
+
+extern "C" {
+
+int mlh_riemann_exact_batch_device_noalloc(mlh_ctx *c, int64_t n, double gamma,
+                                           const double *states, double *star, int32_t *flag,
+                                           int32_t *iters, int32_t precision, int32_t seed,
+                                           double tol, int64_t *fallbacks) {
+    if (n > MLH_NOALLOC_MAX_FACES)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT,
+                        "n exceeds MLH_NOALLOC_MAX_FACES (the static scratch holds one count per "
+                        "256 faces for at most 2^26 faces); split the batch");
+    if (n < 0 || (n > 0 && (!states || !star)))
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    if (precision != MLH_FP64 && precision != MLH_FP32)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad precision");
+    const MlpFrags *F = nullptr;
+    if (seed == MLH_SEED_LEARNED) {
+        if (!c || !ctx_mlp(c).params) return ctx_fail(MLH_ERR_NOT_TRAINED, "no network parameters");
+        F = ctx_mlp(c).frags;
+    }
+    if (fallbacks) *fallbacks = 0;
+    if (n == 0) return MLH_OK;
+    unsigned long long *scr = nullptr;
+    int st = batch_scratch(&scr);
+    if (st) return st;
+    hipStream_t s = c ? ctx_stream(c) : nullptr;
+    const int nblocks = (int)((n + 255) / 256);  // <= kNoAllocBlocks
+    hipLaunchKernelGGL(k_riemann_batch, dim3(nblocks), dim3(256), 0, s, n, gamma, states,
+                       states + n, states + 2 * n, states + 3 * n, states + 4 * n, states + 5 * n,
+                       star, star + n, star + 2 * n, flag, iters, precision, F, tol, 50, 3,
+                       nullptr, nullptr, scr);
+    hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(256), 0, s, scr, nblocks, scr + kNoAllocBlocks);
+    unsigned long long total = 0;
+    CHK(hipMemcpyAsync(&total, scr + kNoAllocBlocks, sizeof(total), hipMemcpyDeviceToHost, s));
+    CHK(hipStreamSynchronize(s));
+    CHK(hipGetLastError());
+    if (fallbacks) *fallbacks = (int64_t)total;
     return MLH_OK;
 }
 
@@ -567,7 +670,7 @@ int mlh_riemann_pstar_batch(int64_t n, double gamma, const double *rhoL, const d
     for (int k = 0; k < 6; ++k) hipMemcpy(buf + k * n, in[k], b, hipMemcpyHostToDevice);
     hipLaunchKernelGGL(k_riemann_batch, dim3((n + 255) / 256), dim3(256), 0, 0, n, gamma, buf,
                        buf + n, buf + 2 * n, buf + 3 * n, buf + 4 * n, buf + 5 * n, buf, buf, buf,
-                       nullptr, ib, MLH_FP64, nullptr, tol, 50, 3, fb, buf + 6 * n);
+                       nullptr, ib, MLH_FP64, nullptr, tol, 50, 3, fb, buf + 6 * n, nullptr);
     hipMemcpy(p_star, buf + 6 * n, b, hipMemcpyDeviceToHost);
     if (iters) hipMemcpy(iters, ib, sizeof(int) * n, hipMemcpyDeviceToHost);
     hipError_t e = hipDeviceSynchronize();

@@ -52,6 +52,10 @@ SIGNATURES = {
     "mlh_riemann_exact_batch_device": (C.c_int, [_vp, C.c_int64, C.c_double, C.c_void_p, C.c_void_p,
                                                  C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                                  C.c_double, C.POINTER(C.c_int64), _fp]),
+    "mlh_riemann_exact_batch_device_noalloc": (C.c_int, [_vp, C.c_int64, C.c_double, C.c_void_p,
+                                                         C.c_void_p, C.c_void_p, C.c_void_p,
+                                                         C.c_int32, C.c_int32, C.c_double,
+                                                         C.POINTER(C.c_int64)]),
     "mlh_mlp_features": (C.c_int, [C.c_int64, C.c_double] + [_dp] * 6 + [_fp, _dp]),
     "mlh_mlp_set_params": (C.c_int, [_vp, _fp]),
     "mlh_mlp_get_params": (C.c_int, [_vp, _fp]),
@@ -347,6 +351,21 @@ def riemann_batch_device(gamma, states, star, flag=None, iters=None, precision=M
                                               precision, seed, tol, C.byref(fb), ms),
          "mlh_riemann_exact_batch_device")
     return fb.value, float(ms[0])
+
+
+def riemann_batch_device_noalloc(gamma, states, star, flag=None, iters=None, precision=MLH_FP64,
+                                 seed=MLH_SEED_STANDARD, tol=1e-6, ctx=None, n=None):
+    """mlh_riemann_exact_batch_device_noalloc on DeviceBuffers laid out as
+    for riemann_batch_device ([6, n] states, [3, n] star); n defaults to the
+    buffers' length.  Returns the fallback count."""
+    if n is None:
+        n = states.shape[1]
+    fb = C.c_int64()
+    ptr = lambda b: b.ptr if b is not None else None
+    _chk(lib().mlh_riemann_exact_batch_device_noalloc(ctx, n, gamma, ptr(states), ptr(star), ptr(flag),
+                                                      ptr(iters), precision, seed, tol, C.byref(fb)),
+         "mlh_riemann_exact_batch_device_noalloc")
+    return fb.value
 
 
 def riemann_pstar(gamma, rl, ul, pl, rr, ur, pr, tol=1e-6):
