@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MLH_ABI_VERSION 1
+#define MLH_ABI_VERSION 2
 
 /* Status codes.  1, 3, 6, 9 are the reference's exit codes
  * (src/Particles.cpp:351, :2252, :1824-1827/:1878-1881,
@@ -122,6 +122,12 @@ typedef struct mlh_diag {
                                        * network's p* (strong jumps only)     */
     int32_t max_neighbours;           /* largest real neighbour count          */
     int32_t max_ghost_neighbours;     /* largest periodic-image neighbour count */
+    /* counted at debug_level >= 1 only (results are the same either way): */
+    int64_t singular_matrices;        /* E_i with a zero or non-finite pivot  *
+                                       * (LAPACK INFO > 0, ignored by the     *
+                                       * reference, Helper.cpp:13-14)         */
+    int64_t nonfinite_particles;      /* particles whose new state is not     *
+                                       * finite after an update               */
 } mlh_diag;
 
 typedef struct mlh_ctx mlh_ctx;
@@ -173,12 +179,54 @@ int mlh_timestep(mlh_ctx *ctx, double *dt);
 /* compPsijTilde + gradient x(D+2) + slopeLimiter (Particles.cpp:1186-1270,
  * 1313-1444, 2292-2502). */
 int mlh_gradients(mlh_ctx *ctx);
+/* compPsijTilde + gradient x(D+2) without the limiter (Particles.h:137-138;
+ * Particles.cpp:1186-1270, 2292-2502): for a caller that runs slopeLimiter
+ * itself with mlh_limit.  Also the CFL step, as mlh_gradients. */
+int mlh_psi_gradients(mlh_ctx *ctx);
+/* slopeLimiter (Particles.h:139-140; Particles.cpp:1313-1444) on the
+ * gradients of mlh_psi_gradients, whatever params.slope_limiting says; the
+ * result is bitwise that of mlh_gradients. */
+int mlh_limit(mlh_ctx *ctx);
+/* gradient(f, grad) (Particles.h:138; Particles.cpp:1257-1270, periodic
+ * :2457-2502 with the image's value f[parent]) of any field: f [N] and grad
+ * [N][D] host arrays in the caller's particle order; psi-tilde from the last
+ * gradient stage. */
+int mlh_gradient_field(mlh_ctx *ctx, const double *f, double *grad);
 /* compEffectiveFace + compRiemannStatesLR + solveRiemannProblems +
  * collectFluxes (Particles.cpp:1290-1311, 1488-2011, 2504-2683); also
  * prepares the updated state, committed by mlh_update. */
 int mlh_fluxes(mlh_ctx *ctx, double dt);
 /* updateStateAndPosition (Particles.cpp:2013-2110). */
 int mlh_update(mlh_ctx *ctx);
+
+/* ---- the face stages one method at a time ------------------------------
+ * For callers that drive solveRiemannProblems' neighbours one by one, as the
+ * reference's MeshlessScheme::run does (MeshlessScheme.cpp:120-160).  The
+ * per-entry arrays live on the device (FP64, max_interactions +
+ * max_ghost_interactions entries per particle), allocated on first use;
+ * mlh_fluxes computes the same sums without them.  After
+ * mlh_collect_fluxes, mlh_update commits the step, bitwise as after
+ * mlh_fluxes. */
+/* compEffectiveFace (Particles.h:142; Particles.cpp:1290-1311, 2504-2531):
+ * A_ij of every row entry. */
+int mlh_effective_faces(mlh_ctx *ctx);
+/* compRiemannStatesLR(dt) (Particles.h:145; Particles.cpp:1488-1733,
+ * 2533-2683): W_R (particle i), W_L (neighbour j) and vFrame of every row
+ * entry, seen from the row's particle. */
+int mlh_riemann_states(mlh_ctx *ctx, double dt);
+/* solveRiemannProblems (Particles.h:147; Particles.cpp:1787-1911): F_ij of
+ * every entry from the stored faces and states, solved once from the lower
+ * index (standard initial guess) and negated for the other side. */
+int mlh_solve_faces(mlh_ctx *ctx);
+/* collectFluxes (Particles.h:149; Particles.cpp:1913-2011): the flux sums
+ * of the stored F_ij in the reference's order. */
+int mlh_collect_fluxes(mlh_ctx *ctx);
+/* checkFluxSymmetry (Particles.cpp:2888-2976) on the stored F_ij with the
+ * reference's one-sided test F_ij + F_ji > tol (FLUX_SYM_TOL = 1e-20; image
+ * entries test components 0..3 as the reference's ghost branch does):
+ * entries failing it, and entries whose partner does not list the particle. */
+int mlh_check_flux_symmetry(mlh_ctx *ctx, double tol, int64_t *asymmetric,
+                            int64_t *missing);
 
 /* ---- readbacks (host buffers owned by the caller) ---------------------
  * Neighbour entries are packed int32: bits 0..26 particle index, bits
@@ -193,6 +241,12 @@ int mlh_get_density(mlh_ctx *ctx, double *rho, double *P, double *omega);
 int mlh_get_gradients(mlh_ctx *ctx, double *grads, double *B);
 /* mF [N], vF [N][D], eF [N] (collectFluxes, Particles.cpp:1913-2011). */
 int mlh_get_fluxes(mlh_ctx *ctx, double *mF, double *vF, double *eF);
+/* The per-entry arrays of the face stages, rows as in mlh_get_neighbours:
+ * Aij [N][stride][D], WR / WL [N][stride][D+2] {rho, P, v...}, vFrame
+ * [N][stride][D], Fij [N][stride][D+2] {mass, energy, momentum...}; any may
+ * be NULL; zero past each row's entries. */
+int mlh_get_face_arrays(mlh_ctx *ctx, double *Aij, double *WR, double *WL,
+                        double *vFrame, double *Fij);
 int mlh_get_diagnostics(mlh_ctx *ctx, mlh_diag *d);
 /* sumVolume, sumMass, sumEnergy, sumMomentumX/Y/Z (Particles.cpp:2830-2886)
  * into out[0..5]. */

@@ -75,6 +75,8 @@ class Diag(C.Structure):
         ("learned_seeded", C.c_int64),
         ("max_neighbours", C.c_int32),
         ("max_ghost_neighbours", C.c_int32),
+        ("singular_matrices", C.c_int64),
+        ("nonfinite_particles", C.c_int64),
     ]
 
     def as_dict(self):

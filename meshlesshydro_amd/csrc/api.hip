@@ -60,6 +60,14 @@ struct mlh_ctx {
     int defer_words = 0;
     MlpState mlp;
     int stage = 0;  // 0 state, 1 neighbours, 2 density, 3 gradients, 4 fluxes
+    // per-method stages: 1 gradients unlimited (mlh_psi_gradients), 2 limited
+    int grad_state = 0;
+    // per-entry face arrays (allocated on first use); bits of face_stage:
+    // 1 A_ij, 2 W_R / W_L / vFrame, 4 F_ij
+    FaceArrays fa{};
+    int face_stage = 0;
+    double *field_in = nullptr, *field_grad = nullptr;  // mlh_gradient_field
+    unsigned long long *sym_counts = nullptr;            // mlh_check_flux_symmetry
     int img_window = 2;
     std::vector<hipEvent_t> evpool;
     float timings[4] = {0, 0, 0, 0};
@@ -195,6 +203,7 @@ static HydroArgs hydro_args(mlh_ctx *c) {
     a.quirk3d = p.quirk_3d_predictor;
     a.write_fluxes = 0;
     a.learned = p.riemann_seed == MLH_SEED_LEARNED;
+    a.debug = p.debug_level >= 1 ? 1 : 0;
     const bool co = c->order != MLH_ORDER_CELL;
     a.sidx = co ? nullptr : c->sorted_idx;
     a.pos = co ? c->pos_w : c->pos;
@@ -438,6 +447,11 @@ static int enqueue_neighbours(mlh_ctx *c) {
     a.max_counts = c->max_counts;
     a.img_window = c->img_window;
     launch_neighbours(a, c->D, c->s);
+    c->face_stage = 0;
+    c->grad_state = 0;
+    // DEBUG_LVL >= 1: every entry's partner must list the particle (the face
+    // kernel's F_ji = -F_ij relies on it); misses go to missing_mirror_pairs
+    if (c->p.debug_level >= 1) launch_mirror_check(hydro_args(c), c->s);
     return MLH_OK;
 }
 
@@ -629,23 +643,46 @@ int mlh_density(mlh_ctx *c) {
     return MLH_OK;
 }
 
-int mlh_gradients(mlh_ctx *c) {
+static int gradients_stage(mlh_ctx *c, bool limit) {
     if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
     if (c->stage < 2) return fail(MLH_ERR_CALL_ORDER, "mlh_density first");
     HydroArgs a = hydro_args(c);
+    if (!limit) a.slope_limiting = 0;
     launch_gradients(a, c->D, c->prec, c->s);
     launch_dt_finalize(a, -1.0, -1.0, -1.0, nullptr, nullptr, 0, c->s);
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipGetLastError());
     c->stage = 3;
+    c->grad_state = limit ? 2 : 1;
+    c->face_stage = 0;
+    return MLH_OK;
+}
+
+int mlh_gradients(mlh_ctx *c) { return gradients_stage(c, true); }
+
+int mlh_psi_gradients(mlh_ctx *c) { return gradients_stage(c, false); }
+
+int mlh_limit(mlh_ctx *c) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (c->stage < 3 || c->grad_state != 1)
+        return fail(MLH_ERR_CALL_ORDER, "mlh_psi_gradients first (gradients not yet limited)");
+    launch_stage(hydro_args(c), c->D, c->prec, kOpLimit, 0, c->fa, nullptr, nullptr, c->s);
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipGetLastError());
+    c->grad_state = 2;
+    c->face_stage = 0;
     return MLH_OK;
 }
 
 int mlh_timestep(mlh_ctx *c, double *dt) {
     if (!c || !dt) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
     if (c->stage < 2) return fail(MLH_ERR_CALL_ORDER, "mlh_density first");
-    int st = mlh_gradients(c);
-    if (st) return st;
+    // the CFL step is pass 1 of the gradient kernel: run it unless a
+    // gradient stage (limited or not) already has
+    if (c->stage < 3) {
+        int st = mlh_gradients(c);
+        if (st) return st;
+    }
     double dtv[3];
     HIPCHK(hipMemcpy(dtv, c->dt, sizeof(dtv), hipMemcpyDeviceToHost));
     *dt = dtv[1];
@@ -675,6 +712,119 @@ int mlh_update(mlh_ctx *c) {
     if (c->stage < 4) return fail(MLH_ERR_CALL_ORDER, "mlh_fluxes first");
     swap_state(c);
     c->stage = 0;
+    return MLH_OK;
+}
+
+}  // extern "C"
+
+// ---- the per-method face stages ---------------------------------------------
+// per-entry FP64 arrays, allocated on first use: A, vF [D], W_R, W_L, F [D+2]
+// per row entry
+static int ensure_face_arrays(mlh_ctx *c) {
+    if (c->fa.A) return MLH_OK;
+    const size_t ent = (size_t)c->stride * c->N * sizeof(double);
+    const int D = c->D;
+    int st;
+    if ((st = dalloc(c, &c->fa.A, ent * D)) || (st = dalloc(c, &c->fa.vF, ent * D)) ||
+        (st = dalloc(c, &c->fa.WR, ent * (D + 2))) || (st = dalloc(c, &c->fa.WL, ent * (D + 2))) ||
+        (st = dalloc(c, &c->fa.F, ent * (D + 2)))) {
+        c->fa = FaceArrays{};
+        return st;
+    }
+    c->fa.rowlen = c->stride;
+    return MLH_OK;
+}
+
+static int face_stage_run(mlh_ctx *c, int op, int what, const HydroArgs &a) {
+    launch_stage(a, c->D, c->prec, op, what, c->fa, nullptr, nullptr, c->s);
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipGetLastError());
+    return MLH_OK;
+}
+
+extern "C" {
+
+int mlh_gradient_field(mlh_ctx *c, const double *f, double *grad) {
+    if (!c || !f || !grad) return fail(MLH_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->stage < 3) return fail(MLH_ERR_CALL_ORDER, "a gradient stage first (psi-tilde)");
+    if (!c->field_in) {
+        int st;
+        if ((st = dalloc(c, &c->field_in, sizeof(double) * c->N)) ||
+            (st = dalloc(c, &c->field_grad, sizeof(double) * c->N * c->D)))
+            return st;
+    }
+    HIPCHK(hipMemcpyAsync(c->field_in, f, sizeof(double) * c->N, hipMemcpyHostToDevice, c->s));
+    launch_stage(hydro_args(c), c->D, c->prec, kOpGradField, 0, c->fa, c->field_in, c->field_grad,
+                 c->s);
+    HIPCHK(hipMemcpyAsync(grad, c->field_grad, sizeof(double) * c->N * c->D,
+                          hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipGetLastError());
+    return MLH_OK;
+}
+
+int mlh_effective_faces(mlh_ctx *c) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (c->stage < 3) return fail(MLH_ERR_CALL_ORDER, "a gradient stage first (psi-tilde)");
+    int st = ensure_face_arrays(c);
+    if (!st) st = face_stage_run(c, kOpFaceStates, 1, hydro_args(c));
+    if (st) return st;
+    c->face_stage = (c->face_stage & 2) | 1;
+    return MLH_OK;
+}
+
+int mlh_riemann_states(mlh_ctx *c, double dt) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (c->stage < 3) return fail(MLH_ERR_CALL_ORDER, "the gradient stages first");
+    int st = ensure_face_arrays(c);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(c->dt, &dt, sizeof(double), hipMemcpyHostToDevice, c->s));
+    if ((st = face_stage_run(c, kOpFaceStates, 2, hydro_args(c)))) return st;
+    c->face_stage = (c->face_stage & 1) | 2;
+    return MLH_OK;
+}
+
+int mlh_solve_faces(mlh_ctx *c) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if ((c->face_stage & 3) != 3)
+        return fail(MLH_ERR_CALL_ORDER, "mlh_effective_faces and mlh_riemann_states first");
+    unsigned long long before[kNumCounters], after[kNumCounters];
+    int st = read_counters(c, before);
+    const HydroArgs a = hydro_args(c);
+    if (!st) st = face_stage_run(c, kOpSolveOwn, 0, a);
+    if (!st) st = face_stage_run(c, kOpSolveMirror, 0, a);
+    if (!st) st = read_counters(c, after);
+    if (st) return st;
+    c->face_stage |= 4;
+    return face_pressure_status(c, before, after);
+}
+
+int mlh_collect_fluxes(mlh_ctx *c) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (!(c->face_stage & 4)) return fail(MLH_ERR_CALL_ORDER, "mlh_solve_faces first");
+    HydroArgs a = hydro_args(c);
+    a.write_fluxes = 1;
+    int st = face_stage_run(c, kOpCollect, 0, a);
+    if (st) return st;
+    c->stage = 4;
+    return MLH_OK;
+}
+
+int mlh_check_flux_symmetry(mlh_ctx *c, double tol, int64_t *asymmetric, int64_t *missing) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if (!(c->face_stage & 4)) return fail(MLH_ERR_CALL_ORDER, "mlh_solve_faces first");
+    if (!c->sym_counts) {
+        int st = dalloc(c, &c->sym_counts, sizeof(unsigned long long) * 2);
+        if (st) return st;
+    }
+    HIPCHK(hipMemsetAsync(c->sym_counts, 0, sizeof(unsigned long long) * 2, c->s));
+    launch_flux_check(hydro_args(c), c->D, c->fa, tol, c->sym_counts, c->s);
+    unsigned long long h[2];
+    HIPCHK(hipMemcpyAsync(h, c->sym_counts, sizeof(h), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipGetLastError());
+    if (asymmetric) *asymmetric = (int64_t)h[0];
+    if (missing) *missing = (int64_t)h[1];
     return MLH_OK;
 }
 
@@ -724,6 +874,45 @@ static int read_derived(mlh_ctx *c, void *dev, size_t count, std::vector<double>
 }
 
 extern "C" {
+
+// one per-entry array [nc][stride][N] (work order) -> [N][stride][nc] by the
+// caller's particle index, zero past each row's entries
+static int read_entry_array(mlh_ctx *c, const double *dev, int nc, double *out,
+                            const std::vector<uint32_t> &perm, const std::vector<int32_t> &cnt) {
+    const size_t N = c->N, S = c->stride;
+    std::vector<double> h(nc * S * N);
+    HIPCHK(hipMemcpy(h.data(), dev, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < N; ++q) {
+        const size_t i = perm[q];
+        for (size_t n = 0; n < S; ++n)
+            for (int k = 0; k < nc; ++k)
+                out[(i * S + n) * nc + k] = (int)n < cnt[q] ? h[((size_t)k * S + n) * N + q] : 0.;
+    }
+    return MLH_OK;
+}
+
+int mlh_get_face_arrays(mlh_ctx *c, double *Aij, double *WR, double *WL, double *vFrame,
+                        double *Fij) {
+    if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
+    if ((Aij && !(c->face_stage & 1)) || ((WR || WL || vFrame) && !(c->face_stage & 2)) ||
+        (Fij && !(c->face_stage & 4)))
+        return fail(MLH_ERR_CALL_ORDER, "the face stage of a requested array has not run");
+    std::vector<uint32_t> perm;
+    int st = read_order(c, perm);
+    if (st) return st;
+    const size_t N = c->N;
+    std::vector<int32_t> cn(N), cg(N);
+    HIPCHK(hipMemcpy(cn.data(), c->noi, 4 * N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cg.data(), c->nog, 4 * N, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < N; ++q) cn[q] += cg[q];
+    const int D = c->D;
+    if (Aij && (st = read_entry_array(c, c->fa.A, D, Aij, perm, cn))) return st;
+    if (WR && (st = read_entry_array(c, c->fa.WR, D + 2, WR, perm, cn))) return st;
+    if (WL && (st = read_entry_array(c, c->fa.WL, D + 2, WL, perm, cn))) return st;
+    if (vFrame && (st = read_entry_array(c, c->fa.vF, D, vFrame, perm, cn))) return st;
+    if (Fij && (st = read_entry_array(c, c->fa.F, D + 2, Fij, perm, cn))) return st;
+    return MLH_OK;
+}
 
 int mlh_get_density(mlh_ctx *c, double *rho, double *P, double *omega) {
     if (!c) return fail(MLH_ERR_INVALID_ARGUMENT, "null ctx");
@@ -790,6 +979,8 @@ int mlh_get_diagnostics(mlh_ctx *c, mlh_diag *d) {
     d->newton_iterations = (int64_t)k[kCntIters];
     d->faces_solved = (int64_t)k[kCntFaces];
     d->learned_seeded = (int64_t)k[kCntSeeded];
+    d->singular_matrices = (int64_t)k[kCntSingular];
+    d->nonfinite_particles = (int64_t)k[kCntNonfinite];
     d->max_neighbours = mc[0];
     d->max_ghost_neighbours = mc[1];
     return MLH_OK;

@@ -2,6 +2,7 @@
 // kernels common to both precisions and the launchers.  The FP32
 // instantiations are in hydro_fp32.hip (compiled with FMA contraction).
 #include "hydro_impl.h"
+#include "stages_impl.h"
 
 namespace mlh {
 
@@ -55,6 +56,51 @@ void launch_riemann_flux(const HydroArgs &a, int D, long long n, const double *w
         hipLaunchKernelGGL(k_riemann_flux<2>, grid, dim3(256), 0, s, a, n, wl, wr, vf, aij, f, flag);
     else
         hipLaunchKernelGGL(k_riemann_flux<3>, grid, dim3(256), 0, s, a, n, wl, wr, vf, aij, f, flag);
+}
+
+void launch_stage(const HydroArgs &a, int D, int precision, int op, int what,
+                  const FaceArrays &fa, const double *f, double *grad, hipStream_t s) {
+    if (precision != 0) {
+        launch_stage_fp32(a, D, op, what, fa, f, grad, s);
+        return;
+    }
+    dim3 grid((a.N + 255) / 256), blk(256);
+    switch (op) {
+    case kOpLimit:
+        if (D == 2) hipLaunchKernelGGL((k_limit<2, double>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((k_limit<3, double>), grid, blk, 0, s, a);
+        break;
+    case kOpGradField:
+        if (D == 2) hipLaunchKernelGGL((k_gradient_field<2, double>), grid, blk, 0, s, a, f, grad);
+        else hipLaunchKernelGGL((k_gradient_field<3, double>), grid, blk, 0, s, a, f, grad);
+        break;
+    case kOpFaceStates:
+        if (D == 2) hipLaunchKernelGGL((k_face_states<2, double>), grid, blk, 0, s, a, fa, what);
+        else hipLaunchKernelGGL((k_face_states<3, double>), grid, blk, 0, s, a, fa, what);
+        break;
+    case kOpSolveOwn:
+    case kOpSolveMirror: {
+        const int pass = op == kOpSolveOwn ? 0 : 1;
+        if (D == 2) hipLaunchKernelGGL((k_face_solve<2, double>), grid, blk, 0, s, a, fa, pass);
+        else hipLaunchKernelGGL((k_face_solve<3, double>), grid, blk, 0, s, a, fa, pass);
+        break;
+    }
+    case kOpCollect:
+        if (D == 2) hipLaunchKernelGGL((k_collect<2, double>), grid, blk, 0, s, a, fa);
+        else hipLaunchKernelGGL((k_collect<3, double>), grid, blk, 0, s, a, fa);
+        break;
+    }
+}
+
+void launch_flux_check(const HydroArgs &a, int D, const FaceArrays &fa, double tol,
+                       unsigned long long *out, hipStream_t s) {
+    dim3 grid((a.N + 255) / 256);
+    if (D == 2) hipLaunchKernelGGL(k_flux_check<2>, grid, dim3(256), 0, s, a, fa, tol, out);
+    else hipLaunchKernelGGL(k_flux_check<3>, grid, dim3(256), 0, s, a, fa, tol, out);
+}
+
+void launch_mirror_check(const HydroArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_mirror_check, dim3((a.N + 255) / 256), dim3(256), 0, s, a);
 }
 
 void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s) {

@@ -8,6 +8,7 @@
 // without it)
 #pragma clang fp contract(on)
 #include "hydro_impl.h"
+#include "stages_impl.h"
 
 namespace mlh {
 
@@ -37,6 +38,36 @@ void launch_faces_fp32(const HydroArgs &a, int D, hipStream_t s) {
     } else {
         if (D == 2) hipLaunchKernelGGL((k_faces<2, float, false>), grid, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_faces<3, float, false>), grid, dim3(256), 0, s, a);
+    }
+}
+
+void launch_stage_fp32(const HydroArgs &a, int D, int op, int what, const FaceArrays &fa,
+                       const double *f, double *grad, hipStream_t s) {
+    dim3 grid((a.N + 255) / 256), blk(256);
+    switch (op) {
+    case kOpLimit:
+        if (D == 2) hipLaunchKernelGGL((k_limit<2, float>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((k_limit<3, float>), grid, blk, 0, s, a);
+        break;
+    case kOpGradField:
+        if (D == 2) hipLaunchKernelGGL((k_gradient_field<2, float>), grid, blk, 0, s, a, f, grad);
+        else hipLaunchKernelGGL((k_gradient_field<3, float>), grid, blk, 0, s, a, f, grad);
+        break;
+    case kOpFaceStates:
+        if (D == 2) hipLaunchKernelGGL((k_face_states<2, float>), grid, blk, 0, s, a, fa, what);
+        else hipLaunchKernelGGL((k_face_states<3, float>), grid, blk, 0, s, a, fa, what);
+        break;
+    case kOpSolveOwn:
+    case kOpSolveMirror: {
+        const int pass = op == kOpSolveOwn ? 0 : 1;
+        if (D == 2) hipLaunchKernelGGL((k_face_solve<2, float>), grid, blk, 0, s, a, fa, pass);
+        else hipLaunchKernelGGL((k_face_solve<3, float>), grid, blk, 0, s, a, fa, pass);
+        break;
+    }
+    case kOpCollect:
+        if (D == 2) hipLaunchKernelGGL((k_collect<2, float>), grid, blk, 0, s, a, fa);
+        else hipLaunchKernelGGL((k_collect<3, float>), grid, blk, 0, s, a, fa);
+        break;
     }
 }
 

@@ -189,8 +189,11 @@ __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
 
 // Helper::inverseMatrix (Helper.cpp:7-18, LAPACK dgetrf_/dgetri_): Gauss-
 // Jordan with partial pivoting, FP64.
+// Returns false when a pivot is zero or not finite (LAPACK dgetrf's INFO > 0,
+// which the reference does not check: the inverse is then inf / NaN).
 template <int D>
-__device__ inline void invert(double *A) {
+__device__ inline bool invert(double *A) {
+    bool ok = true;
     double M[D][2 * D];
     for (int r = 0; r < D; ++r)
         for (int c = 0; c < 2 * D; ++c) M[r][c] = c < D ? A[r * D + c] : (c - D == r ? 1. : 0.);
@@ -212,6 +215,7 @@ __device__ inline void invert(double *A) {
             }
         }
         double d = M[k][k];
+        ok &= d != 0. && isfinite(d);
         for (int c = 0; c < 2 * D; ++c) M[k][c] /= d;
         for (int r = 0; r < D; ++r) {
             if (r == k) continue;
@@ -221,9 +225,60 @@ __device__ inline void invert(double *A) {
     }
     for (int r = 0; r < D; ++r)
         for (int c = 0; c < D; ++c) A[r * D + c] = M[r][D + c];
+    return ok;
 }
 
 // ---------------------------------------------------------------- gradients
+// slopeLimiter (Particles.cpp:1313-1444) for particle t (work position) with
+// fi its fields, grad its gradients (limited in place) and ngbmax / ngbmin
+// the extrema of its neighbours' fields.  The reconstructed extrema and the
+// ratios are evaluated in FP64 in both precisions, with separate rounding
+// as in the reference: with its int abs() a rounding tie f_ij == f_i turns
+// into a 2^31 scaling, and FP32 reconstructions would create such ties
+// where FP64 has none.
+template <int D, typename T>
+__device__ inline void slope_limit(const HydroArgs &a, int t, int cnt, const double *xi,
+                                   const T *fi, T (&grad)[D + 2][D], const T *ngbmax,
+                                   const T *ngbmin) {
+#pragma clang fp contract(off)
+    constexpr int NF = D + 2;
+    const int N = a.N;
+    double midmax[NF], midmin[NF];
+    for (int f = 0; f < NF; ++f) {
+        midmax[f] = tmin_pos<T>();
+        midmin[f] = tmax_val<T>();
+    }
+    for (int n = 0; n < cnt; ++n) {
+        double xj[3];
+        int s[3];
+        nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
+        double xijxi[D];
+        for (int ax = 0; ax < D; ++ax) {
+            double xij = a.foqp ? (xi[ax] + xj[ax]) / 2.
+                                : xi[ax] + a.g.h / 4. * (xj[ax] - xi[ax]);
+            xijxi[ax] = xij - xi[ax];
+        }
+        for (int f = 0; f < NF; ++f) {
+            double gd[D];
+            for (int ax = 0; ax < D; ++ax) gd[ax] = (double)grad[f][ax];
+            double fij = (double)fi[f] + dotT<D, double>(gd, xijxi);
+            if (midmax[f] < fij) midmax[f] = fij;
+            if (midmin[f] > fij) midmin[f] = fij;
+        }
+    }
+    const double beta = a.beta;
+    for (int f = 0; f < NF; ++f) {
+        double ff = (double)fi[f];
+        double amax = lim_abs<double>(((double)ngbmax[f] - ff) / (midmax[f] - ff), a.limiter_abs);
+        double amin = lim_abs<double>((ff - (double)ngbmin[f]) / (ff - midmin[f]), a.limiter_abs);
+        if (amin <= amax && beta * amin < 1.) {
+            for (int ax = 0; ax < D; ++ax) grad[f][ax] = T((double)grad[f][ax] * amin);
+        } else if (amax <= amin && beta * amax < 1.) {
+            for (int ax = 0; ax < D; ++ax) grad[f][ax] = T((double)grad[f][ax] * amax);
+        }
+    }
+}
+
 #ifndef MLH_GRAD_BATCH
 #define MLH_GRAD_BATCH 4
 #endif
@@ -239,6 +294,7 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
     __shared__ double sdt[256];
     int t = xcd_block() * blockDim.x + threadIdx.x;
     double dt_i = DBL_MAX;
+    bool singular = false;
     if (t < a.N) {
         const int N = a.N;
         const int i = a.sidx ? (int)a.sidx[t] : t;
@@ -363,7 +419,8 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
         }
         for (int p = 1; p < D; ++p)
             for (int q = 0; q < p; ++q) E[D * p + q] = E[D * q + p];
-        invert<D>(E);
+        const bool regular = invert<D>(E);
+        singular = a.debug && !regular;
         T B[D * D];
         for (int k = 0; k < D * D; ++k) B[k] = T(E[k]);
         if (kFused) {
@@ -397,49 +454,8 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                 }
             }
         }
-        // pass 3: slope limiter.  The reconstructed extrema and the ratios
-        // are evaluated in FP64 in both precisions: with the reference's
-        // int abs() a rounding tie f_ij == f_i turns into a 2^31 scaling,
-        // and FP32 reconstructions would create such ties where FP64 has none.
-        if (a.slope_limiting) {
-            // separate rounding as in the reference: with its int abs() a
-            // tie f_ij == f_i decides the limiting
-#pragma clang fp contract(off)
-            double midmax[NF], midmin[NF];
-            for (int f = 0; f < NF; ++f) {
-                midmax[f] = tmin_pos<T>();
-                midmin[f] = tmax_val<T>();
-            }
-            for (int n = 0; n < cnt; ++n) {
-                double xj[3];
-                int s[3];
-                nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
-                double xijxi[D];
-                for (int ax = 0; ax < D; ++ax) {
-                    double xij = a.foqp ? (xi[ax] + xj[ax]) / 2.
-                                        : xi[ax] + a.g.h / 4. * (xj[ax] - xi[ax]);
-                    xijxi[ax] = xij - xi[ax];
-                }
-                for (int f = 0; f < NF; ++f) {
-                    double gd[D];
-                    for (int ax = 0; ax < D; ++ax) gd[ax] = (double)grad[f][ax];
-                    double fij = (double)fi[f] + dotT<D, double>(gd, xijxi);
-                    if (midmax[f] < fij) midmax[f] = fij;
-                    if (midmin[f] > fij) midmin[f] = fij;
-                }
-            }
-            const double beta = a.beta;
-            for (int f = 0; f < NF; ++f) {
-                double ff = (double)fi[f];
-                double amax = lim_abs<double>(((double)ngbmax[f] - ff) / (midmax[f] - ff), a.limiter_abs);
-                double amin = lim_abs<double>((ff - (double)ngbmin[f]) / (ff - midmin[f]), a.limiter_abs);
-                if (amin <= amax && beta * amin < 1.) {
-                    for (int ax = 0; ax < D; ++ax) grad[f][ax] = T((double)grad[f][ax] * amin);
-                } else if (amax <= amin && beta * amax < 1.) {
-                    for (int ax = 0; ax < D; ++ax) grad[f][ax] = T((double)grad[f][ax] * amax);
-                }
-            }
-        }
+        // pass 3: slope limiter
+        if (a.slope_limiting) slope_limit<D, T>(a, t, cnt, xi, fi, grad, ngbmax, ngbmin);
         T *fr = (T *)a.frec + (size_t)t * FRN;
         for (int f = 0; f < NF; ++f)
             for (int ax = 0; ax < D; ++ax) fr[FRec<D>::G + f * D + ax] = grad[f][ax];
@@ -457,6 +473,11 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
         __syncthreads();
     }
     if (threadIdx.x == 0) a.dt_block[blockIdx.x] = sdt[0];
+    if (a.debug) {
+        // E_i with a zero pivot (DEBUG_LVL >= 1): one atomic per wavefront
+        const unsigned long long ns = wave_sum((unsigned long long)(singular ? 1 : 0));
+        if (lane_id() == 0 && ns) atomicAdd(&a.counters[kCntSingular], ns);
+    }
 }
 
 #ifndef MLH_FP32_TU
@@ -533,6 +554,7 @@ __device__ inline View<D, T> make_view(const HydroArgs &a, int j) {
 
 struct FaceStats {
     int neg_real, neg_ghost, vacuum, seeded;
+    int nonfinite;  // particles whose new state is not finite (DEBUG_LVL >= 1)
     SolveStats st;
 };
 
@@ -576,6 +598,8 @@ struct FacePrep {
     T A[D], hA[D], WR[D + 2], WL[D + 2], vF[D];
     T ui, uj, ti0, tj0;  // normal / transverse (2D) velocities in the face frame
 };
+template <int D, typename T>
+__device__ inline void face_frame(FacePrep<D, T> &fp);
 
 template <int D, typename T>
 __device__ inline void face_prepare(const HydroArgs &a, const View<D, T> &L,
@@ -721,7 +745,15 @@ __device__ inline void face_prepare(const HydroArgs &a, const View<D, T> &L,
         if (image) fs.neg_ghost++;
         else fs.neg_real++;
     }
-    // Riemann (Riemann.cpp:7-141): face frame along A/|A|; solver left = L
+    face_frame<D, T>(fp);
+}
+
+// Riemann (Riemann.cpp:7-141): face frame along A/|A| of fp.A, normal
+// (and 2D transverse) velocities of the solver's left (WR) and right (WL)
+// states in it
+template <int D, typename T>
+__device__ inline void face_frame(FacePrep<D, T> &fp) {
+    const T *A = fp.A, *WR = fp.WR, *WL = fp.WL;
     T An = tsqrt(dotT<D, T>(A, A));
     T *hA = fp.hA;
     for (int ax = 0; ax < D; ++ax) hA[ax] = T(1) / An * A[ax];
@@ -1003,8 +1035,10 @@ __device__ inline void particle_keep(const HydroArgs &a, int i) {
 
 // Flux sums and updateStateAndPosition (Particles.cpp:2013-2110) of the
 // particle at work position t (original index i).
+// Returns 1 when the debug counters are on (DEBUG_LVL >= 1) and the new
+// state (m, u, v, x) is not finite, else 0.
 template <int D>
-__device__ inline void particle_finish(const HydroArgs &a, int i, const FluxSums acc) {
+__device__ inline int particle_finish(const HydroArgs &a, int i, const FluxSums acc) {
     const double dtd = a.dt[0];
     const double mF = acc.m, eF = acc.e;
     const double vFs[3] = {acc.v[0], acc.v[1], acc.v[2]};
@@ -1032,6 +1066,7 @@ __device__ inline void particle_finish(const HydroArgs &a, int i, const FluxSums
     double u1 = Q0 / m1 - .5 * kn;
     a.mn[i] = m1;
     a.un[i] = u1;
+    bool fin = isfinite(m1) && isfinite(u1);
     for (int ax = 0; ax < D; ++ax) {
         a.vn[ax][i] = vn[ax];
         double x = a.x[ax][i];
@@ -1044,7 +1079,9 @@ __device__ inline void particle_finish(const HydroArgs &a, int i, const FluxSums
             }
         }
         a.xn[ax][i] = x;
+        fin &= isfinite(vn[ax]) && isfinite(x);
     }
+    return a.debug && !fin ? 1 : 0;
 }
 
 // diagnostics: one atomic per wavefront and counter
@@ -1053,7 +1090,8 @@ __device__ inline void add_counters(const HydroArgs &a, const FaceStats fs, int 
         (unsigned long long)fs.neg_real, (unsigned long long)fs.neg_ghost,
         (unsigned long long)fs.vacuum, (unsigned long long)fs.st.nonconv, 0ull,
         (unsigned long long)fs.st.fallback, (unsigned long long)fs.st.iters,
-        (unsigned long long)faces, (unsigned long long)fs.seeded};
+        (unsigned long long)faces, (unsigned long long)fs.seeded, 0ull,
+        (unsigned long long)fs.nonfinite};
     for (int k = 0; k < kNumCounters; ++k) {
         unsigned long long v = wave_sum(c[k]);
         if (lane_id() == 0 && v) atomicAdd(&a.counters[k], v);
@@ -1186,7 +1224,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
             }
         }
     }
-    if (finish) particle_finish<D>(a, i, acc);
+    if (finish) fs.nonfinite += particle_finish<D>(a, i, acc);
     add_counters(a, fs, cnt);
 }
 
@@ -1314,7 +1352,7 @@ __global__ __launch_bounds__(256) void k_faces_deferred(HydroArgs a) {
             }
             wave_lds_sync();
         }
-        if (owner) particle_finish<D>(a, i, acc);
+        if (owner) fs.nonfinite += particle_finish<D>(a, i, acc);
         add_counters(a, fs, 0);
         // the next group's masks and offsets overwrite this group's
         wave_lds_sync();

@@ -35,6 +35,7 @@ struct HydroArgs {
     int riemann_max_iter, learned_iters;
     int limiter_abs, slope_limiting, pairwise_limiter, mfm, enforce_flux_sym;
     int move_particles, foqp, quirk3d, write_fluxes, learned;
+    int debug;                   // DEBUG_LVL >= 1: singular-E / non-finite counters
     const double4 *pos;
     const uint32_t *nbr;
     const int *noi, *nog;
@@ -70,6 +71,26 @@ struct HydroArgs {
     int collect_stride;
 };
 
+// Per-entry face arrays of the per-method stages (FP64, component-major:
+// value c of row entry n of work position t at (c * rowlen + n) * N + t)
+struct FaceArrays {
+    double *A;        // [D][rowlen][N]   effective face, i's perspective
+    double *WR, *WL;  // [D+2][rowlen][N] {rho, P, v...} of i (R) and j (L)
+    double *vF;       // [D][rowlen][N]   frame velocity
+    double *F;        // [D+2][rowlen][N] {mass, energy, momentum...}
+    int rowlen;       // maxr + maxg
+};
+
+// the per-method stages (stages_impl.h)
+enum StageOp {
+    kOpLimit = 0,       // slopeLimiter on the stored gradients
+    kOpGradField,       // gradient(f, grad)
+    kOpFaceStates,      // compEffectiveFace (what & 1), compRiemannStatesLR (what & 2)
+    kOpSolveOwn,        // solveRiemannProblems: the entries the particle solves
+    kOpSolveMirror,     // solveRiemannProblems: F_ij = -F_ji for the others
+    kOpCollect          // collectFluxes + the update
+};
+
 // error bits of HydroArgs::status[0] (the reference's exit codes 1 and 3)
 constexpr int kStatusErrMask = (1 << 1) | (1 << 3);
 // longest neighbour row (maxr + maxg) the FP32 learned seed's deferral masks
@@ -78,7 +99,8 @@ constexpr int kDeferMaxEntries = 2048;
 
 enum CounterIdx {
     kCntNegReal = 0, kCntNegGhost, kCntVacuum, kCntNonconv, kCntMissing,
-    kCntFallback, kCntIters, kCntFaces, kCntSeeded, kNumCounters
+    kCntFallback, kCntIters, kCntFaces, kCntSeeded, kCntSingular, kCntNonfinite,
+    kNumCounters
 };
 
 void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
@@ -111,6 +133,17 @@ void launch_faces_fp32(const HydroArgs &a, int D, hipStream_t s);
 // bytes of one deferred-particle record (FP32 learned seed)
 size_t defer_record_bytes();
 void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s);
+void launch_stage(const HydroArgs &a, int D, int precision, int op, int what,
+                  const FaceArrays &fa, const double *f, double *grad, hipStream_t s);
+void launch_stage_fp32(const HydroArgs &a, int D, int op, int what, const FaceArrays &fa,
+                       const double *f, double *grad, hipStream_t s);
+// checkFluxSymmetry on the stored F_ij: out[0] entries failing the test,
+// out[1] entries without a mirror (added to)
+void launch_flux_check(const HydroArgs &a, int D, const FaceArrays &fa, double tol,
+                       unsigned long long *out, hipStream_t s);
+// neighbour rows whose partner does not list the particle -> the
+// missing_mirror_pairs counter
+void launch_mirror_check(const HydroArgs &a, hipStream_t s);
 void launch_riemann_flux(const HydroArgs &a, int D, long long n, const double *wl,
                          const double *wr, const double *vf, const double *aij, double *f,
                          int *flag, hipStream_t s);

@@ -37,6 +37,15 @@ SIGNATURES = {
     "mlh_gradients": (C.c_int, [_vp]),
     "mlh_fluxes": (C.c_int, [_vp, C.c_double]),
     "mlh_update": (C.c_int, [_vp]),
+    "mlh_psi_gradients": (C.c_int, [_vp]),
+    "mlh_limit": (C.c_int, [_vp]),
+    "mlh_gradient_field": (C.c_int, [_vp, _dp, _dp]),
+    "mlh_effective_faces": (C.c_int, [_vp]),
+    "mlh_riemann_states": (C.c_int, [_vp, C.c_double]),
+    "mlh_solve_faces": (C.c_int, [_vp]),
+    "mlh_collect_fluxes": (C.c_int, [_vp]),
+    "mlh_check_flux_symmetry": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mlh_get_face_arrays": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp]),
     "mlh_get_neighbours": (C.c_int, [_vp, _ip, _ip, _ip]),
     "mlh_get_density": (C.c_int, [_vp, _dp, _dp, _dp]),
     "mlh_get_gradients": (C.c_int, [_vp, _dp, _dp]),
@@ -176,6 +185,51 @@ class Simulation:
 
     def update(self):
         _chk(self.L.mlh_update(self.h), "mlh_update")
+
+    # the reference's methods one at a time
+    def psi_gradients(self):
+        """compPsijTilde + gradient x (D+2), unlimited."""
+        _chk(self.L.mlh_psi_gradients(self.h), "mlh_psi_gradients")
+
+    def limit(self):
+        """slopeLimiter on the gradients of psi_gradients()."""
+        _chk(self.L.mlh_limit(self.h), "mlh_limit")
+
+    def gradient_field(self, f):
+        """gradient(f, grad) of any field f [N] (caller's order) -> [N][D]."""
+        f = np.ascontiguousarray(f, np.float64)
+        assert f.shape == (self.N,)
+        g = np.zeros(self.N * self.D)
+        _chk(self.L.mlh_gradient_field(self.h, _d(f), _d(g)), "mlh_gradient_field")
+        return g.reshape(self.N, self.D)
+
+    def effective_faces(self):
+        _chk(self.L.mlh_effective_faces(self.h), "mlh_effective_faces")
+
+    def riemann_states(self, dt):
+        _chk(self.L.mlh_riemann_states(self.h, dt), "mlh_riemann_states")
+
+    def solve_faces(self):
+        _chk(self.L.mlh_solve_faces(self.h), "mlh_solve_faces")
+
+    def collect_fluxes(self):
+        _chk(self.L.mlh_collect_fluxes(self.h), "mlh_collect_fluxes")
+
+    def check_flux_symmetry(self, tol=1e-20):
+        asym, miss = C.c_int64(), C.c_int64()
+        _chk(self.L.mlh_check_flux_symmetry(self.h, tol, C.byref(asym), C.byref(miss)),
+             "mlh_check_flux_symmetry")
+        return asym.value, miss.value
+
+    def get_face_arrays(self, which=("A", "WR", "WL", "vF", "F")):
+        """Per-entry arrays of the face stages, [N][stride][components]."""
+        D = self.D
+        S = self.p.max_interactions + self.p.max_ghost_interactions
+        nc = {"A": D, "WR": D + 2, "WL": D + 2, "vF": D, "F": D + 2}
+        out = {k: np.zeros((self.N, S, nc[k])) for k in which}
+        args = [_d(out[k]) if k in out else None for k in ("A", "WR", "WL", "vF", "F")]
+        _chk(self.L.mlh_get_face_arrays(self.h, *args), "mlh_get_face_arrays")
+        return out
 
     def get_neighbours(self):
         stride = self.p.max_interactions + self.p.max_ghost_interactions

@@ -64,6 +64,7 @@ struct orc {
     double *psi, *psig, *A, *Ag, *F, *Fg;
     double *mF, *vF, *eF;
     int stage;
+    int limited;  /* gradients of the last gradient stage are limited */
     int brute;
     mlh_diag diag;
 };
@@ -711,8 +712,8 @@ static void limit_field(orc *o, int f, int i) {
 }
 
 /* compPsijTilde (Particles.cpp:1186-1254; ghosts :2292-2455), gradient
- * (:1257-1270; :2457-2502), slopeLimiter (:1313-1444) */
-int orc_gradients(orc *o) {
+ * (:1257-1270; :2457-2502), slopeLimiter (:1313-1444) when limit */
+static int gradients_impl(orc *o, int limit) {
     if (o->stage < 2) return MLH_ERR_CALL_ORDER;
     int D = o->D, NF = D + 2;
     double h = o->p.kernel_size;
@@ -753,12 +754,49 @@ int orc_gradients(orc *o) {
             }
         }
     }
-    if (o->p.slope_limiting) {
+    if (limit) {
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < o->N; ++i)
             for (int f = 0; f < NF; ++f) limit_field(o, f, i);
     }
     o->stage = 3;
+    o->limited = limit;
+    return 0;
+}
+
+int orc_gradients(orc *o) { return gradients_impl(o, o->p.slope_limiting); }
+
+/* compPsijTilde + gradient without slopeLimiter */
+int orc_psi_gradients(orc *o) { return gradients_impl(o, 0); }
+
+/* slopeLimiter (Particles.cpp:1313-1444) on unlimited gradients */
+int orc_limit(orc *o) {
+    if (o->stage < 3 || o->limited) return MLH_ERR_CALL_ORDER;
+    int NF = o->D + 2;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < o->N; ++i)
+        for (int f = 0; f < NF; ++f) limit_field(o, f, i);
+    o->limited = 1;
+    return 0;
+}
+
+/* gradient(f, grad) (Particles.cpp:1257-1270; with ghosts :2457-2502, the
+ * ghost's value being its parent's, updateGhostState :2193-2206) */
+int orc_gradient_field(orc *o, const double *f, double *grad) {
+    if (o->stage < 3) return MLH_ERR_CALL_ORDER;
+    int D = o->D;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < o->N; ++i) {
+        double xj[MAXD];
+        double *g = &grad[(size_t)i * D];
+        for (int a = 0; a < D; ++a) g[a] = 0.;
+        int cnt = nb_count(o, i);
+        for (int n = 0; n < cnt; ++n) {
+            int j = nb_get(o, i, n, xj);
+            const double *ps = psi_at(o, i, n);
+            for (int a = 0; a < D; ++a) g[a] += (f[j] - f[i]) * ps[a];
+        }
+    }
     return 0;
 }
 
@@ -1147,6 +1185,79 @@ int orc_fluxes(orc *o, double dt) {
     o->stage = 4;
     if (o->p.debug_level >= 1 && neg_ghost > 0) return MLH_ERR_NEGATIVE_FACE_PRESSURE;
     if (o->p.debug_level > 1 && neg_real > 0) return MLH_ERR_NEGATIVE_FACE_PRESSURE;
+    return 0;
+}
+
+/* The per-pair arrays of the face stages in the C ABI's row layout
+ * ([N][maxr + maxg][components], real entries then ghosts): Aij / AijGhosts
+ * and Fij / FijGhosts as orc_fluxes stored them, WijR / WijL / vFrame of
+ * compRiemannStatesLR with step dt.  Any output may be NULL. */
+int orc_get_face_arrays(orc *o, double dt, double *A, double *WR, double *WL, double *vF,
+                        double *F) {
+    if (o->stage < 4) return MLH_ERR_CALL_ORDER;
+    int D = o->D, NF = D + 2;
+    size_t S = (size_t)o->maxr + o->maxg;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < o->N; ++i) {
+        double xj[MAXD], wr[MAXD + 2], wl[MAXD + 2], vf[MAXD];
+        int cnt = nb_count(o, i);
+        for (int n = 0; n < cnt; ++n) {
+            int ghost = n >= o->noi[i];
+            int j = nb_get(o, i, n, xj);
+            size_t e = (size_t)i * S + n;
+            const double *Ap = ghost ? &o->Ag[(o->offg[i] + n - o->noi[i]) * D]
+                                     : &o->A[(o->offr[i] + n) * D];
+            const double *Fp = ghost ? &o->Fg[(o->offg[i] + n - o->noi[i]) * NF]
+                                     : &o->F[(o->offr[i] + n) * NF];
+            if (A) memcpy(&A[e * D], Ap, sizeof(double) * D);
+            if (F) memcpy(&F[e * NF], Fp, sizeof(double) * NF);
+            if (WR || WL || vF) {
+                face_states(o, i, j, xj, ghost, dt, wr, wl, vf);
+                if (WR) memcpy(&WR[e * NF], wr, sizeof(double) * NF);
+                if (WL) memcpy(&WL[e * NF], wl, sizeof(double) * NF);
+                if (vF) memcpy(&vF[e * D], vf, sizeof(double) * D);
+            }
+        }
+    }
+    return 0;
+}
+
+/* checkFluxSymmetry (Particles.cpp:2888-2976): per pair, the reverse entry
+ * and the one-sided test F_ij[c] + F_ji[c] > tol (ghost pairs: c = 0..3, as
+ * the reference's ghost branch).  A missing reverse entry (the reference
+ * then reads past the partner's list) is counted apart. */
+int orc_check_flux_symmetry(orc *o, double tol, int64_t *asym, int64_t *missing) {
+    if (o->stage < 4) return MLH_ERR_CALL_ORDER;
+    int D = o->D, NF = D + 2;
+    int64_t na = 0, nm = 0;
+#pragma omp parallel for schedule(static) reduction(+ : na, nm)
+    for (int i = 0; i < o->N; ++i) {
+        for (int n = 0; n < o->noi[i]; ++n) {
+            int j = o->nnl[(size_t)i * o->maxr + n];
+            int ij;
+            for (ij = 0; ij < o->noi[j]; ++ij)
+                if (o->nnl[(size_t)j * o->maxr + ij] == i) break;
+            if (ij == o->noi[j]) { nm++; continue; }
+            const double *a = &o->F[(o->offr[i] + n) * NF], *b = &o->F[(o->offr[j] + ij) * NF];
+            int bad = 0;
+            for (int c = 0; c < NF; ++c) bad |= b[c] + a[c] > tol;
+            na += bad;
+        }
+        if (!o->p.periodic) continue;
+        for (int n = 0; n < o->noig[i]; ++n) {
+            int p = o->gpar[o->nnlg[(size_t)i * o->maxg + n]];
+            int ij;
+            for (ij = 0; ij < o->noig[p]; ++ij)
+                if (o->gpar[o->nnlg[(size_t)p * o->maxg + ij]] == i) break;
+            if (ij == o->noig[p]) { nm++; continue; }
+            const double *a = &o->Fg[(o->offg[i] + n) * NF], *b = &o->Fg[(o->offg[p] + ij) * NF];
+            int bad = 0;
+            for (int c = 0; c < 4; ++c) bad |= b[c] + a[c] > tol;
+            na += bad;
+        }
+    }
+    if (asym) *asym = na;
+    if (missing) *missing = nm;
     return 0;
 }
 

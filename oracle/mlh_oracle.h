@@ -41,6 +41,18 @@ int orc_update(orc *o, double dt);
 /* full step: dt_fixed > 0 uses it, else CFL dt clipped to dt_max (> 0) */
 int orc_step(orc *o, double dt_fixed, double dt_max, double *dt_used);
 
+/* the gradient stage split as the reference's methods: compPsijTilde +
+ * gradient (unlimited), slopeLimiter, and gradient(f) of any field */
+int orc_psi_gradients(orc *o);
+int orc_limit(orc *o);
+int orc_gradient_field(orc *o, const double *f, double *grad);
+/* per-pair arrays of the face stages in the C ABI's row layout
+ * [N][maxr + maxg][components] (after orc_fluxes with the same dt) */
+int orc_get_face_arrays(orc *o, double dt, double *A, double *WR, double *WL, double *vF,
+                        double *F);
+/* checkFluxSymmetry (Particles.cpp:2888-2976) on the stored F_ij */
+int orc_check_flux_symmetry(orc *o, double tol, int64_t *asym, int64_t *missing);
+
 int orc_get_neighbours(orc *o, int32_t *noi, int32_t *noi_ghost,
                        int32_t *nbr);
 int orc_get_density(orc *o, double *rho, double *P, double *omega);

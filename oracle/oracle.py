@@ -48,6 +48,12 @@ def lib():
         L.orc_conserved_sums.argtypes = [C.c_void_p, _dp]
         L.orc_get_diag.argtypes = [C.c_void_p, C.POINTER(Diag)]
         L.orc_set_bruteforce_ghosts.argtypes = [C.c_void_p, C.c_int]
+        L.orc_psi_gradients.argtypes = [C.c_void_p]
+        L.orc_limit.argtypes = [C.c_void_p]
+        L.orc_gradient_field.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_get_face_arrays.argtypes = [C.c_void_p, C.c_double] + [_dp] * 5
+        L.orc_check_flux_symmetry.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_int64),
+                                              C.POINTER(C.c_int64)]
         L.orc_riemann_flux_batch.argtypes = [C.c_void_p, C.c_int64] + [_dp] * 5 + [_ip]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_riemann_solve.restype = C.c_int
@@ -214,6 +220,35 @@ class Oracle:
 
     def set_bruteforce_ghosts(self, on=True):
         self.L.orc_set_bruteforce_ghosts(self.h, int(on))
+
+    def psi_gradients(self):
+        self._chk(self.L.orc_psi_gradients(self.h), "psi_gradients")
+
+    def limit(self):
+        self._chk(self.L.orc_limit(self.h), "limit")
+
+    def gradient_field(self, f):
+        f = np.ascontiguousarray(f, np.float64)
+        g = np.zeros(self.N * self.D)
+        self._chk(self.L.orc_gradient_field(self.h, _d(f), _d(g)), "gradient_field")
+        return g.reshape(self.N, self.D)
+
+    def get_face_arrays(self, dt, which=("A", "WR", "WL", "vF", "F")):
+        """Per-pair arrays of the face stages, [N][stride][components] (after
+        fluxes(dt))."""
+        D = self.D
+        S = self.p.max_interactions + self.p.max_ghost_interactions
+        nc = {"A": D, "WR": D + 2, "WL": D + 2, "vF": D, "F": D + 2}
+        out = {k: np.zeros((self.N, S, nc[k])) for k in which}
+        args = [_d(out[k]) if k in out else None for k in ("A", "WR", "WL", "vF", "F")]
+        self._chk(self.L.orc_get_face_arrays(self.h, dt, *args), "get_face_arrays")
+        return out
+
+    def check_flux_symmetry(self, tol=1e-20):
+        a, m = C.c_int64(), C.c_int64()
+        self._chk(self.L.orc_check_flux_symmetry(self.h, tol, C.byref(a), C.byref(m)),
+                  "check_flux_symmetry")
+        return a.value, m.value
 
 
 class OracleError(RuntimeError):

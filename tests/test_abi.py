@@ -40,7 +40,7 @@ def test_default_params_match_python_mirror():
             assert list(a) == list(b), name
         else:
             assert a == b, name
-    assert lib.lib().mlh_version() == 1
+    assert lib.lib().mlh_version() == 2
 
 
 def test_status_codes_match_reference_exit_codes():
