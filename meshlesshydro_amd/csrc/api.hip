@@ -416,7 +416,9 @@ static int prepare_open_grid(mlh_ctx *c) {
     return MLH_OK;
 }
 
-static int enqueue_neighbours(mlh_ctx *c) {
+// fuse_density: the step's compDensity + compPressure run inside the search
+// (the same records, bitwise, as mlh_density's k_density)
+static int enqueue_neighbours(mlh_ctx *c, bool fuse_density = false) {
     if (!c->p.periodic) {
         int st = prepare_open_grid(c);
         if (st) return st;
@@ -446,6 +448,14 @@ static int enqueue_neighbours(mlh_ctx *c) {
     a.status = c->status;
     a.max_counts = c->max_counts;
     a.img_window = c->img_window;
+    a.dens = fuse_density ? (c->prec == MLH_FP32 ? 4 : 8) : 0;
+    if (fuse_density) {
+        a.prim = c->prim;
+        a.m = c->m;
+        a.u = c->u;
+        for (int k = 0; k < 3; ++k) a.v[k] = c->v[k];
+        a.gamma = c->p.gamma;
+    }
     launch_neighbours(a, c->D, c->s);
     c->face_stage = 0;
     c->grad_state = 0;
@@ -537,13 +547,13 @@ static int ensure_events(mlh_ctx *c, int nsteps) {
 static int enqueue_step(mlh_ctx *c, double dt_fixed, double t_now, double t_clip,
                          const double *dt_in, double *dt_hist, int step, hipEvent_t *ev) {
     if (ev) hipEventRecord(ev[0], c->s);
-    int st = enqueue_neighbours(c);
+    // the search with the density fused in (compDensity + compPressure)
+    int st = enqueue_neighbours(c, true);
     if (st) return st;
     // after the search: it settles the work order (and, without periodic
     // boundaries, the grid) the arguments depend on
     HydroArgs a = hydro_args(c);
     if (ev) hipEventRecord(ev[1], c->s);
-    launch_density(a, c->D, c->prec, c->s);
     if (ev) hipEventRecord(ev[2], c->s);
     launch_gradients(a, c->D, c->prec, c->s);
     launch_dt_finalize(a, dt_fixed, t_now, t_clip, dt_in, dt_hist, step, c->s);

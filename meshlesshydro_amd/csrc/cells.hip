@@ -32,6 +32,7 @@
 // cell order is ascending original index (stable sort of the identity), so
 // the reference's neighbour order is kept.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "device_common.h"
 #include "kernels.h"
 
@@ -267,13 +268,38 @@ __device__ inline double box_gap(double x, int k, double bmin, double cs, int nc
 // (pos_cell); an entry stores the candidate's work position: its cell-order
 // position (kCaller false), or its original index when the work order is
 // the caller's (kCaller true).
-template <int D, bool kCaller>
+__device__ inline double wsqrt(double x) { return sqrt(x); }
+__device__ inline float wsqrt(float x) { return __builtin_sqrtf(x); }
+
+// |a - b|^2 with a * b + c fused, as the FP32 kernels (compiled with FMA
+// contraction) evaluate the distance of their kernel weights
+template <int D>
+__device__ inline double dist2_fma(const double *a, const double *b) {
+#pragma clang fp contract(on)
+    double d = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]);
+    if (D == 3) d += (a[2] - b[2]) * (a[2] - b[2]);
+    return d;
+}
+// the distance the density of record type TW uses: the search's own for
+// FP64 records, the fused form for FP32 records
+template <int D, typename TW>
+__device__ inline TW wdist(double d_search, const double *a, const double *b) {
+    if constexpr (sizeof(TW) == 4) return wsqrt(TW(dist2_fma<D>(a, b)));
+    else return wsqrt(TW(d_search));
+}
+
+// kDens: 0, or the bytes of the primitive records (4, 8) when the step's
+// density is fused into the search (NbrArgs::dens)
+template <int D, bool kCaller, int kDens>
 __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
+    using TW = typename std::conditional<kDens == 4, float, double>::type;
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     const bool live = t < a.N;
     const Geom &g = a.g;
     const int N = a.N;
     int cnt = 0, gcnt = 0;
+    const TW wh2 = TW(g.h / 2.), wsig = TW(g.sigma);
+    TW omg = TW(0);
     if (live) {
         double4 pi = a.pos_w[t];
         double xi[3] = {pi.x, pi.y, pi.z};
@@ -334,6 +360,10 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
                                 *wp = val | (kCodeReal << kIdxBits);
                                 wp += N;
                                 ++cnt;
+                                // density: the hits in row order, with the
+                                // distance of the test ((-a)^2 = a^2 exactly)
+                                if constexpr (kDens != 0)
+                                    omg += kernel_w<TW>(wdist<D, TW>(d, xj, xi), wh2, wsig);
                             }
                         }
                         if (cnt > a.maxr) {
@@ -403,6 +433,24 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
             if (err) atomicOr(a.status, 1 << 3);
         }
         a.nog[t] = gcnt;
+        if constexpr (kDens != 0) {
+            // compDensity (Particles.cpp:1151-1184): the self term after the
+            // real neighbours, then the images in ghost order (:2262-2290)
+            TW om = omg + kernel_w<TW>(TW(0), wh2, wsig);
+            if (g.periodic) {
+                const uint32_t *grow = a.nbr + t + (size_t)cnt * N;
+                for (int p = 0; p < gcnt; ++p) {
+                    const uint32_t e = grow[(size_t)p * N];
+                    const double4 pj = kCaller ? a.pos_w[nb_index(e)] : a.pos_cell[nb_index(e)];
+                    double xj[3] = {pj.x, pj.y, pj.z};
+                    for (int ax = 0; ax < D; ++ax)
+                        xj[ax] = image_coord(xj[ax], code_shift(nb_code(e), ax), g.bmin[ax], g.bmax[ax]);
+                    om += kernel_w<TW>(wdist<D, TW>(dist2<D>(xi, xj), xi, xj), wh2, wsig);
+                }
+            }
+            write_prim<D, TW>((TW *)a.prim + (size_t)t * kPrimN, om, a.m[io], a.u[io],
+                              a.v[0][io], a.v[1][io], D == 3 ? a.v[2][io] : 0., a.gamma);
+        }
     }
     // largest counts: a wavefront maximum, one atomic per wavefront
     for (int off = 32; off > 0; off >>= 1) {
@@ -574,11 +622,6 @@ __global__ void k_pack_pos(const double *__restrict__ x0, const double *__restri
     pos[i] = make_double4(x0[i], x1[i], D == 3 ? x2[i] : 0.0, (double)i);
 }
 
-template __global__ void k_neighbours<2, false>(NbrArgs);
-template __global__ void k_neighbours<3, false>(NbrArgs);
-template __global__ void k_neighbours<2, true>(NbrArgs);
-template __global__ void k_neighbours<3, true>(NbrArgs);
-
 void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
                   const double *x2, int N, double4 *pos, uint32_t *key, uint32_t *idx,
                   uint32_t *skey, uint32_t *scratch, int *cstart, int *cend, hipStream_t s) {
@@ -605,14 +648,21 @@ void launch_pack_pos(const double *x0, const double *x1, const double *x2, int D
     hipLaunchKernelGGL(k_pack_pos, dim3((N + 255) / 256), dim3(256), 0, s, x0, x1, x2, D, N, pos);
 }
 
+template <int D, bool kCaller>
+static void launch_nbr(const NbrArgs &a, dim3 grid, hipStream_t s) {
+    if (a.dens == 4) hipLaunchKernelGGL((k_neighbours<D, kCaller, 4>), grid, dim3(256), 0, s, a);
+    else if (a.dens == 8) hipLaunchKernelGGL((k_neighbours<D, kCaller, 8>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_neighbours<D, kCaller, 0>), grid, dim3(256), 0, s, a);
+}
+
 void launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {
-    int nb = (a.N + 255) / 256;
+    dim3 grid((a.N + 255) / 256);
     if (a.caller_order) {
-        if (D == 2) hipLaunchKernelGGL((k_neighbours<2, true>), dim3(nb), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_neighbours<3, true>), dim3(nb), dim3(256), 0, s, a);
+        if (D == 2) launch_nbr<2, true>(a, grid, s);
+        else launch_nbr<3, true>(a, grid, s);
     } else {
-        if (D == 2) hipLaunchKernelGGL((k_neighbours<2, false>), dim3(nb), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_neighbours<3, false>), dim3(nb), dim3(256), 0, s, a);
+        if (D == 2) launch_nbr<2, false>(a, grid, s);
+        else launch_nbr<3, false>(a, grid, s);
     }
 }
 

@@ -123,19 +123,52 @@ __device__ inline double cube_cr(double t) {
 __device__ inline float cube_cr(float t) { return t * t * t; }
 
 // Kernel::cubicSpline (Particles.cpp:7-24); sigma precomputed on the host
-// with the reference's expression.
+// with the reference's expression.  The rounding does not depend on the
+// translation unit: FP64 keeps the reference's separate roundings, FP32
+// fuses a * b + c within an expression (as every FP32 kernel does).
 template <typename T>
 __device__ inline T kernel_w(T r, T h2, T sigma) {
-    T q = r / h2;
-    if (T(0) <= q && q <= T(1)) return sigma * (T(1) - T(3) / T(2) * q * q * (T(1) - q / T(2)));
-    if (T(1) < q && q < T(2)) return sigma / T(4) * cube_cr(T(2) - q);
-    return T(0);
+    if constexpr (sizeof(T) == 4) {
+#pragma clang fp contract(on)
+        T q = r / h2;
+        if (T(0) <= q && q <= T(1)) return sigma * (T(1) - T(3) / T(2) * q * q * (T(1) - q / T(2)));
+        if (T(1) < q && q < T(2)) return sigma / T(4) * cube_cr(T(2) - q);
+        return T(0);
+    } else {
+#pragma clang fp contract(off)
+        T q = r / h2;
+        if (T(0) <= q && q <= T(1)) return sigma * (T(1) - T(3) / T(2) * q * q * (T(1) - q / T(2)));
+        if (T(1) < q && q < T(2)) return sigma / T(4) * cube_cr(T(2) - q);
+        return T(0);
+    }
 }
 
 // Per-particle primitive record (AoS, 8 values of T).
 constexpr int kPrimRho = 0, kPrimV = 1, kPrimP = 4, kPrimC = 5, kPrimOmega = 6, kPrimN = 8;
 template <int D>
 __host__ __device__ constexpr int field_slot(int f) { return f == D + 1 ? kPrimP : f; }
+
+// compDensity's rho = m omega and compPressure (Particles.cpp:1151-1184,
+// 1272-1288), with c = sqrt(gamma P / rho) for the time step (:1453), into
+// the particle's primitive record pr.  FP64 throughout (no a * b + c forms,
+// so the rounding is the same in every translation unit).
+template <int D, typename T>
+__device__ inline void write_prim(T *pr, T om, double m, double u, double v0, double v1,
+                                  double v2, double gamma) {
+    double rho = m * (double)om;
+    double P = (gamma - 1.) * rho * u;
+    double c = sqrt(gamma * P / rho);
+    T rec[kPrimN];
+    rec[kPrimRho] = T(rho);
+    rec[kPrimV + 0] = T(v0);
+    rec[kPrimV + 1] = T(v1);
+    rec[kPrimV + 2] = D == 3 ? T(v2) : T(0);
+    rec[kPrimP] = T(P);
+    rec[kPrimC] = T(c);
+    rec[kPrimOmega] = om;
+    rec[7] = T(0);
+    for (int k = 0; k < kPrimN; ++k) pr[k] = rec[k];
+}
 
 // Face-input record: (D+2)*D limited gradients then D*D B = E^-1.
 template <int D>

@@ -171,20 +171,8 @@ __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
         }
         om = og;
     }
-    double rho = a.m[i] * (double)om;
-    double P = (a.gamma - 1.) * rho * a.u[i];
-    double c = sqrt(a.gamma * P / rho);
-    T *pr = (T *)a.prim + (size_t)t * kPrimN;
-    T rec[kPrimN];
-    rec[kPrimRho] = T(rho);
-    rec[kPrimV + 0] = T(a.v[0][i]);
-    rec[kPrimV + 1] = T(a.v[1][i]);
-    rec[kPrimV + 2] = D == 3 ? T(a.v[2][i]) : T(0);
-    rec[kPrimP] = T(P);
-    rec[kPrimC] = T(c);
-    rec[kPrimOmega] = om;
-    rec[7] = T(0);
-    for (int k = 0; k < kPrimN; ++k) pr[k] = rec[k];
+    write_prim<D, T>((T *)a.prim + (size_t)t * kPrimN, om, a.m[i], a.u[i], a.v[0][i], a.v[1][i],
+                     D == 3 ? a.v[2][i] : 0., a.gamma);
 }
 
 // Helper::inverseMatrix (Helper.cpp:7-18, LAPACK dgetrf_/dgetri_): Gauss-

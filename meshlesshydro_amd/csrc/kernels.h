@@ -20,6 +20,14 @@ struct NbrArgs {
     int *status;                 // [0] bit 1: real overflow, bit 3: image overflow
     int *max_counts;             // [2]
     int img_window;              // virtual-cell reach of the image search
+    // The step's compDensity + compPressure fused into the search (dens =
+    // 4: FP32 records, 8: FP64 records, 0: off): omega summed over the hits
+    // in row order, the self term, the images in row order, then the
+    // primitive record at the work position, bitwise as k_density.
+    int dens;
+    void *prim;
+    const double *m, *u, *v[3];
+    double gamma;
 };
 
 // Arguments of the physics kernels.  Storage type of derived per-particle

@@ -65,105 +65,144 @@ __global__ void k_mlp_pack(const float *__restrict__ P, MlpFrags *__restrict__ F
 }
 
 // ---- training --------------------------------------------------------------
-// Per sample: forward, dy, dz2 = dy w3 (1-h2^2), dz1 = (W2^T dz2)(1-h1^2).
-__global__ __launch_bounds__(256) void k_mlp_fwd_bwd(
-    const float *__restrict__ P, const float *__restrict__ X, const float *__restrict__ T,
-    const int *__restrict__ idx, int B, float *__restrict__ H1, float *__restrict__ DZ1,
-    float *__restrict__ H2, float *__restrict__ DZ2, float *__restrict__ DY,
-    float *__restrict__ E2, float *__restrict__ XB) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    int s = idx ? idx[b] : b;
+// One wavefront per chunk of 64 samples of the batch (lane = sample): the
+// forward pass, the loss term and the backward pass per lane, activations
+// staged in LDS, then the chunk's partial gradient with lane o computing
+// row o of every weight matrix as a sum over the chunk's samples in sample
+// order.  k_mlp_reduce sums the chunks' partials in chunk order and applies
+// Adam or stores the gradient: the result does not depend on scheduling and
+// needs no atomics.
+//   dy = 2 e / B, dz2 = dy w3 (1 - h2^2), dz1 = (W2^T dz2)(1 - h1^2)
+//   dW3 = sum dy h2, dW2 = sum dz2 h1^T, dW1 = sum dz1 x^T, biases the sums
+constexpr int kTrainChunk = 64;
+constexpr int kTrainLd = kMlpH + 1;   // LDS row pitch (no bank conflicts)
+constexpr int kPartialN = kMlpN + 1;  // gradient + the chunk's sum of e^2
+
+__global__ __launch_bounds__(64) void k_mlp_chunk_grad(const float *__restrict__ P,
+                                                       const float *__restrict__ X,
+                                                       const float *__restrict__ T,
+                                                       const int *__restrict__ idx, int B,
+                                                       float *__restrict__ partial) {
+    __shared__ float H1[kTrainChunk][kTrainLd];
+    __shared__ float H2[kTrainChunk][kTrainLd];  // h2, then dz1
+    __shared__ float DZ2[kTrainChunk][kTrainLd];
+    __shared__ float XS[kTrainChunk][kMlpIn];
+    __shared__ float DY[kTrainChunk], E2[kTrainChunk];
+    const int l = threadIdx.x;
+    const int b = blockIdx.x * kTrainChunk + l;
+    const bool act = b < B;
+    const int s = act ? (idx ? idx[b] : b) : 0;
     float x[kMlpIn];
+#pragma unroll
     for (int k = 0; k < kMlpIn; ++k) {
-        x[k] = X[(size_t)s * kMlpIn + k];
-        XB[(size_t)b * kMlpIn + k] = x[k];
+        x[k] = act ? X[(size_t)s * kMlpIn + k] : 0.f;
+        XS[l][k] = x[k];
     }
-    float h1[kMlpH], h2[kMlpH];
+    // register arrays are indexed with compile-time indices only (unrolled
+    // loops), so they stay in registers
+    float h1[kMlpH];
+#pragma unroll
     for (int o = 0; o < kMlpH; ++o) {
         float a = P[kMlpB1 + o];
+#pragma unroll
         for (int k = 0; k < kMlpIn; ++k) a += P[kMlpW1 + o * kMlpIn + k] * x[k];
         h1[o] = tanhf(a);
+        H1[l][o] = h1[o];
     }
     float y = P[kMlpB3];
     for (int o = 0; o < kMlpH; ++o) {
         float a = P[kMlpB2 + o];
+#pragma unroll
         for (int k = 0; k < kMlpH; ++k) a += P[kMlpW2 + o * kMlpH + k] * h1[k];
-        h2[o] = tanhf(a);
-        y += P[kMlpW3 + o] * h2[o];
+        const float h2 = tanhf(a);
+        H2[l][o] = h2;
+        y += P[kMlpW3 + o] * h2;
     }
-    float e = y - T[s];
-    float dy = 2.f * e / (float)B;
-    DY[b] = dy;
-    E2[b] = e * e;
-    float dz2[kMlpH];
+    // samples past the batch end contribute nothing (dy = 0)
+    const float e = act ? y - T[s] : 0.f;
+    const float dy = 2.f * e / (float)B;
+    DY[l] = dy;
+    E2[l] = e * e;
     for (int o = 0; o < kMlpH; ++o) {
-        dz2[o] = dy * P[kMlpW3 + o] * (1.f - h2[o] * h2[o]);
-        H2[(size_t)b * kMlpH + o] = h2[o];
-        DZ2[(size_t)b * kMlpH + o] = dz2[o];
-        H1[(size_t)b * kMlpH + o] = h1[o];
+        const float h2 = H2[l][o];
+        DZ2[l][o] = dy * P[kMlpW3 + o] * (1.f - h2 * h2);
     }
+    __syncthreads();
+    float *g = partial + (size_t)blockIdx.x * kPartialN;
+    {
+        // lane o: dW3[o], db2[o], row o of dW2; lane 0 also db3 and sum e^2
+        const int o = l;
+        float w3 = 0.f, bb = 0.f, acc[kMlpH];
+#pragma unroll
+        for (int k = 0; k < kMlpH; ++k) acc[k] = 0.f;
+        for (int q = 0; q < kTrainChunk; ++q) {
+            const float d = DZ2[q][o];
+            w3 += DY[q] * H2[q][o];
+            bb += d;
+#pragma unroll
+            for (int k = 0; k < kMlpH; ++k) acc[k] += d * H1[q][k];
+        }
+        g[kMlpW3 + o] = w3;
+        g[kMlpB2 + o] = bb;
+#pragma unroll
+        for (int k = 0; k < kMlpH; ++k) g[kMlpW2 + o * kMlpH + k] = acc[k];
+        if (l == 0) {
+            float b3 = 0.f, se = 0.f;
+            for (int q = 0; q < kTrainChunk; ++q) {
+                b3 += DY[q];
+                se += E2[q];
+            }
+            g[kMlpB3] = b3;
+            g[kMlpN] = se;
+        }
+    }
+    __syncthreads();  // H2 is read above, dz1 goes there below
     for (int k = 0; k < kMlpH; ++k) {
         float a = 0.f;
-        for (int o = 0; o < kMlpH; ++o) a += P[kMlpW2 + o * kMlpH + k] * dz2[o];
-        DZ1[(size_t)b * kMlpH + k] = a * (1.f - h1[k] * h1[k]);
+        for (int o = 0; o < kMlpH; ++o) a += P[kMlpW2 + o * kMlpH + k] * DZ2[l][o];
+        const float hk = H1[l][k];
+        H2[l][k] = a * (1.f - hk * hk);
     }
-}
-
-// One thread per parameter; sequential sum over the batch (deterministic).
-__global__ __launch_bounds__(256) void k_mlp_wgrad(int B, const float *__restrict__ XB,
-                                                   const float *__restrict__ H1,
-                                                   const float *__restrict__ DZ1,
-                                                   const float *__restrict__ H2,
-                                                   const float *__restrict__ DZ2,
-                                                   const float *__restrict__ DY,
-                                                   float *__restrict__ G) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kMlpN) return;
-    const float *A;
-    const float *Bv;
-    int sa, sb;
-    if (p < kMlpB1) {  // W1[o][k]
-        int o = p / kMlpIn, k = p % kMlpIn;
-        A = DZ1 + o; sa = kMlpH; Bv = XB + k; sb = kMlpIn;
-    } else if (p < kMlpW2) {
-        A = DZ1 + (p - kMlpB1); sa = kMlpH; Bv = nullptr; sb = 0;
-    } else if (p < kMlpB2) {
-        int q = p - kMlpW2, o = q / kMlpH, k = q % kMlpH;
-        A = DZ2 + o; sa = kMlpH; Bv = H1 + k; sb = kMlpH;
-    } else if (p < kMlpW3) {
-        A = DZ2 + (p - kMlpB2); sa = kMlpH; Bv = nullptr; sb = 0;
-    } else if (p < kMlpB3) {
-        A = DY; sa = 1; Bv = H2 + (p - kMlpW3); sb = kMlpH;
-    } else {
-        A = DY; sa = 1; Bv = nullptr; sb = 0;
-    }
-    float acc = 0.f;
-    for (int b = 0; b < B; ++b) acc += A[(size_t)b * sa] * (Bv ? Bv[(size_t)b * sb] : 1.f);
-    G[p] = acc;
-}
-
-__global__ void k_sum_f32(const float *__restrict__ v, int n, float *__restrict__ out) {
-    __shared__ float sh[256];
-    float a = 0.f;
-    for (int k = threadIdx.x; k < n; k += 256) a += v[k];
-    sh[threadIdx.x] = a;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-        __syncthreads();
+    {
+        const int o = l;
+        float bb = 0.f, acc[kMlpIn];
+#pragma unroll
+        for (int k = 0; k < kMlpIn; ++k) acc[k] = 0.f;
+        for (int q = 0; q < kTrainChunk; ++q) {
+            const float d = H2[q][o];
+            bb += d;
+#pragma unroll
+            for (int k = 0; k < kMlpIn; ++k) acc[k] += d * XS[q][k];
+        }
+        g[kMlpB1 + o] = bb;
+#pragma unroll
+        for (int k = 0; k < kMlpIn; ++k) g[kMlpW1 + o * kMlpIn + k] = acc[k];
     }
-    if (threadIdx.x == 0) *out = sh[0] / (float)n;
 }
 
-__global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float *__restrict__ M,
-                       float *__restrict__ V, float lr, float b1c, float b2c) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kMlpN) return;
+// Sum of the chunks' partials in chunk order; thread p < kMlpN: gradient
+// entry p (stored in G, and with adam the Adam step of parameter p);
+// thread kMlpN: the loss, mean of e^2 over the batch.
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float *__restrict__ partial,
+                                                    int nchunks, int B, float *__restrict__ G,
+                                                    float *__restrict__ loss, int adam,
+                                                    float *__restrict__ P, float *__restrict__ M,
+                                                    float *__restrict__ V, float lr, float b1c,
+                                                    float b2c) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > kMlpN) return;
+    float g = 0.f;
+    for (int c = 0; c < nchunks; ++c) g += partial[(size_t)c * kPartialN + p];
+    if (p == kMlpN) {
+        *loss = g / (float)B;
+        return;
+    }
+    G[p] = g;
+    if (!adam) return;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    float g = G[p];
-    float m = b1 * M[p] + (1.f - b1) * g;
-    float v = b2 * V[p] + (1.f - b2) * g * g;
+    const float m = b1 * M[p] + (1.f - b1) * g;
+    const float v = b2 * V[p] + (1.f - b2) * g * g;
     M[p] = m;
     V[p] = v;
     P[p] -= lr * (m / b1c) / (sqrtf(v / b2c) + eps);
@@ -355,40 +394,39 @@ int mlh_mlp_get_params(mlh_ctx *c, float *params) {
 }
 
 struct TrainBufs {
-    float *X = nullptr, *T = nullptr, *H1 = nullptr, *DZ1 = nullptr, *H2 = nullptr,
-          *DZ2 = nullptr, *DY = nullptr, *E2 = nullptr, *XB = nullptr, *loss = nullptr;
+    float *X = nullptr, *T = nullptr, *partial = nullptr, *loss = nullptr;
     int *idx = nullptr;
     ~TrainBufs() {
-        float *f[] = {X, T, H1, DZ1, H2, DZ2, DY, E2, XB, loss};
+        float *f[] = {X, T, partial, loss};
         for (float *p : f)
             if (p) hipFree(p);
         if (idx) hipFree(idx);
     }
 };
 
+static int train_chunks(int batch) { return (batch + kTrainChunk - 1) / kTrainChunk; }
+
 static int alloc_train(TrainBufs &b, int64_t n, int batch) {
     CHK(dmalloc(&b.X, sizeof(float) * n * kMlpIn));
     CHK(dmalloc(&b.T, sizeof(float) * n));
-    CHK(dmalloc(&b.H1, sizeof(float) * batch * kMlpH));
-    CHK(dmalloc(&b.DZ1, sizeof(float) * batch * kMlpH));
-    CHK(dmalloc(&b.H2, sizeof(float) * batch * kMlpH));
-    CHK(dmalloc(&b.DZ2, sizeof(float) * batch * kMlpH));
-    CHK(dmalloc(&b.DY, sizeof(float) * batch));
-    CHK(dmalloc(&b.E2, sizeof(float) * batch));
-    CHK(dmalloc(&b.XB, sizeof(float) * batch * kMlpIn));
+    CHK(dmalloc(&b.partial, sizeof(float) * (size_t)train_chunks(batch) * kPartialN));
     CHK(dmalloc(&b.loss, sizeof(float)));
     CHK(dmalloc(&b.idx, sizeof(int) * n));
     return MLH_OK;
 }
 
-static void enqueue_grad(mlh_ctx *c, TrainBufs &b, const int *idx, int B) {
+// gradient (and loss) of the batch idx[0..B) (idx null: samples 0..B-1);
+// with adam, the Adam step at bias corrections b1c / b2c
+static void enqueue_grad(mlh_ctx *c, TrainBufs &b, const int *idx, int B, bool adam = false,
+                         float lr = 0.f, float b1c = 1.f, float b2c = 1.f) {
     MlpState &m = ctx_mlp(c);
     hipStream_t s = ctx_stream(c);
-    hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3((B + 255) / 256), dim3(256), 0, s, m.params, b.X, b.T,
-                       idx, B, b.H1, b.DZ1, b.H2, b.DZ2, b.DY, b.E2, b.XB);
-    hipLaunchKernelGGL(k_mlp_wgrad, dim3((kMlpN + 255) / 256), dim3(256), 0, s, B, b.XB, b.H1,
-                       b.DZ1, b.H2, b.DZ2, b.DY, m.grad);
-    hipLaunchKernelGGL(k_sum_f32, dim3(1), dim3(256), 0, s, b.E2, B, b.loss);
+    const int nch = train_chunks(B);
+    hipLaunchKernelGGL(k_mlp_chunk_grad, dim3(nch), dim3(kTrainChunk), 0, s, m.params, b.X, b.T,
+                       idx, B, b.partial);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartialN + 255) / 256), dim3(256), 0, s, b.partial,
+                       nch, B, m.grad, b.loss, adam ? 1 : 0, m.params, m.adam_m, m.adam_v, lr,
+                       b1c, b2c);
 }
 
 int mlh_mlp_loss_grad(mlh_ctx *c, int64_t n, const float *feats, const float *target,
@@ -435,12 +473,10 @@ int mlh_mlp_train(mlh_ctx *c, int64_t n, const float *feats, const float *target
         }
         CHK(hipMemcpyAsync(b.idx, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
         for (int64_t off = 0; off + batch <= n; off += batch) {
-            enqueue_grad(c, b, b.idx + off, batch);
             m.adam_t++;
             float b1c = 1.f - powf(0.9f, (float)m.adam_t);
             float b2c = 1.f - powf(0.999f, (float)m.adam_t);
-            hipLaunchKernelGGL(k_adam, dim3((kMlpN + 255) / 256), dim3(256), 0, s, m.params,
-                               m.grad, m.adam_m, m.adam_v, lr, b1c, b2c);
+            enqueue_grad(c, b, b.idx + off, batch, true, lr, b1c, b2c);
         }
         CHK(hipMemcpyAsync(&last, b.loss, sizeof(float), hipMemcpyDeviceToHost, s));
         CHK(hipStreamSynchronize(s));

@@ -198,3 +198,26 @@ def test_mirror_check_and_debug_counters(hip):
     print(f"\n[stages] non-finite particles {d['nonfinite_particles']} (state: {bad.sum()})", flush=True)
     assert d["nonfinite_particles"] == bad.sum() > 0
     sim.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("prec", [MLH_FP64, MLH_FP32])
+@pytest.mark.parametrize("order", [0, 2])
+def test_fused_step_equals_stages(hip, case, prec, order):
+    """mlh_step runs compDensity + compPressure inside the neighbour search;
+    mlh_neighbours + mlh_density run them as two kernels.  The same dt and
+    the same state, bitwise, in both precisions and both work orders."""
+    p, s = CASES[case](precision=prec, work_order=order)
+    a = hip.Simulation(p, s)
+    dt_a = a.step()
+    sa = a.state()
+    a.close()
+    b = hip.Simulation(p, s)
+    b.neighbours(); b.density()
+    dt_b = b.timestep()
+    b.gradients(); b.fluxes(dt_b); b.update()
+    sb = b.state()
+    b.close()
+    assert dt_a == dt_b
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
