@@ -33,6 +33,10 @@ def hostlib():
         L.mlh_host_read_parameters.argtypes = [C.c_char_p, C.POINTER(Params), _ip]
         L.mlh_host_config_value.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32]
         L.mlh_host_run.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, C.POINTER(C.c_int64), _dp, _ip]
+        L.mlh_host_run_methods.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32,
+                                           C.POINTER(C.c_int64), _dp, _ip]
+        L.mlh_host_ghost_images.argtypes = [C.POINTER(Params), C.c_int32, _dp, _dp, _dp, C.c_int32,
+                                            _ip, _dp, _dp, _dp, _ip]
         L.mlh_h5_write_ic.argtypes = [C.c_char_p, C.c_int32, C.c_int32] + [_dp] * 4 + [_ip]
         L.mlh_h5_write_snapshot.argtypes = ([C.c_char_p, C.c_int32, C.c_int32] + [C.c_double] * 3 +
                                             [_dp] * 8 + [_ip])
@@ -123,12 +127,35 @@ def read_dataset_int(path, name):
     return out.reshape(v.shape)
 
 
-def run(config, parameter_h=None, log_level=2):
+def run(config, parameter_h=None, log_level=2, per_method=False):
     """The mlh driver in-process: returns (steps, t_end, dumps); raises
-    HostError with the reference's exit code (1, 3, 6, 9) or the status."""
+    HostError with the reference's exit code (1, 3, 6, 9) or the status.
+    per_method: MeshlessScheme calls the Particles methods one at a time in
+    the reference's order (the driver's --per-method)."""
     steps, t, dumps = C.c_int64(), C.c_double(), C.c_int32()
-    st = hostlib().mlh_host_run(_b(config), _b(parameter_h) if parameter_h else None, log_level,
-                                C.byref(steps), C.byref(t), C.byref(dumps))
+    st = hostlib().mlh_host_run_methods(_b(config), _b(parameter_h) if parameter_h else None,
+                                        log_level, int(per_method), C.byref(steps), C.byref(t),
+                                        C.byref(dumps))
     if st != 0:
         raise HostError(st, hostlib().mlh_host_error().decode())
     return steps.value, t.value, dumps.value
+
+
+def ghost_images(params, state):
+    """createGhostParticles on the host (Particles::createGhostParticles of
+    the C++ mirror): (parent [n], coordinates [n][dim]) of the periodic
+    images of the particles in `state`, in the reference's ghost order."""
+    N = len(state["x"])
+    g = lambda k: np.ascontiguousarray(state[k], np.float64) if k in state else np.zeros(N)
+    xs = [g(k) for k in ("x", "y", "z")]
+    cap = 7 * N
+    par = np.zeros(cap, np.int32)
+    out = [np.zeros(cap) for _ in range(3)]
+    n = C.c_int32()
+    st = hostlib().mlh_host_ghost_images(C.byref(params), N, *[a.ctypes.data_as(_dp) for a in xs], cap,
+                                         par.ctypes.data_as(_ip), *[a.ctypes.data_as(_dp) for a in out],
+                                         C.byref(n))
+    if st != 0:
+        raise HostError(st, hostlib().mlh_host_error().decode())
+    k = n.value
+    return par[:k].copy(), np.stack([a[:k] for a in out[: params.dim]], 1)

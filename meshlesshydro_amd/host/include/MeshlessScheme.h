@@ -32,7 +32,15 @@ public:
         double gamma = 0;
     };
 
-    MeshlessScheme(Configuration config, Particles *particles, bool adaptiveTimestep);
+    // perMethod: every step calls the Particles methods one at a time in
+    // the reference's order (gridNNS, compDensity, compPressure,
+    // compGlobalTimestep, compPsijTilde, slopeLimiter, compEffectiveFace,
+    // compRiemannStatesLR, solveRiemannProblems, checkFluxSymmetry at
+    // DEBUG_LVL >= 1, collectFluxes, updateStateAndPosition,
+    // MeshlessScheme.cpp:53-224); the results are bitwise those of the
+    // default path
+    MeshlessScheme(Configuration config, Particles *particles, bool adaptiveTimestep,
+                   bool perMethod = false);
     void run();
 
     int steps() const { return step_; }
@@ -43,6 +51,7 @@ private:
     Configuration config_;
     Particles *particles_;
     bool adaptive_;
+    bool per_method_;
     int step_ = 0, dumps_ = 0;
     double t_ = 0;
 };

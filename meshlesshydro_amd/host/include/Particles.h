@@ -19,6 +19,7 @@
 #pragma once
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../../include/mlh.h"
@@ -34,18 +35,31 @@ void check(int status, const char *where);
 
 class Particles {
 public:
-    // Particles(int numParticles, bool ghosts=false), Particles.h:30; the
-    // params fix dim, box, h, gamma and the algorithm flags for the run
+    // The params fix dim, box, h, gamma and the algorithm flags for the run.
     Particles(int numParticles, const mlh_params &params);
+    // Particles(int numParticles, bool ghosts=false), Particles.h:30, with
+    // the process-wide parameters of defaultParams() (the reference fixes
+    // them at compile time in parameter.h).  ghosts = true makes the
+    // container of periodic images that MeshlessScheme owns
+    // (MeshlessScheme.h:40): host arrays only, room for dim * numParticles
+    // images, filled by createGhostParticles; N is then the image count.
+    explicit Particles(int numParticles, bool ghosts = false);
     ~Particles();
     Particles(const Particles &) = delete;
     Particles &operator=(const Particles &) = delete;
 
+    // parameters that Particles(int, bool) uses; mlh_params_default() until
+    // the caller changes them (e.g. with a parameter.h / config.info read)
+    static mlh_params &defaultParams();
+
     int N;
     int dim;
+    bool ghosts = false;
     std::vector<int> matId, noi, noiGhosts;
     std::vector<double> m, u, x, y, z, vx, vy, vz, rho, P;
     std::vector<double> rhoGrad, vxGrad, vyGrad, vzGrad, PGrad;  // [N][dim]
+    // image container only: the particle each image copies
+    std::vector<int> parent;
 
     // host arrays -> device state / device state -> host arrays
     void upload();
@@ -68,6 +82,42 @@ public:
     void solveRiemannProblems(double dt);
     // updateStateAndPosition (Particles.cpp:2013-2110)
     void updateStateAndPosition(double dt);
+
+    // ---- the reference's methods one at a time (Particles.h:137-151) ----
+    // compPsijTilde + gradient of rho, v, P, unlimited (Particles.cpp:
+    // 1186-1270 and the image twins); with periodic boundaries the
+    // ghostParticles argument of the reference is implicit
+    void compPsijTilde(double kernelSize);
+    // gradient(f, grad) (Particles.cpp:1257-1270): f [N], grad [N][dim]
+    void gradient(const double *f, double *grad);
+    // slopeLimiter (Particles.cpp:1313-1444) on the gradients of
+    // compPsijTilde
+    void slopeLimiter(double kernelSize, Particles *ghostParticles = nullptr);
+    // compEffectiveFace (Particles.cpp:1290-1311, 2504-2531)
+    void compEffectiveFace();
+    // compRiemannStatesLR (Particles.cpp:1488-1733, 2533-2683)
+    void compRiemannStatesLR(double dt, double kernelSize, double gamma);
+    // solveRiemannProblems (Particles.cpp:1787-1911) on the stored states
+    void solveRiemannProblems(double gamma, const Particles &ghostParticles);
+    // collectFluxes (Particles.cpp:1913-2011); updateStateAndPosition then
+    // commits the step
+    void collectFluxes();
+    // checkFluxSymmetry (Particles.cpp:2888-2976): a WARN line per failing
+    // pair count, as the reference warns per pair; returns the counts
+    // (asymmetric entries, entries without a mirror)
+    std::pair<int64_t, int64_t> checkFluxSymmetry(Particles *ghostParticles = nullptr);
+
+    // createGhostParticles (Particles.cpp:2113-2191) on the host arrays x, y
+    // (z) of this set: the images of the particles within kernelSize of a
+    // periodic box face, in the reference's ghost order (per particle the
+    // image shifted along every axis it is near first, 2D: corner, y, x;
+    // 3D the same rule over the seven axis subsets).  The device search
+    // forms the same images implicitly; this fills the container for
+    // callers that inspect them (e.g. dumping them).
+    void createGhostParticles(Particles &ghostParticles, double kernelSize) const;
+    // updateGhostState (Particles.cpp:2193-2206): rho, P, v of every image
+    // from its parent's host arrays
+    void updateGhostState(Particles &ghostParticles) const;
 
     // the whole step on the device (one status check); dt_fixed > 0 uses
     // that step, else the CFL step clipped at t_clip (MeshlessScheme.cpp:
@@ -99,5 +149,11 @@ private:
     mlh_ctx *ctx_ = nullptr;
     void sums(double out[6]);
 };
+
+// The periodic images of n particles at xs[0..dim) (createGhostParticles,
+// Particles.cpp:2113-2191, over dim axes): parents and image coordinates in
+// the reference's ghost order.
+void ghost_images(const mlh_params &p, int n, const double *const xs[3], double kernelSize,
+                  std::vector<int> &parent, std::vector<double> *gx[3]);
 
 }  // namespace mlh_host

@@ -9,8 +9,10 @@
 
 namespace mlh_host {
 
-MeshlessScheme::MeshlessScheme(Configuration config, Particles *particles, bool adaptiveTimestep)
-    : config_(std::move(config)), particles_(particles), adaptive_(adaptiveTimestep) {
+MeshlessScheme::MeshlessScheme(Configuration config, Particles *particles, bool adaptiveTimestep,
+                               bool perMethod)
+    : config_(std::move(config)), particles_(particles), adaptive_(adaptiveTimestep),
+      per_method_(perMethod) {
     if (config_.h5DumpInterval <= 0)
         throw MlhError(MLH_ERR_INVALID_ARGUMENT, "h5DumpInterval must be positive");
     if (config_.timeStep <= 0)
@@ -50,7 +52,42 @@ void MeshlessScheme::run() {
             break;
         }
         double dt;
-        if (dumpNow) {
+        if (per_method_) {
+            // the reference's sequence of Particles methods
+            P.gridNNS(h);
+            P.compDensity(h);
+            P.compPressure(gamma);
+            if (adaptive_) {
+                dt = P.compGlobalTimestep(gamma, h);
+                if (t + dt >= tclip) {
+                    dumpNext = true;
+                    dt = tclip - t;
+                }
+            } else {
+                dt = config_.timeStep;
+            }
+            P.compPsijTilde(h);
+            if (P.params().slope_limiting) P.slopeLimiter(h);
+            P.compEffectiveFace();
+            P.compRiemannStatesLR(dt, h, gamma);
+            if (dumpNow) {
+                char name[32];
+                std::snprintf(name, sizeof(name), "%06d", adaptive_ ? dumpStep : step);
+                Logger(INFO) << "      > Dump particles to file";
+                P.dump2file(config_.outDir + "/" + name + ".h5", t);
+                ++dumpStep;
+                ++dumps_;
+                if (adaptive_) dump = false;
+                if (t >= config_.timeEnd) {
+                    Logger(INFO) << "    > t = " << t << " -> FINISHED!";
+                    break;
+                }
+            }
+            P.solveRiemannProblems(gamma, P);
+            if (P.params().debug_level >= 1) P.checkFluxSymmetry();
+            P.collectFluxes();
+            P.updateStateAndPosition(dt);
+        } else if (dumpNow) {
             P.gridNNS(h);
             P.compDensity(h);
             if (adaptive_) {

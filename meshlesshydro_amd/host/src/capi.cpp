@@ -100,8 +100,42 @@ int mlh_host_config_value(const char *path, const char *key, char *out, int32_t 
 // the whole driver: config.info + parameter.h (may be NULL: defaults) ->
 // snapshots in outDir; returns 0 or the status of the failing call
 // (1, 3, 6, 9 as the reference's exit codes), -1 for other errors
+int mlh_host_run_methods(const char *config, const char *parameter_h, int32_t log_level,
+                         int32_t per_method, int64_t *steps, double *t_end, int32_t *dumps);
+
 int mlh_host_run(const char *config, const char *parameter_h, int32_t log_level,
                  int64_t *steps, double *t_end, int32_t *dumps) {
+    return mlh_host_run_methods(config, parameter_h, log_level, 0, steps, t_end, dumps);
+}
+
+// the periodic images of n particles (host restatement of
+// createGhostParticles): up to cap images into parent / gx, gy, gz
+int mlh_host_ghost_images(const mlh_params *p, int32_t n, const double *x, const double *y,
+                          const double *z, int32_t cap, int32_t *parent, double *gx,
+                          double *gy, double *gz, int32_t *n_out) {
+    try {
+        const double *xs[3] = {x, y, z};
+        std::vector<int> par;
+        std::vector<double> g[3];
+        std::vector<double> *gp[3] = {&g[0], &g[1], &g[2]};
+        mlh_host::ghost_images(*p, n, xs, p->kernel_size, par, gp);
+        *n_out = (int32_t)par.size();
+        if ((int64_t)par.size() > cap) throw std::runtime_error("image capacity exceeded");
+        double *out[3] = {gx, gy, gz};
+        for (size_t k = 0; k < par.size(); ++k) {
+            parent[k] = par[k];
+            for (int a = 0; a < 3; ++a)
+                if (out[a]) out[a][k] = g[a][k];
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        g_err2 = e.what();
+        return -1;
+    }
+}
+
+int mlh_host_run_methods(const char *config, const char *parameter_h, int32_t log_level,
+                         int32_t per_method, int64_t *steps, double *t_end, int32_t *dumps) {
     try {
         mlh_host::log_config().level = (mlh_host::LogLevel)log_level;
         mlh_host::Run r = mlh_host::load_run(config, parameter_h ? parameter_h : "");
@@ -110,7 +144,7 @@ int mlh_host_run(const char *config, const char *parameter_h, int32_t log_level,
         mlh_host::Particles P(ic.N, r.prm.params);
         mlh_host::fill_particles(P, ic);
         P.upload();
-        mlh_host::MeshlessScheme s(r.cfg, &P, r.prm.adaptive_timestep);
+        mlh_host::MeshlessScheme s(r.cfg, &P, r.prm.adaptive_timestep, per_method != 0);
         s.run();
         if (steps) *steps = s.steps();
         if (t_end) *t_end = s.time();

@@ -25,6 +25,8 @@ static void usage() {
         "  -p, --parameters FILE parameter.h read at run time (default: built-in\n"
         "                        Kelvin-Helmholtz long-run set)\n"
         "      --mlp FILE        trained p* seed network (4929 float32 values)\n"
+        "      --per-method      call the Particles methods one at a time, in the\n"
+        "                        reference's order (same results)\n"
         "  -v, --verbose         more printouts\n"
         "  -s, --silent          warnings and errors only\n"
         "  -h, --help            this help\n");
@@ -32,7 +34,7 @@ static void usage() {
 
 int main(int argc, char **argv) {
     std::string config = "config.info", params, mlp;
-    bool verbose = false, silent = false;
+    bool verbose = false, silent = false, per_method = false;
     for (int k = 1; k < argc; ++k) {
         std::string a = argv[k];
         auto next = [&](const char *what) -> std::string {
@@ -45,6 +47,7 @@ int main(int argc, char **argv) {
         if (a == "-c" || a == "--config") config = next("-c");
         else if (a == "-p" || a == "--parameters") params = next("-p");
         else if (a == "--mlp") mlp = next("--mlp");
+        else if (a == "--per-method") per_method = true;
         else if (a == "-v" || a == "--verbose") verbose = true;
         else if (a == "-s" || a == "--silent") silent = true;
         else if (a == "-h" || a == "--help") { usage(); return 0; }
@@ -76,7 +79,7 @@ int main(int argc, char **argv) {
                 throw MlhError(MLH_ERR_INVALID_ARGUMENT, "cannot read " + mlp);
             check(mlh_mlp_set_params(P.context(), w.data()), "mlh_mlp_set_params");
         }
-        MeshlessScheme s(r.cfg, &P, r.prm.adaptive_timestep);
+        MeshlessScheme s(r.cfg, &P, r.prm.adaptive_timestep, per_method);
         Logger(INFO) << "Starting time integration ...";
         s.run();
         mlh_diag d = P.diagnostics();

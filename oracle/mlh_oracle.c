@@ -1222,6 +1222,23 @@ int orc_get_face_arrays(orc *o, double dt, double *A, double *WR, double *WL, do
     return 0;
 }
 
+/* the ghost particles of the last neighbour search (createGhostParticles):
+ * parents, image codes and coordinates in ghost-index order; up to cap */
+int orc_get_ghosts(orc *o, int32_t cap, int32_t *parent, int32_t *code, double *gx, double *gy,
+                   double *gz, int32_t *n) {
+    if (o->stage < 1 || !o->p.periodic) return MLH_ERR_CALL_ORDER;
+    *n = o->NG;
+    if (o->NG > cap) return MLH_ERR_INVALID_ARGUMENT;
+    double *out[3] = {gx, gy, gz};
+    for (int k = 0; k < o->NG; ++k) {
+        if (parent) parent[k] = o->gpar[k];
+        if (code) code[k] = o->gcode[k];
+        for (int a = 0; a < o->D; ++a)
+            if (out[a]) out[a][k] = o->gx[a][k];
+    }
+    return 0;
+}
+
 /* checkFluxSymmetry (Particles.cpp:2888-2976): per pair, the reverse entry
  * and the one-sided test F_ij[c] + F_ji[c] > tol (ghost pairs: c = 0..3, as
  * the reference's ghost branch).  A missing reverse entry (the reference

@@ -50,6 +50,9 @@ int orc_gradient_field(orc *o, const double *f, double *grad);
  * [N][maxr + maxg][components] (after orc_fluxes with the same dt) */
 int orc_get_face_arrays(orc *o, double dt, double *A, double *WR, double *WL, double *vF,
                         double *F);
+/* the ghost particles of the last neighbour search, in ghost-index order */
+int orc_get_ghosts(orc *o, int32_t cap, int32_t *parent, int32_t *code, double *gx, double *gy,
+                   double *gz, int32_t *n);
 /* checkFluxSymmetry (Particles.cpp:2888-2976) on the stored F_ij */
 int orc_check_flux_symmetry(orc *o, double tol, int64_t *asym, int64_t *missing);
 

@@ -48,6 +48,7 @@ def lib():
         L.orc_conserved_sums.argtypes = [C.c_void_p, _dp]
         L.orc_get_diag.argtypes = [C.c_void_p, C.POINTER(Diag)]
         L.orc_set_bruteforce_ghosts.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_ghosts.argtypes = [C.c_void_p, C.c_int32, _ip, _ip, _dp, _dp, _dp, _ip]
         L.orc_psi_gradients.argtypes = [C.c_void_p]
         L.orc_limit.argtypes = [C.c_void_p]
         L.orc_gradient_field.argtypes = [C.c_void_p, _dp, _dp]
@@ -220,6 +221,18 @@ class Oracle:
 
     def set_bruteforce_ghosts(self, on=True):
         self.L.orc_set_bruteforce_ghosts(self.h, int(on))
+
+    def ghosts(self):
+        """(parent, code, coordinates [n][D]) of the ghost particles of the
+        last neighbour search."""
+        cap = 7 * self.N
+        par, code = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        g = [np.zeros(cap) for _ in range(3)]
+        n = C.c_int32()
+        self._chk(self.L.orc_get_ghosts(self.h, cap, _i(par), _i(code), _d(g[0]), _d(g[1]), _d(g[2]),
+                                        C.byref(n)), "get_ghosts")
+        k = n.value
+        return par[:k].copy(), code[:k].copy(), np.stack([a[:k] for a in g[: self.D]], 1)
 
     def psi_gradients(self):
         self._chk(self.L.orc_psi_gradients(self.h), "psi_gradients")

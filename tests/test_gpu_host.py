@@ -141,3 +141,33 @@ def test_adaptive_dump_schedule(tmp_path, hip):
         assert host.read_dataset(f, "/time")[0] == tk
         X = host.read_dataset(f, "/x")
         assert np.array_equal(X[:, 0], st["x"]) and np.array_equal(host.read_dataset(f, "/u"), st["u"])
+
+
+@pytest.mark.parametrize("adaptive", [0, 1])
+def test_per_method_driver_equals_default(tmp_path, hip, adaptive):
+    """MeshlessScheme with --per-method calls the Particles methods one at a
+    time in the reference's order (compPsijTilde, slopeLimiter,
+    compEffectiveFace, compRiemannStatesLR, solveRiemannProblems,
+    checkFluxSymmetry, collectFluxes, ...): every snapshot equals the
+    default run's, dataset for dataset, bitwise."""
+    if adaptive:
+        ts, te, every = 2.0 ** -6, 3 * 2.0 ** -6, 1
+    else:
+        ts, te, every = 0.002, 0.012, 3
+    _setup(tmp_path, adaptive, ts, te, every)
+    out = {}
+    for per_method in (False, True):
+        d = tmp_path / ("out_m" if per_method else "out")
+        os.makedirs(d, exist_ok=True)
+        cfg = (tmp_path / "config.info").read_text().replace("outDir out", f"outDir {d.name}")
+        (tmp_path / f"config_{int(per_method)}.info").write_text(cfg)
+        res = host.run(str(tmp_path / f"config_{int(per_method)}.info"), str(tmp_path / "parameter.h"),
+                       per_method=per_method)
+        out[per_method] = (res, sorted(os.listdir(d)), d)
+    assert out[False][0] == out[True][0]
+    assert out[False][1] == out[True][1] and len(out[False][1]) >= 3
+    for name in out[False][1]:
+        a, b = str(out[False][2] / name), str(out[True][2] / name)
+        for ds in ("/time", "/x", "/v", "/m", "/u", "/rho", "/P", "/rhoGrad", "/totalMass", "/energy"):
+            assert np.array_equal(host.read_dataset(a, ds), host.read_dataset(b, ds)), (name, ds)
+        assert np.array_equal(host.read_dataset_int(a, "/noi"), host.read_dataset_int(b, "/noi"))

@@ -140,3 +140,33 @@ def test_driver_command_line(built, tmp_path):
     r = subprocess.run([built.MLH_EXE, "-c", str(tmp_path / "missing.info")], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode != 0 and "cannot open" in r.stderr
+
+
+def test_ghost_images_rule(built):
+    """createGhostParticles of the C++ mirror (Particles.cpp:2113-2191 over D
+    axes): hand-worked images in the reference's order (corner, then y,
+    then x), and the oracle's ghost set of two workloads, bitwise."""
+    from meshlesshydro_amd import host
+    from meshlesshydro_amd._abi import default_params
+    p = default_params(kernel_size=0.1)
+    st = dict(x=np.array([0.05, 0.5, 0.95]), y=np.array([0.97, 0.5, 0.5]))
+    par, X = host.ghost_images(p, st)
+    assert list(par) == [0, 0, 0, 2]
+    want = [(1.0 + (0.05 - 0.0), 0.0 - (1.0 - 0.97)),   # corner
+            (0.05, 0.0 - (1.0 - 0.97)),                 # y image
+            (1.0 + (0.05 - 0.0), 0.97),                 # x image
+            (0.0 - (1.0 - 0.95), 0.5)]
+    assert np.array_equal(X, np.array(want))
+
+
+@pytest.mark.parametrize("case", ["khrand48", "jitter12"])
+def test_ghost_images_equal_oracle(built, oracle_mod, case):
+    from cases import CASES
+    from meshlesshydro_amd import host
+    p, s = CASES[case]()
+    par, X = host.ghost_images(p, s)
+    o = oracle_mod.Oracle(p, s)
+    o.neighbours()
+    opar, _, oX = o.ghosts()
+    assert len(par) > 0
+    assert np.array_equal(par, opar) and np.array_equal(X, oX)
