@@ -65,88 +65,104 @@ __global__ void k_mlp_pack(const float *__restrict__ P, MlpFrags *__restrict__ F
 }
 
 // ---- training --------------------------------------------------------------
-// One wavefront per chunk of 64 samples of the batch (lane = sample): the
-// forward pass, the loss term and the backward pass per lane, activations
-// staged in LDS, then the chunk's partial gradient with lane o computing
-// row o of every weight matrix as a sum over the chunk's samples in sample
-// order.  k_mlp_reduce sums the chunks' partials in chunk order and applies
-// Adam or stores the gradient: the result does not depend on scheduling and
-// needs no atomics.
+// One block of four wavefronts per chunk of 64 samples of the batch: lane l
+// of every wavefront is sample l of the chunk, and wavefront w computes the
+// quarter [16w, 16w + 16) of each hidden layer's outputs (forward) and of
+// dz1 (backward), with the activations shared through LDS.  The chunk's
+// partial gradient follows with thread p forming columns [16(p%4), +16) of
+// row p/4 of dW2 as sums over the chunk's samples in sample order.
+// k_mlp_reduce sums the chunks' partials in chunk order and applies Adam or
+// stores the gradient: the result does not depend on scheduling and needs
+// no atomics.
 //   dy = 2 e / B, dz2 = dy w3 (1 - h2^2), dz1 = (W2^T dz2)(1 - h1^2)
 //   dW3 = sum dy h2, dW2 = sum dz2 h1^T, dW1 = sum dz1 x^T, biases the sums
 constexpr int kTrainChunk = 64;
 constexpr int kTrainLd = kMlpH + 1;   // LDS row pitch (no bank conflicts)
+constexpr int kTrainQ = kMlpH / 4;    // outputs per wavefront
 constexpr int kPartialN = kMlpN + 1;  // gradient + the chunk's sum of e^2
 
-__global__ __launch_bounds__(64) void k_mlp_chunk_grad(const float *__restrict__ P,
-                                                       const float *__restrict__ X,
-                                                       const float *__restrict__ T,
-                                                       const int *__restrict__ idx, int B,
-                                                       float *__restrict__ partial) {
+__global__ __launch_bounds__(256) void k_mlp_chunk_grad(const float *__restrict__ P,
+                                                        const float *__restrict__ X,
+                                                        const float *__restrict__ T,
+                                                        const int *__restrict__ idx, int B,
+                                                        float *__restrict__ partial) {
     __shared__ float H1[kTrainChunk][kTrainLd];
     __shared__ float H2[kTrainChunk][kTrainLd];  // h2, then dz1
     __shared__ float DZ2[kTrainChunk][kTrainLd];
     __shared__ float XS[kTrainChunk][kMlpIn];
+    __shared__ float YP[4][kTrainChunk];
     __shared__ float DY[kTrainChunk], E2[kTrainChunk];
-    const int l = threadIdx.x;
+    const int tid = threadIdx.x, l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wavefront-uniform
+    const int o0 = kTrainQ * w;
     const int b = blockIdx.x * kTrainChunk + l;
     const bool act = b < B;
     const int s = act ? (idx ? idx[b] : b) : 0;
-    float x[kMlpIn];
-#pragma unroll
-    for (int k = 0; k < kMlpIn; ++k) {
-        x[k] = act ? X[(size_t)s * kMlpIn + k] : 0.f;
-        XS[l][k] = x[k];
-    }
     // register arrays are indexed with compile-time indices only (unrolled
     // loops), so they stay in registers
-    float h1[kMlpH];
+    float x[kMlpIn];
 #pragma unroll
-    for (int o = 0; o < kMlpH; ++o) {
+    for (int k = 0; k < kMlpIn; ++k) x[k] = act ? X[(size_t)s * kMlpIn + k] : 0.f;
+    if (w == 0)
+        for (int k = 0; k < kMlpIn; ++k) XS[l][k] = x[k];
+    for (int oo = 0; oo < kTrainQ; ++oo) {
+        const int o = o0 + oo;
         float a = P[kMlpB1 + o];
 #pragma unroll
         for (int k = 0; k < kMlpIn; ++k) a += P[kMlpW1 + o * kMlpIn + k] * x[k];
-        h1[o] = tanhf(a);
-        H1[l][o] = h1[o];
+        H1[l][o] = tanhf(a);
     }
-    float y = P[kMlpB3];
-    for (int o = 0; o < kMlpH; ++o) {
+    __syncthreads();
+    float yp = 0.f;
+    for (int oo = 0; oo < kTrainQ; ++oo) {
+        const int o = o0 + oo;
         float a = P[kMlpB2 + o];
-#pragma unroll
-        for (int k = 0; k < kMlpH; ++k) a += P[kMlpW2 + o * kMlpH + k] * h1[k];
+        for (int k = 0; k < kMlpH; ++k) a += P[kMlpW2 + o * kMlpH + k] * H1[l][k];
         const float h2 = tanhf(a);
         H2[l][o] = h2;
-        y += P[kMlpW3 + o] * h2;
+        yp += P[kMlpW3 + o] * h2;
     }
+    YP[w][l] = yp;
+    __syncthreads();
     // samples past the batch end contribute nothing (dy = 0)
+    const float y = P[kMlpB3] + ((YP[0][l] + YP[1][l]) + (YP[2][l] + YP[3][l]));
     const float e = act ? y - T[s] : 0.f;
     const float dy = 2.f * e / (float)B;
-    DY[l] = dy;
-    E2[l] = e * e;
-    for (int o = 0; o < kMlpH; ++o) {
+    if (w == 0) {
+        DY[l] = dy;
+        E2[l] = e * e;
+    }
+    for (int oo = 0; oo < kTrainQ; ++oo) {
+        const int o = o0 + oo;
         const float h2 = H2[l][o];
         DZ2[l][o] = dy * P[kMlpW3 + o] * (1.f - h2 * h2);
     }
     __syncthreads();
     float *g = partial + (size_t)blockIdx.x * kPartialN;
     {
-        // lane o: dW3[o], db2[o], row o of dW2; lane 0 also db3 and sum e^2
-        const int o = l;
-        float w3 = 0.f, bb = 0.f, acc[kMlpH];
+        // thread (o, kq): columns [16 kq, 16 kq + 16) of row o of dW2; the
+        // kq = 0 threads also dW3[o] and db2[o]; thread 0 db3 and sum e^2
+        const int o = tid >> 2, kq = tid & 3;
+        float acc[kTrainQ];
 #pragma unroll
-        for (int k = 0; k < kMlpH; ++k) acc[k] = 0.f;
+        for (int k = 0; k < kTrainQ; ++k) acc[k] = 0.f;
         for (int q = 0; q < kTrainChunk; ++q) {
             const float d = DZ2[q][o];
-            w3 += DY[q] * H2[q][o];
-            bb += d;
 #pragma unroll
-            for (int k = 0; k < kMlpH; ++k) acc[k] += d * H1[q][k];
+            for (int k = 0; k < kTrainQ; ++k) acc[k] += d * H1[q][kTrainQ * kq + k];
         }
-        g[kMlpW3 + o] = w3;
-        g[kMlpB2 + o] = bb;
 #pragma unroll
-        for (int k = 0; k < kMlpH; ++k) g[kMlpW2 + o * kMlpH + k] = acc[k];
-        if (l == 0) {
+        for (int k = 0; k < kTrainQ; ++k) g[kMlpW2 + o * kMlpH + kTrainQ * kq + k] = acc[k];
+        if (kq == 0) {
+            float w3 = 0.f, bb = 0.f;
+            for (int q = 0; q < kTrainChunk; ++q) {
+                w3 += DY[q] * H2[q][o];
+                bb += DZ2[q][o];
+            }
+            g[kMlpW3 + o] = w3;
+            g[kMlpB2 + o] = bb;
+        }
+        if (tid == 0) {
             float b3 = 0.f, se = 0.f;
             for (int q = 0; q < kTrainChunk; ++q) {
                 b3 += DY[q];
@@ -157,15 +173,16 @@ __global__ __launch_bounds__(64) void k_mlp_chunk_grad(const float *__restrict__
         }
     }
     __syncthreads();  // H2 is read above, dz1 goes there below
-    for (int k = 0; k < kMlpH; ++k) {
+    for (int kk = 0; kk < kTrainQ; ++kk) {
+        const int k = o0 + kk;
         float a = 0.f;
         for (int o = 0; o < kMlpH; ++o) a += P[kMlpW2 + o * kMlpH + k] * DZ2[l][o];
         const float hk = H1[l][k];
         H2[l][k] = a * (1.f - hk * hk);
     }
     __syncthreads();
-    {
-        const int o = l;
+    if (tid < kMlpH) {
+        const int o = tid;
         float bb = 0.f, acc[kMlpIn];
 #pragma unroll
         for (int k = 0; k < kMlpIn; ++k) acc[k] = 0.f;
@@ -422,8 +439,8 @@ static void enqueue_grad(mlh_ctx *c, TrainBufs &b, const int *idx, int B, bool a
     MlpState &m = ctx_mlp(c);
     hipStream_t s = ctx_stream(c);
     const int nch = train_chunks(B);
-    hipLaunchKernelGGL(k_mlp_chunk_grad, dim3(nch), dim3(kTrainChunk), 0, s, m.params, b.X, b.T,
-                       idx, B, b.partial);
+    hipLaunchKernelGGL(k_mlp_chunk_grad, dim3(nch), dim3(256), 0, s, m.params, b.X, b.T, idx, B,
+                       b.partial);
     hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartialN + 255) / 256), dim3(256), 0, s, b.partial,
                        nch, B, m.grad, b.loss, adam ? 1 : 0, m.params, m.adam_m, m.adam_v, lr,
                        b1c, b2c);
