@@ -347,17 +347,72 @@ int mlh_mlp_features(int64_t n, double gamma, const double *rhoL,
                      const double *uL, const double *PL, const double *rhoR,
                      const double *uR, const double *PR, float *feats,
                      double *p_ref);
-/* Set / get the network parameters (FP32, layout documented in DESIGN.md). */
+/* Set / get the network parameters: MLH_MLP_NPARAM FP32 values W1[64][10],
+ * b1[64], W2[64][64], b2[64], W3[64], b3.  Setting them resets Adam's
+ * moments. */
 int mlh_mlp_set_params(mlh_ctx *ctx, const float *params);
 int mlh_mlp_get_params(mlh_ctx *ctx, float *params);
 /* Forward + backward on a batch (device kernels), loss = mean squared
  * error of log p*; returns the loss and the gradient w.r.t. all params. */
 int mlh_mlp_loss_grad(mlh_ctx *ctx, int64_t n, const float *feats,
                       const float *target, float *loss, float *grad);
-/* Train with Adam for n_epochs over (feats, target) resident on the device. */
+/* Train with Adam for n_epochs over (feats, target), copied to the device
+ * once; each epoch visits the samples in an order fixed by (seed, epoch)
+ * in batches of `batch` (a partial last batch is left out). */
 int mlh_mlp_train(mlh_ctx *ctx, int64_t n, const float *feats,
                   const float *target, int32_t n_epochs, int32_t batch,
                   float lr, uint64_t seed, float *final_loss);
+
+/* Training set and training of the seed network built from face problems
+ * on the device.  No counterpart in the reference, which has no learned
+ * seed: these replace the host-side set construction (exact FP64 p*,
+ * features, targets, subsampling of trivial faces, shuffling) so that an
+ * online retraining needs no copy of the record to the host.
+ *   - faces with a non-positive or non-finite input state, or p* = 0, are
+ *     dropped; the target is log(p* / p_ref) - f7 in FP32, p* from the
+ *     plain FP64 exact solver at tol 1e-12 (mlh_riemann_pstar_batch);
+ *   - faces with |target| < 1e-5 (the standard guess converged) are
+ *     trivial; all non-trivial faces are kept, and each trivial one with
+ *     probability keep / n_trivial, keep = min(n_trivial,
+ *     max(keep_trivial * max(n_nontrivial, 1), 1024)), decided by a hash
+ *     of the face's six values;
+ *   - the kept faces are ordered by that hash, so the set does not depend
+ *     on the order of the input (the record's order is not deterministic);
+ *     the first max(n / 10, 64) are the validation faces;
+ *   - training continues from the current parameters and Adam moments.
+ * With fewer than 256 kept faces nothing is trained (train_faces = 0). */
+typedef struct mlh_train_info {
+    int64_t faces;          /* face problems in the input                    */
+    int64_t valid;          /* not dropped                                  */
+    int64_t nontrivial;     /* |target| >= 1e-5                             */
+    int64_t trivial_kept;   /* trivial faces kept by the subsampling        */
+    int64_t train_faces;    /* 0: too few faces, nothing trained            */
+    int64_t val_faces;
+    int32_t epochs, batch;
+    float loss_init;        /* MSE over the set's first min(n, 65536) faces */
+    float loss_final;       /* the last batch's loss                        */
+    double val_mse_log_pstar;       /* BF16 network on the validation faces */
+    double val_mse_standard_guess;  /* the standard guess there (target 0) */
+} mlh_train_info;
+/* On n faces states6[n][6] (rhoL, uL, PL, rhoR, uR, PR) copied from the host. */
+int mlh_mlp_train_faces(mlh_ctx *ctx, int64_t n, const double *states6,
+                        double gamma, int32_t n_epochs, int32_t batch,
+                        float lr, uint64_t seed, double keep_trivial,
+                        mlh_train_info *info);
+/* On the faces this context recorded (mlh_set_face_collection), at the
+ * context's gamma; restarts the record. */
+int mlh_mlp_train_recorded(mlh_ctx *ctx, int32_t n_epochs, int32_t batch,
+                           float lr, uint64_t seed, double keep_trivial,
+                           mlh_train_info *info);
+/* The training set alone, in its order: of states6[n][6], or of the record
+ * when states6 is null (the record is not restarted).  Copies the first
+ * min(set size, cap) entries to feats[.][10], target and (if not null)
+ * rows, each kept face's input row (for the record: its row in the order
+ * mlh_get_collected_faces copies it out), and the set size to *n_out. */
+int mlh_mlp_dataset(mlh_ctx *ctx, int64_t n, const double *states6,
+                    double gamma, double keep_trivial, int64_t cap,
+                    float *feats, float *target, int64_t *rows,
+                    int64_t *n_out);
 /* Inference: FP32 reference path (bf16 = 0) or BF16 MFMA path (bf16 = 1). */
 int mlh_mlp_infer(mlh_ctx *ctx, int64_t n, const float *feats, float *out,
                   int32_t bf16);

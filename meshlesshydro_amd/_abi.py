@@ -62,6 +62,27 @@ class Params(C.Structure):
     ]
 
 
+class TrainInfo(C.Structure):
+    """mlh_train_info: one training of the seed network on a device-built set."""
+    _fields_ = [
+        ("faces", C.c_int64),
+        ("valid", C.c_int64),
+        ("nontrivial", C.c_int64),
+        ("trivial_kept", C.c_int64),
+        ("train_faces", C.c_int64),
+        ("val_faces", C.c_int64),
+        ("epochs", C.c_int32),
+        ("batch", C.c_int32),
+        ("loss_init", C.c_float),
+        ("loss_final", C.c_float),
+        ("val_mse_log_pstar", C.c_double),
+        ("val_mse_standard_guess", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Diag(C.Structure):
     _fields_ = [
         ("neg_face_pressure_real", C.c_int64),

@@ -1102,4 +1102,5 @@ int mlh_get_timings(mlh_ctx *c, float *ms4) {
 // internal accessors for the MLP module
 MlpState &ctx_mlp(mlh_ctx *c) { return c->mlp; }
 hipStream_t ctx_stream(mlh_ctx *c) { return c ? c->s : nullptr; }
+double ctx_gamma(mlh_ctx *c) { return c->p.gamma; }
 int ctx_fail(int code, const char *msg) { return fail(code, msg); }

@@ -124,6 +124,10 @@ void launch_pack_pos(const double *x0, const double *x1, const double *x2, int D
                      double4 *pos, hipStream_t s);
 
 size_t radix_sort_scratch_words(int N);
+// stable LSD sort of (k0, v0) by the low `bits` bits of k0, result in
+// (k0, v0); (k1, v1) ping-pong space, scratch radix_sort_scratch_words(N)
+void radix_sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int N,
+                      int bits, uint32_t *scratch, hipStream_t s);
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *tmp,
                         hipStream_t s);
 

@@ -17,6 +17,7 @@
 #include <vector>
 #include "../../include/mlh.h"
 #include "device_common.h"
+#include "kernels.h"
 #include "mlp_api.h"
 #include "mlp_device.h"
 #include "mlp_mfma.h"
@@ -338,6 +339,152 @@ __global__ __launch_bounds__(256) void k_sum_u64(const unsigned long long *__res
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+// ---- training sets built on the device -------------------------------------
+// The learned seed's training set is built from face problems (rhoL, uL,
+// PL, rhoR, uR, PR) already on the device, the run's record or an uploaded
+// array, without a round trip through the host:
+//   1. k_ds_prep: per face, the FP64 exact p* (tol 1e-12, at most 50
+//      iterations, as mlh_riemann_pstar_batch), the FP32 features, the target
+//      log(p* / p_ref) - f7 in FP32 and a class: dropped (a non-positive or
+//      non-finite input state, or p* = 0), trivial (|target| < 1e-5: the
+//      standard guess is converged) or non-trivial.
+//   2. k_ds_select keeps every non-trivial face and each trivial one with
+//      probability keep / n_trivial, keep = min(n_trivial, max(keep_trivial
+//      * max(n_nontrivial, 1), 1024)), decided by a hash of the face.
+//   3. The kept faces are sorted by their content hash h (by h >> 33, then
+//      by the low 32 bits), which fixes the set's order whatever order the
+//      record was filled in; the first max(n / 10, 64) are the validation
+//      faces, the rest the training faces.
+//   4. Each epoch visits the training faces in the order of a hash of
+//      (seed, epoch, position), sorted the same way.
+// Every step is deterministic, so a retraining gives the same weights
+// whenever the same faces were recorded.
+
+// splitmix64's output function (Steele, Lea and Flood, OOPSLA 2014)
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+constexpr uint64_t kSelectSalt = 0x5851f42d4c957f2dull;
+constexpr uint64_t kEpochSalt = 0xa0761d6478bd642full;
+constexpr float kTrivialTarget = 1e-5f;
+constexpr int kMinTrainFaces = 256;
+
+// face slot k of the input: counts null -> every slot [0, nslots) holds a
+// face; else slot k is row r = k % scap of stripe s = k / scap, filled when
+// r < counts[s]
+__global__ __launch_bounds__(256) void k_ds_prep(const double *__restrict__ F6,
+                                                  const unsigned long long *__restrict__ counts,
+                                                  int64_t scap, int64_t nslots, double gamma,
+                                                  float *__restrict__ Xall,
+                                                  float *__restrict__ Tall,
+                                                  uint32_t *__restrict__ hlo,
+                                                  uint32_t *__restrict__ hhi,
+                                                  uint8_t *__restrict__ cls,
+                                                  unsigned long long *__restrict__ ncls) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool used = k < nslots;
+    if (used && counts) used = (unsigned long long)(k % scap) < counts[k / scap];
+    uint8_t cl = 0;
+    uint64_t h = ~0ull;
+    if (used) {
+        double a[6];
+        h = 0;
+        for (int q = 0; q < 6; ++q) {
+            a[q] = F6[6 * k + q];
+            h = mix64(h ^ (uint64_t)__double_as_longlong(a[q]));
+        }
+        bool ok = a[0] > 0. && a[3] > 0. && a[2] > 0. && a[5] > 0.;
+        for (int q = 0; q < 6; ++q) ok = ok && isfinite(a[q]);
+        if (ok) {
+            GasConst<double> gc(gamma);
+            SolveStats st{};
+            const double ps = star_pressure_fp64(gc, a[0], a[1], a[2], a[3], a[4], a[5], 1e-12,
+                                                 50, st);
+            if (ps > 0.) {
+                float f[kMlpIn];
+                const float pref = mlp_features((float)gamma, (float)a[0], (float)a[1],
+                                                (float)a[2], (float)a[3], (float)a[4],
+                                                (float)a[5], f);
+                const float t = (float)(log(ps / (double)pref) - (double)f[7]);
+                cl = fabsf(t) < kTrivialTarget ? 1 : 2;
+                for (int q = 0; q < kMlpIn; ++q) Xall[(size_t)k * kMlpIn + q] = f[q];
+                Tall[k] = t;
+            }
+        }
+    }
+    if (k < nslots) {
+        cls[k] = cl;
+        hlo[k] = (uint32_t)h;
+        hhi[k] = (uint32_t)(h >> 32);
+    }
+    // class counts: one atomic per wavefront and class
+    const unsigned long long n1 = __popcll(__ballot(cl == 1)), n2 = __popcll(__ballot(cl == 2));
+    if ((threadIdx.x & 63) == 0) {
+        if (n1) atomicAdd(&ncls[0], n1);
+        if (n2) atomicAdd(&ncls[1], n2);
+    }
+}
+
+// selection (trivial faces kept when the top 53 bits of a second hash are
+// below thr53) and the high sort key: (h >> 33), with bit 31 set for the
+// faces that are not kept
+__global__ __launch_bounds__(256) void k_ds_select(const uint8_t *__restrict__ cls,
+                                                    const uint32_t *__restrict__ hlo,
+                                                    const uint32_t *__restrict__ hhi,
+                                                    int64_t nslots, uint64_t thr53,
+                                                    uint32_t *__restrict__ khi,
+                                                    unsigned long long *__restrict__ nsel) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool sel = false;
+    if (k < nslots) {
+        const uint8_t cl = cls[k];
+        const uint64_t h = (uint64_t)hlo[k] | ((uint64_t)hhi[k] << 32);
+        sel = cl == 2 || (cl == 1 && (mix64(h ^ kSelectSalt) >> 11) < thr53);
+        khi[k] = (hhi[k] >> 1) | (sel ? 0u : 0x80000000u);
+    }
+    const unsigned long long n = __popcll(__ballot(sel));
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(nsel, n);
+}
+
+__global__ __launch_bounds__(256) void k_iota_u32(uint32_t *__restrict__ v, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) v[k] = (uint32_t)k;
+}
+
+// out[k] = in[idx[k]]
+__global__ __launch_bounds__(256) void k_gather_u32(const uint32_t *__restrict__ in,
+                                                     const uint32_t *__restrict__ idx, int64_t n,
+                                                     uint32_t *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = in[idx[k]];
+}
+
+// the kept faces' features and targets in sorted order
+__global__ __launch_bounds__(256) void k_ds_gather(const uint32_t *__restrict__ order, int64_t n,
+                                                    const float *__restrict__ Xall,
+                                                    const float *__restrict__ Tall,
+                                                    float *__restrict__ X, float *__restrict__ T) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t s = order[k];
+    for (int q = 0; q < kMlpIn; ++q) X[(size_t)k * kMlpIn + q] = Xall[s * kMlpIn + q];
+    T[k] = Tall[s];
+}
+
+// epoch permutation keys: key[i] = top 32 bits of a hash of (seed, epoch,
+// i), val[i] = first + i
+__global__ __launch_bounds__(256) void k_epoch_keys(uint32_t *__restrict__ key,
+                                                     uint32_t *__restrict__ val, int64_t n,
+                                                     uint32_t first, uint64_t epoch_hash) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    key[k] = (uint32_t)(mix64(epoch_hash ^ (uint64_t)k) >> 32);
+    val[k] = first + (uint32_t)k;
+}
+
 }  // namespace mlh
 
 // ---- C ABI --------------------------------------------------------------------
@@ -351,6 +498,15 @@ template <typename P>
 static hipError_t dmalloc(P **p, size_t bytes) {
     return hipMalloc((void **)p, bytes ? bytes : 16);
 }
+
+// device buffer freed at scope exit
+template <typename P>
+struct DevBuf {
+    P *p = nullptr;
+    ~DevBuf() {
+        if (p) hipFree(p);
+    }
+};
 
 static int ensure_mlp(mlh_ctx *c) {
     MlpState &m = ctx_mlp(c);
@@ -413,9 +569,14 @@ int mlh_mlp_get_params(mlh_ctx *c, float *params) {
 struct TrainBufs {
     float *X = nullptr, *T = nullptr, *partial = nullptr, *loss = nullptr;
     int *idx = nullptr;
+    // epoch permutation: sort keys and the radix sort's ping-pong / scratch
+    uint32_t *pk = nullptr, *pk1 = nullptr, *pv1 = nullptr, *scr = nullptr;
     ~TrainBufs() {
         float *f[] = {X, T, partial, loss};
         for (float *p : f)
+            if (p) hipFree(p);
+        uint32_t *u[] = {pk, pk1, pv1, scr};
+        for (uint32_t *p : u)
             if (p) hipFree(p);
         if (idx) hipFree(idx);
     }
@@ -423,12 +584,17 @@ struct TrainBufs {
 
 static int train_chunks(int batch) { return (batch + kTrainChunk - 1) / kTrainChunk; }
 
-static int alloc_train(TrainBufs &b, int64_t n, int batch) {
+// n samples; partial sums for batches of up to max_batch samples
+static int alloc_train(TrainBufs &b, int64_t n, int max_batch) {
     CHK(dmalloc(&b.X, sizeof(float) * n * kMlpIn));
     CHK(dmalloc(&b.T, sizeof(float) * n));
-    CHK(dmalloc(&b.partial, sizeof(float) * (size_t)train_chunks(batch) * kPartialN));
+    CHK(dmalloc(&b.partial, sizeof(float) * (size_t)train_chunks(max_batch) * kPartialN));
     CHK(dmalloc(&b.loss, sizeof(float)));
     CHK(dmalloc(&b.idx, sizeof(int) * n));
+    CHK(dmalloc(&b.pk, sizeof(uint32_t) * n));
+    CHK(dmalloc(&b.pk1, sizeof(uint32_t) * n));
+    CHK(dmalloc(&b.pv1, sizeof(uint32_t) * n));
+    CHK(dmalloc(&b.scr, sizeof(uint32_t) * radix_sort_scratch_words((int)n)));
     return MLH_OK;
 }
 
@@ -444,6 +610,162 @@ static void enqueue_grad(mlh_ctx *c, TrainBufs &b, const int *idx, int B, bool a
     hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartialN + 255) / 256), dim3(256), 0, s, b.partial,
                        nch, B, m.grad, b.loss, adam ? 1 : 0, m.params, m.adam_m, m.adam_v, lr,
                        b1c, b2c);
+}
+
+// Adam epochs over the samples [first, first + n) of b.X / b.T in batches
+// of `batch` (a partial last batch is left out), each epoch in the order of
+// k_epoch_keys; *last = the last batch's loss (0 without epochs).  Repacks
+// the BF16 fragments and synchronises.
+static int train_epochs(mlh_ctx *c, TrainBufs &b, int64_t first, int64_t n, int n_epochs,
+                        int batch, float lr, uint64_t seed, float *last) {
+    MlpState &m = ctx_mlp(c);
+    hipStream_t s = ctx_stream(c);
+    const int64_t nb = (n + 255) / 256;
+    *last = 0.f;
+    for (int ep = 0; ep < n_epochs; ++ep) {
+        const uint64_t eh = mix64(mix64(seed ^ kEpochSalt) + (uint64_t)ep);
+        hipLaunchKernelGGL(k_epoch_keys, dim3(nb), dim3(256), 0, s, b.pk, (uint32_t *)b.idx, n,
+                           (uint32_t)first, eh);
+        radix_sort_pairs(b.pk, (uint32_t *)b.idx, b.pk1, b.pv1, (int)n, 32, b.scr, s);
+        for (int64_t off = 0; off + batch <= n; off += batch) {
+            m.adam_t++;
+            float b1c = 1.f - powf(0.9f, (float)m.adam_t);
+            float b2c = 1.f - powf(0.999f, (float)m.adam_t);
+            enqueue_grad(c, b, b.idx + off, batch, true, lr, b1c, b2c);
+        }
+    }
+    CHK(hipGetLastError());
+    if (n_epochs > 0) CHK(hipMemcpyAsync(last, b.loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    repack(c);
+    CHK(hipStreamSynchronize(s));
+    return MLH_OK;
+}
+
+struct DsCounts {
+    int64_t faces = 0, trivial = 0, nontrivial = 0, nsel = 0;
+};
+
+// Builds the training set of the faces at F6 (slots as k_ds_prep) into b,
+// whose partial sums hold batches of up to max(max_batch, 65536) samples.
+// slots (optional): the input slot of each kept face, in the set's order.
+static int build_set(mlh_ctx *c, const double *F6, const unsigned long long *counts,
+                     int64_t scap, int64_t nslots, double gamma, double keep_trivial,
+                     int max_batch, TrainBufs &b, DsCounts &dc,
+                     std::vector<uint32_t> *slots = nullptr) {
+    if (nslots <= 0 || nslots > INT32_MAX)
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "face count out of range");
+    hipStream_t s = ctx_stream(c);
+    DevBuf<float> Xall, Tall;
+    DevBuf<uint32_t> hlo, hhi, khi, ord, k1, v1, scr;
+    DevBuf<uint8_t> cls;
+    DevBuf<unsigned long long> cnt;  // trivial, non-trivial, kept, faces
+    CHK(dmalloc(&Xall.p, sizeof(float) * kMlpIn * nslots));
+    CHK(dmalloc(&Tall.p, sizeof(float) * nslots));
+    uint32_t **u[] = {&hlo.p, &hhi.p, &khi.p, &ord.p, &k1.p, &v1.p};
+    for (uint32_t **q : u) CHK(dmalloc(q, sizeof(uint32_t) * nslots));
+    CHK(dmalloc(&scr.p, sizeof(uint32_t) * radix_sort_scratch_words((int)nslots)));
+    CHK(dmalloc(&cls.p, nslots));
+    CHK(dmalloc(&cnt.p, sizeof(unsigned long long) * 3));
+    CHK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 3, s));
+    const int64_t nb = (nslots + 255) / 256;
+    hipLaunchKernelGGL(k_ds_prep, dim3(nb), dim3(256), 0, s, F6, counts, scap, nslots, gamma,
+                       Xall.p, Tall.p, hlo.p, hhi.p, cls.p, cnt.p);
+    unsigned long long h[3] = {};
+    CHK(hipMemcpyAsync(h, cnt.p, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, s));
+    CHK(hipStreamSynchronize(s));
+    dc.trivial = (int64_t)h[0];
+    dc.nontrivial = (int64_t)h[1];
+    if (counts) {
+        unsigned long long cn[kCollectStripes] = {};
+        const int ns = (int)(nslots / scap);
+        CHK(hipMemcpy(cn, counts, sizeof(unsigned long long) * ns, hipMemcpyDeviceToHost));
+        dc.faces = 0;
+        for (int q = 0; q < ns; ++q)
+            dc.faces += cn[q] < (unsigned long long)scap ? (int64_t)cn[q] : scap;
+    } else {
+        dc.faces = nslots;
+    }
+    // trivial faces kept: keep = min(n_trivial, max(keep_trivial *
+    // max(n_nontrivial, 1), 1024)), each with probability keep / n_trivial
+    int64_t keep = (int64_t)(keep_trivial * (double)(dc.nontrivial > 1 ? dc.nontrivial : 1));
+    if (keep < 1024) keep = 1024;
+    if (keep > dc.trivial) keep = dc.trivial;
+    uint64_t thr53 = 0;
+    if (dc.trivial > 0)
+        thr53 = keep >= dc.trivial ? (1ull << 53)
+                                   : (uint64_t)((double)keep / (double)dc.trivial * 9007199254740992.0);
+    hipLaunchKernelGGL(k_ds_select, dim3(nb), dim3(256), 0, s, cls.p, hlo.p, hhi.p, nslots, thr53,
+                       khi.p, cnt.p + 2);
+    // sort by (khi, hlo): the low word first, then stably by the high one
+    hipLaunchKernelGGL(k_iota_u32, dim3(nb), dim3(256), 0, s, ord.p, nslots);
+    radix_sort_pairs(hlo.p, ord.p, k1.p, v1.p, (int)nslots, 32, scr.p, s);
+    hipLaunchKernelGGL(k_gather_u32, dim3(nb), dim3(256), 0, s, khi.p, ord.p, nslots, hhi.p);
+    radix_sort_pairs(hhi.p, ord.p, k1.p, v1.p, (int)nslots, 32, scr.p, s);
+    CHK(hipMemcpyAsync(h + 2, cnt.p + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    CHK(hipStreamSynchronize(s));
+    dc.nsel = (int64_t)h[2];
+    int st = alloc_train(b, dc.nsel, max_batch > 65536 ? max_batch : 65536);
+    if (st) return st;
+    if (dc.nsel > 0)
+        hipLaunchKernelGGL(k_ds_gather, dim3((dc.nsel + 255) / 256), dim3(256), 0, s, ord.p,
+                           dc.nsel, Xall.p, Tall.p, b.X, b.T);
+    CHK(hipGetLastError());
+    if (slots) {
+        slots->resize(dc.nsel);
+        if (dc.nsel > 0)
+            CHK(hipMemcpyAsync(slots->data(), ord.p, sizeof(uint32_t) * dc.nsel,
+                               hipMemcpyDeviceToHost, s));
+    }
+    CHK(hipStreamSynchronize(s));  // before the temporaries are freed
+    return MLH_OK;
+}
+
+// Loss over the first min(n, 65536) faces of the set, Adam epochs over the
+// training faces, and the validation MSE of the BF16 network against the
+// standard guess (target 0) on the validation faces.  Fewer than
+// kMinTrainFaces faces in the set: nothing is trained, train_faces = 0.
+static int train_on_set(mlh_ctx *c, TrainBufs &b, const DsCounts &dc, int n_epochs, int batch,
+                        float lr, uint64_t seed, mlh_train_info *info) {
+    MlpState &m = ctx_mlp(c);
+    hipStream_t s = ctx_stream(c);
+    *info = mlh_train_info{};
+    info->faces = dc.faces;
+    info->valid = dc.trivial + dc.nontrivial;
+    info->nontrivial = dc.nontrivial;
+    info->trivial_kept = dc.nsel - dc.nontrivial;
+    info->epochs = n_epochs;
+    const int64_t n = dc.nsel;
+    if (n < kMinTrainFaces) return MLH_OK;
+    const int64_t nval = n / 10 > 64 ? n / 10 : 64, ntrain = n - nval;
+    if (batch > ntrain) batch = (int)ntrain;
+    const int ninit = (int)(n < 65536 ? n : 65536);
+    enqueue_grad(c, b, nullptr, ninit);
+    float loss0 = 0.f, last = 0.f;
+    CHK(hipMemcpyAsync(&loss0, b.loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    int st = train_epochs(c, b, nval, ntrain, n_epochs, batch, lr, seed, &last);
+    if (st) return st;
+    DevBuf<float> Y;
+    CHK(dmalloc(&Y.p, sizeof(float) * nval));
+    hipLaunchKernelGGL(k_mlp_infer_bf16, dim3((nval + 255) / 256), dim3(256), 0, s, m.frags, b.X,
+                       nval, Y.p);
+    std::vector<float> y(nval), t(nval);
+    CHK(hipMemcpyAsync(y.data(), Y.p, sizeof(float) * nval, hipMemcpyDeviceToHost, s));
+    CHK(hipMemcpyAsync(t.data(), b.T, sizeof(float) * nval, hipMemcpyDeviceToHost, s));
+    CHK(hipStreamSynchronize(s));
+    double e_net = 0., e_std = 0.;
+    for (int64_t k = 0; k < nval; ++k) {
+        const double d = (double)y[k] - (double)t[k];
+        e_net += d * d;
+        e_std += (double)t[k] * (double)t[k];
+    }
+    info->train_faces = ntrain;
+    info->val_faces = nval;
+    info->batch = batch;
+    info->loss_init = loss0;
+    info->loss_final = last;
+    info->val_mse_log_pstar = e_net / (double)nval;
+    info->val_mse_standard_guess = e_std / (double)nval;
+    return MLH_OK;
 }
 
 int mlh_mlp_loss_grad(mlh_ctx *c, int64_t n, const float *feats, const float *target,
@@ -467,40 +789,121 @@ int mlh_mlp_loss_grad(mlh_ctx *c, int64_t n, const float *feats, const float *ta
 
 int mlh_mlp_train(mlh_ctx *c, int64_t n, const float *feats, const float *target,
                   int32_t n_epochs, int32_t batch, float lr, uint64_t seed, float *final_loss) {
-    if (!c || n <= 0 || !feats || !target || batch <= 0 || n_epochs < 0)
+    if (!c || n <= 0 || n > INT32_MAX || !feats || !target || batch <= 0 || n_epochs < 0)
         return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
     int st = ensure_mlp(c);
     if (st) return st;
-    MlpState &m = ctx_mlp(c);
     hipStream_t s = ctx_stream(c);
     if (batch > n) batch = (int)n;
     TrainBufs b;
     if ((st = alloc_train(b, n, batch))) return st;
     CHK(hipMemcpyAsync(b.X, feats, sizeof(float) * n * kMlpIn, hipMemcpyHostToDevice, s));
     CHK(hipMemcpyAsync(b.T, target, sizeof(float) * n, hipMemcpyHostToDevice, s));
-    std::vector<int> perm(n);
-    uint64_t rng = seed * 6364136223846793005ull + 1442695040888963407ull;
     float last = 0.f;
-    for (int ep = 0; ep < n_epochs; ++ep) {
-        for (int64_t k = 0; k < n; ++k) perm[k] = (int)k;
-        for (int64_t k = n - 1; k > 0; --k) {  // Fisher-Yates with an LCG
-            rng = rng * 6364136223846793005ull + 1442695040888963407ull;
-            int64_t j = (int64_t)((rng >> 33) % (uint64_t)(k + 1));
-            std::swap(perm[k], perm[j]);
-        }
-        CHK(hipMemcpyAsync(b.idx, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-        for (int64_t off = 0; off + batch <= n; off += batch) {
-            m.adam_t++;
-            float b1c = 1.f - powf(0.9f, (float)m.adam_t);
-            float b2c = 1.f - powf(0.999f, (float)m.adam_t);
-            enqueue_grad(c, b, b.idx + off, batch, true, lr, b1c, b2c);
-        }
-        CHK(hipMemcpyAsync(&last, b.loss, sizeof(float), hipMemcpyDeviceToHost, s));
-        CHK(hipStreamSynchronize(s));
-    }
-    repack(c);
-    CHK(hipStreamSynchronize(s));
+    if ((st = train_epochs(c, b, 0, n, n_epochs, batch, lr, seed, &last))) return st;
     if (final_loss) *final_loss = last;
+    return MLH_OK;
+}
+
+int mlh_mlp_train_faces(mlh_ctx *c, int64_t n, const double *states6, double gamma,
+                        int32_t n_epochs, int32_t batch, float lr, uint64_t seed,
+                        double keep_trivial, mlh_train_info *info) {
+    if (!c || n <= 0 || n > INT32_MAX || !states6 || batch <= 0 || n_epochs < 0 || !info ||
+        !(keep_trivial >= 0.))
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
+    MlpState &m = ctx_mlp(c);
+    if (!m.params) return ctx_fail(MLH_ERR_NOT_TRAINED, "no network parameters to train from");
+    hipStream_t s = ctx_stream(c);
+    DevBuf<double> F6;
+    CHK(dmalloc(&F6.p, sizeof(double) * 6 * n));
+    CHK(hipMemcpyAsync(F6.p, states6, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s));
+    TrainBufs b;
+    DsCounts dc;
+    int st = build_set(c, F6.p, nullptr, n, n, gamma, keep_trivial, batch, b, dc);
+    if (st) return st;
+    return train_on_set(c, b, dc, n_epochs, batch, lr, seed, info);
+}
+
+int mlh_mlp_train_recorded(mlh_ctx *c, int32_t n_epochs, int32_t batch, float lr, uint64_t seed,
+                           double keep_trivial, mlh_train_info *info) {
+    if (!c || batch <= 0 || n_epochs < 0 || !info || !(keep_trivial >= 0.))
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
+    MlpState &m = ctx_mlp(c);
+    if (!m.params) return ctx_fail(MLH_ERR_NOT_TRAINED, "no network parameters to train from");
+    if (!m.collect || !m.collect_n || m.collect_stripes <= 0)
+        return ctx_fail(MLH_ERR_CALL_ORDER, "no face record (mlh_set_face_collection)");
+    hipStream_t s = ctx_stream(c);
+    TrainBufs b;
+    DsCounts dc;
+    const int64_t nslots = (int64_t)m.collect_stripes * m.collect_scap;
+    int st = build_set(c, m.collect, m.collect_n, m.collect_scap, nslots, ctx_gamma(c),
+                       keep_trivial, batch, b, dc);
+    // the record restarts whether or not the set was large enough
+    CHK(hipMemsetAsync(m.collect_n, 0, sizeof(unsigned long long) * kCollectStripes, s));
+    CHK(hipStreamSynchronize(s));
+    if (st) return st;
+    return train_on_set(c, b, dc, n_epochs, batch, lr, seed, info);
+}
+
+int mlh_mlp_dataset(mlh_ctx *c, int64_t n, const double *states6, double gamma,
+                    double keep_trivial, int64_t cap, float *feats, float *target,
+                    int64_t *rows, int64_t *n_out) {
+    if (!c || n < 0 || n > INT32_MAX || (n > 0 && !states6) || cap < 0 || !n_out ||
+        (cap > 0 && (!feats || !target)) || !(keep_trivial >= 0.))
+        return ctx_fail(MLH_ERR_INVALID_ARGUMENT, "bad arguments");
+    *n_out = 0;
+    MlpState &m = ctx_mlp(c);
+    hipStream_t s = ctx_stream(c);
+    DevBuf<double> F6;
+    const double *src = nullptr;
+    const unsigned long long *counts = nullptr;
+    int64_t scap = n, nslots = n;
+    if (states6) {
+        CHK(dmalloc(&F6.p, sizeof(double) * 6 * n));
+        CHK(hipMemcpyAsync(F6.p, states6, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s));
+        src = F6.p;
+    } else {
+        if (!m.collect || !m.collect_n || m.collect_stripes <= 0)
+            return ctx_fail(MLH_ERR_CALL_ORDER, "no face record (mlh_set_face_collection)");
+        src = m.collect;
+        counts = m.collect_n;
+        scap = m.collect_scap;
+        nslots = (int64_t)m.collect_stripes * scap;
+    }
+    if (nslots == 0) return MLH_OK;
+    TrainBufs b;
+    DsCounts dc;
+    std::vector<uint32_t> slots;
+    int st = build_set(c, src, counts, scap, nslots, gamma, keep_trivial, 1, b, dc, &slots);
+    if (st) return st;
+    const int64_t k = dc.nsel < cap ? dc.nsel : cap;
+    if (k > 0) {
+        CHK(hipMemcpyAsync(feats, b.X, sizeof(float) * kMlpIn * k, hipMemcpyDeviceToHost, s));
+        CHK(hipMemcpyAsync(target, b.T, sizeof(float) * k, hipMemcpyDeviceToHost, s));
+    }
+    CHK(hipStreamSynchronize(s));
+    if (rows && k > 0) {
+        // slot -> input row; the record's rows in mlh_get_collected_faces'
+        // order (each stripe's filled prefix, stripe after stripe)
+        std::vector<int64_t> first(1, 0);
+        if (counts) {
+            const int ns = (int)(nslots / scap);
+            std::vector<unsigned long long> cn(ns);
+            CHK(hipMemcpy(cn.data(), counts, sizeof(unsigned long long) * ns,
+                          hipMemcpyDeviceToHost));
+            first.resize(ns);
+            int64_t o = 0;
+            for (int q = 0; q < ns; ++q) {
+                first[q] = o;
+                o += cn[q] < (unsigned long long)scap ? (int64_t)cn[q] : scap;
+            }
+        }
+        for (int64_t q = 0; q < k; ++q) {
+            const int64_t sl = slots[q];
+            rows[q] = counts ? first[sl / scap] + sl % scap : sl;
+        }
+    }
+    *n_out = dc.nsel;
     return MLH_OK;
 }
 

@@ -44,4 +44,5 @@ void mlp_free(MlpState &m);
 struct mlh_ctx;
 mlh::MlpState &ctx_mlp(mlh_ctx *c);
 hipStream_t ctx_stream(mlh_ctx *c);
+double ctx_gamma(mlh_ctx *c);  // the context's adiabatic index
 int ctx_fail(int code, const char *msg);

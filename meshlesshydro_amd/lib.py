@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from ._abi import Diag, Params, MLP_IN, MLP_NPARAM, MLH_FP64, MLH_SEED_STANDARD
+from ._abi import Diag, Params, TrainInfo, MLP_IN, MLP_NPARAM, MLH_FP64, MLH_SEED_STANDARD
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # MLH_LIBRARY selects another build of the same library (e.g. a variant
@@ -71,6 +71,12 @@ SIGNATURES = {
     "mlh_mlp_loss_grad": (C.c_int, [_vp, C.c_int64, _fp, _fp, _fp, _fp]),
     "mlh_mlp_train": (C.c_int, [_vp, C.c_int64, _fp, _fp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]),
     "mlh_mlp_infer": (C.c_int, [_vp, C.c_int64, _fp, _fp, C.c_int32]),
+    "mlh_mlp_train_faces": (C.c_int, [_vp, C.c_int64, _dp, C.c_double, C.c_int32, C.c_int32, C.c_float,
+                                      C.c_uint64, C.c_double, C.POINTER(TrainInfo)]),
+    "mlh_mlp_train_recorded": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, C.c_double,
+                                         C.POINTER(TrainInfo)]),
+    "mlh_mlp_dataset": (C.c_int, [_vp, C.c_int64, _dp, C.c_double, C.c_double, C.c_int64, _fp, _fp,
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mlh_riemann_pstar_batch": (C.c_int, [C.c_int64, C.c_double] + [_dp] * 6 + [C.c_double, _dp, _ip]),
     "mlh_set_face_collection": (C.c_int, [_vp, C.c_int64, C.c_int32]),
     "mlh_get_collected_faces": (C.c_int, [_vp, _dp, C.POINTER(C.c_int64)]),
@@ -302,6 +308,45 @@ class Simulation:
         out = np.zeros(len(feats), np.float32)
         _chk(self.L.mlh_mlp_infer(self.h, len(feats), _f(feats), _f(out), int(bf16)), "mlp_infer")
         return out
+
+    def mlp_train_faces(self, faces, gamma, epochs=4, batch=4096, lr=2e-3, seed=1, keep_trivial=1.0):
+        """Build the training set of faces [n, 6] on the device and train
+        the network on it (mlh_mlp_train_faces); returns the info dict."""
+        faces = np.ascontiguousarray(faces, np.float64).reshape(-1, 6)
+        info = TrainInfo()
+        _chk(self.L.mlh_mlp_train_faces(self.h, len(faces), _d(faces), gamma, epochs, batch, lr, seed,
+                                        keep_trivial, C.byref(info)), "mlp_train_faces")
+        return info.as_dict()
+
+    def mlp_train_recorded(self, epochs=2, batch=4096, lr=2e-3, seed=1, keep_trivial=1.0):
+        """Train on the faces this context recorded and restart the record
+        (mlh_mlp_train_recorded); returns the info dict."""
+        info = TrainInfo()
+        _chk(self.L.mlh_mlp_train_recorded(self.h, epochs, batch, lr, seed, keep_trivial, C.byref(info)),
+             "mlp_train_recorded")
+        return info.as_dict()
+
+    def mlp_dataset(self, faces=None, gamma=None, keep_trivial=1.0, rows=False):
+        """The device-built training set (features [n, 10], targets [n]) of
+        faces [n, 6], or of the record when faces is None; with rows=True
+        also each kept face's input row."""
+        gamma = self.p.gamma if gamma is None else gamma
+        n = C.c_int64()
+        if faces is None:
+            cap = max(getattr(self, "_collect_cap", 0), 0)
+            src, nf = None, 0
+        else:
+            src = np.ascontiguousarray(faces, np.float64).reshape(-1, 6)
+            cap, nf = len(src), len(src)
+        feats = np.zeros((max(cap, 1), MLP_IN), np.float32)
+        target = np.zeros(max(cap, 1), np.float32)
+        r = np.zeros(max(cap, 1), np.int64)
+        _chk(self.L.mlh_mlp_dataset(self.h, nf, None if src is None else _d(src), gamma, keep_trivial, cap,
+                                    _f(feats), _f(target), r.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    C.byref(n)), "mlp_dataset")
+        k = n.value
+        out = (feats[:k].copy(), target[:k].copy())
+        return out + (r[:k].copy(),) if rows else out
 
     def set_face_collection(self, cap, particle_stride):
         self._collect_cap = int(cap)

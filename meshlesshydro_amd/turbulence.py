@@ -163,11 +163,8 @@ def run_online(sim, steps, retrain_every=50, stride=8, epochs=2, dts=None, log=N
         done += k
         if done < steps:
             t0 = time.perf_counter()
-            faces = sim.collected_faces()
-            sim.set_face_collection(1 << 21, stride)  # restart the record
             try:
-                _, info = L.train_seed_from_faces(sim, faces, sim.p.gamma, n_epochs=epochs,
-                                                  init=sim.mlp_get_params(), log=log)
+                _, info = L.train_seed_from_record(sim, n_epochs=epochs, log=log)
                 info["step"] = done
             except ValueError as e:  # too few recorded faces: keep the weights
                 info = {"step": done, "skipped": str(e)}

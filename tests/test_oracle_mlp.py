@@ -41,3 +41,13 @@ def test_features_encode_the_guess(oracle_mod):
         if pg > 1e-5 * pref[k]:
             assert abs(f[k, 7] - np.log(pg / pref[k])) < 1e-5
     assert np.all(np.abs(f[:, 8] - (g - 1) / (g + 1)) < 1e-7)
+
+
+def test_training_set_hash_is_splitmix64():
+    """The training set's face hash mixer is splitmix64's output function:
+    the generator's published first outputs from seed 0 (state advanced by
+    0x9E3779B97F4A7C15 before each output)."""
+    from meshlesshydro_amd.learned import mix64
+    state = np.array([0, 0x9E3779B97F4A7C15, (2 * 0x9E3779B97F4A7C15) % (1 << 64)], np.uint64)
+    out = mix64(state)
+    assert [int(v) for v in out] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
