@@ -66,152 +66,220 @@ __global__ void k_mlp_pack(const float *__restrict__ P, MlpFrags *__restrict__ F
 }
 
 // ---- training --------------------------------------------------------------
-// One block of four wavefronts per chunk of 64 samples of the batch: lane l
-// of every wavefront is sample l of the chunk, and wavefront w computes the
-// quarter [16w, 16w + 16) of each hidden layer's outputs (forward) and of
-// dz1 (backward), with the activations shared through LDS.  The chunk's
-// partial gradient follows with thread p forming columns [16(p%4), +16) of
-// row p/4 of dW2 as sums over the chunk's samples in sample order.
-// k_mlp_reduce sums the chunks' partials in chunk order and applies Adam or
-// stores the gradient: the result does not depend on scheduling and needs
-// no atomics.
-//   dy = 2 e / B, dz2 = dy w3 (1 - h2^2), dz1 = (W2^T dz2)(1 - h1^2)
-//   dW3 = sum dy h2, dW2 = sum dz2 h1^T, dW1 = sum dz1 x^T, biases the sums
+// One block of four wavefronts per chunk of 64 samples of the batch.  The
+// GEMM-shaped products run on the matrix cores in FP32
+// (v_mfma_f32_16x16x4f32; lane l, g = l / 16, j = l % 16: A holds row j,
+// k slot g; B holds column j, k slot g; D holds column j, rows 4g..4g+3).
+// K is a label inside one MFMA, so step ks of lane group g takes the k
+// index 16g + ks (4g + ks for the 16-wide input layer): a lane's A
+// operands of one tile are then 16 consecutive floats of an LDS row.
+// Wavefront w owns 16 neurons of each product:
+//   forward    Z1 = X W1^T + b1, Z2 = H1 W2^T + b2    (samples x neurons)
+//   backward   dW2 = dz2^T H1, dz1 = (dz2 W2)(1 - h1^2), [dW1 | db1] =
+//              dz1^T [X | 1]                          (the ones column)
+// and the small vector terms (the output layer, dy = 2 e / B, dz2 = dy w3
+// (1 - h2^2), dW3, db2, db3) run on the VALU.  Each chunk stores its
+// partial gradient; k_mlp_reduce sums the chunks' partials in chunk order
+// and applies Adam: the result does not depend on scheduling and needs no
+// atomics.
 constexpr int kTrainChunk = 64;
-constexpr int kTrainLd = kMlpH + 1;   // LDS row pitch (no bank conflicts)
-constexpr int kTrainQ = kMlpH / 4;    // outputs per wavefront
+constexpr int kTrainLd = kMlpH + 4;   // LDS row pitch: rows 16-byte aligned
+constexpr int kTrainXLd = 16;         // input tile: 10 features, 1, zeros
 constexpr int kPartialN = kMlpN + 1;  // gradient + the chunk's sum of e^2
+
+__device__ inline f32x4_t mfma4(float a, float b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
 
 __global__ __launch_bounds__(256) void k_mlp_chunk_grad(const float *__restrict__ P,
                                                         const float *__restrict__ X,
                                                         const float *__restrict__ T,
                                                         const int *__restrict__ idx, int B,
                                                         float *__restrict__ partial) {
-    __shared__ float H1[kTrainChunk][kTrainLd];
-    __shared__ float H2[kTrainChunk][kTrainLd];  // h2, then dz1
-    __shared__ float DZ2[kTrainChunk][kTrainLd];
-    __shared__ float XS[kTrainChunk][kMlpIn];
+    __shared__ __align__(16) float XS[kTrainChunk][kTrainXLd];
+    __shared__ __align__(16) float H1[kTrainChunk][kTrainLd];
+    __shared__ __align__(16) float H2[kTrainChunk][kTrainLd];  // h2, then dz1
+    __shared__ __align__(16) float DZ2[kTrainChunk][kTrainLd];
     __shared__ float YP[4][kTrainChunk];
     __shared__ float DY[kTrainChunk], E2[kTrainChunk];
     const int tid = threadIdx.x, l = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wavefront-uniform
-    const int o0 = kTrainQ * w;
-    const int b = blockIdx.x * kTrainChunk + l;
-    const bool act = b < B;
-    const int s = act ? (idx ? idx[b] : b) : 0;
-    // register arrays are indexed with compile-time indices only (unrolled
-    // loops), so they stay in registers
-    float x[kMlpIn];
+    const int g = l >> 4, j = l & 15;
+    const int c0 = blockIdx.x * kTrainChunk;
+    // the input tile: features, a ones column (db1), zeros; samples past
+    // the batch end are zero rows and contribute nothing (dy = 0)
+    for (int e = tid; e < kTrainChunk * kTrainXLd; e += 256) {
+        const int q = e >> 4, k = e & 15, b = c0 + q;
+        float v = 0.f;
+        if (b < B) {
+            const int s = idx ? idx[b] : b;
+            v = k < kMlpIn ? X[(size_t)s * kMlpIn + k] : (k == kMlpIn ? 1.f : 0.f);
+        }
+        XS[q][k] = v;
+    }
+    // W2 operands (registers): rows 16w + j for the forward product, column
+    // 16w + j for dz1, each at k = 16g + ks
+    float w2r[16], w2c[16];
 #pragma unroll
-    for (int k = 0; k < kMlpIn; ++k) x[k] = act ? X[(size_t)s * kMlpIn + k] : 0.f;
-    if (w == 0)
-        for (int k = 0; k < kMlpIn; ++k) XS[l][k] = x[k];
-    for (int oo = 0; oo < kTrainQ; ++oo) {
-        const int o = o0 + oo;
-        float a = P[kMlpB1 + o];
-#pragma unroll
-        for (int k = 0; k < kMlpIn; ++k) a += P[kMlpW1 + o * kMlpIn + k] * x[k];
-        H1[l][o] = tanhf(a);
+    for (int ks = 0; ks < 16; ++ks) {
+        w2r[ks] = P[kMlpW2 + (16 * w + j) * kMlpH + 16 * g + ks];
+        w2c[ks] = P[kMlpW2 + (16 * g + ks) * kMlpH + 16 * w + j];
     }
     __syncthreads();
-    float yp = 0.f;
-    for (int oo = 0; oo < kTrainQ; ++oo) {
-        const int o = o0 + oo;
-        float a = P[kMlpB2 + o];
-        for (int k = 0; k < kMlpH; ++k) a += P[kMlpW2 + o * kMlpH + k] * H1[l][k];
-        const float h2 = tanhf(a);
-        H2[l][o] = h2;
-        yp += P[kMlpW3 + o] * h2;
-    }
-    YP[w][l] = yp;
-    __syncthreads();
-    // samples past the batch end contribute nothing (dy = 0)
-    const float y = P[kMlpB3] + ((YP[0][l] + YP[1][l]) + (YP[2][l] + YP[3][l]));
-    const float e = act ? y - T[s] : 0.f;
-    const float dy = 2.f * e / (float)B;
-    if (w == 0) {
-        DY[l] = dy;
-        E2[l] = e * e;
-    }
-    for (int oo = 0; oo < kTrainQ; ++oo) {
-        const int o = o0 + oo;
-        const float h2 = H2[l][o];
-        DZ2[l][o] = dy * P[kMlpW3 + o] * (1.f - h2 * h2);
-    }
-    __syncthreads();
-    float *g = partial + (size_t)blockIdx.x * kPartialN;
+    // layer 1: Z1[samples][16w..16w+15], K = 16 (k = 4g + ks)
     {
-        // thread (o, kq): columns [16 kq, 16 kq + 16) of row o of dW2; the
-        // kq = 0 threads also dW3[o] and db2[o]; thread 0 db3 and sum e^2
-        const int o = tid >> 2, kq = tid & 3;
-        float acc[kTrainQ];
+        const f32x4_t bias = {P[kMlpB1 + 16 * w + j], P[kMlpB1 + 16 * w + j],
+                              P[kMlpB1 + 16 * w + j], P[kMlpB1 + 16 * w + j]};
+        float w1[4];
 #pragma unroll
-        for (int k = 0; k < kTrainQ; ++k) acc[k] = 0.f;
+        for (int ks = 0; ks < 4; ++ks) {
+            const int k = 4 * g + ks;
+            w1[ks] = k < kMlpIn ? P[kMlpW1 + (16 * w + j) * kMlpIn + k] : 0.f;
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            f32x4_t acc = bias;
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) acc = mfma4(XS[16 * mt + j][4 * g + ks], w1[ks], acc);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) H1[16 * mt + 4 * g + r][16 * w + j] = tanhf(acc[r]);
+        }
+    }
+    __syncthreads();
+    // layer 2: Z2[samples][16w..16w+15], K = 64
+    {
+        const float b2 = P[kMlpB2 + 16 * w + j];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const f32x4_t *row = (const f32x4_t *)&H1[16 * mt + j][16 * g];
+            f32x4_t a4[4] = {row[0], row[1], row[2], row[3]};
+            f32x4_t acc = {b2, b2, b2, b2};
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks) acc = mfma4(a4[ks >> 2][ks & 3], w2r[ks], acc);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) H2[16 * mt + 4 * g + r][16 * w + j] = tanhf(acc[r]);
+        }
+    }
+    __syncthreads();
+    // output layer, loss term, dz2 (thread: sample l, neurons 16w..16w+15)
+    const float *w3 = P + kMlpW3 + 16 * w;
+    {
+        float yp = 0.f;
+        for (int oo = 0; oo < 16; ++oo) yp += w3[oo] * H2[l][16 * w + oo];
+        YP[w][l] = yp;
+    }
+    __syncthreads();
+    {
+        const int b = c0 + l;
+        const bool act = b < B;
+        const float y = P[kMlpB3] + ((YP[0][l] + YP[1][l]) + (YP[2][l] + YP[3][l]));
+        const float e = act ? y - T[idx ? idx[b] : b] : 0.f;
+        const float dy = 2.f * e / (float)B;
+        if (w == 0) {
+            DY[l] = dy;
+            E2[l] = e * e;
+        }
+        for (int oo = 0; oo < 16; ++oo) {
+            const float h2 = H2[l][16 * w + oo];
+            DZ2[l][16 * w + oo] = dy * w3[oo] * (1.f - h2 * h2);
+        }
+    }
+    __syncthreads();
+    float *gr = partial + (size_t)blockIdx.x * kPartialN;
+    // dW2[16w..16w+15][0..63] = dz2^T H1, K = samples (s = 16g + ks)
+    {
+        f32x4_t acc[4] = {};
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks) {
+            const int q = 16 * g + ks;
+            const float a = DZ2[q][16 * w + j];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[nt] = mfma4(a, H1[q][16 * nt + j], acc[nt]);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                gr[kMlpW2 + (16 * w + 4 * g + r) * kMlpH + 16 * nt + j] = acc[nt][r];
+    }
+    // dz1 before the h1 factor: (dz2 W2)[samples][16w..16w+15], K = 64
+    f32x4_t dz1[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const f32x4_t *row = (const f32x4_t *)&DZ2[16 * mt + j][16 * g];
+        f32x4_t a4[4] = {row[0], row[1], row[2], row[3]};
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks) acc = mfma4(a4[ks >> 2][ks & 3], w2c[ks], acc);
+        dz1[mt] = acc;
+    }
+    // dW3, db2 (threads 0-63, sample order), db3 and the sum of e^2
+    if (tid < kMlpH) {
+        float s3 = 0.f, bb = 0.f;
         for (int q = 0; q < kTrainChunk; ++q) {
-            const float d = DZ2[q][o];
-#pragma unroll
-            for (int k = 0; k < kTrainQ; ++k) acc[k] += d * H1[q][kTrainQ * kq + k];
+            s3 += DY[q] * H2[q][tid];
+            bb += DZ2[q][tid];
         }
-#pragma unroll
-        for (int k = 0; k < kTrainQ; ++k) g[kMlpW2 + o * kMlpH + kTrainQ * kq + k] = acc[k];
-        if (kq == 0) {
-            float w3 = 0.f, bb = 0.f;
-            for (int q = 0; q < kTrainChunk; ++q) {
-                w3 += DY[q] * H2[q][o];
-                bb += DZ2[q][o];
-            }
-            g[kMlpW3 + o] = w3;
-            g[kMlpB2 + o] = bb;
+        gr[kMlpW3 + tid] = s3;
+        gr[kMlpB2 + tid] = bb;
+    } else if (tid == kMlpH) {
+        float b3 = 0.f, se = 0.f;
+        for (int q = 0; q < kTrainChunk; ++q) {
+            b3 += DY[q];
+            se += E2[q];
         }
-        if (tid == 0) {
-            float b3 = 0.f, se = 0.f;
-            for (int q = 0; q < kTrainChunk; ++q) {
-                b3 += DY[q];
-                se += E2[q];
-            }
-            g[kMlpB3] = b3;
-            g[kMlpN] = se;
-        }
+        gr[kMlpB3] = b3;
+        gr[kMlpN] = se;
     }
     __syncthreads();  // H2 is read above, dz1 goes there below
-    for (int kk = 0; kk < kTrainQ; ++kk) {
-        const int k = o0 + kk;
-        float a = 0.f;
-        for (int o = 0; o < kMlpH; ++o) a += P[kMlpW2 + o * kMlpH + k] * DZ2[l][o];
-        const float hk = H1[l][k];
-        H2[l][k] = a * (1.f - hk * hk);
-    }
-    __syncthreads();
-    if (tid < kMlpH) {
-        const int o = tid;
-        float bb = 0.f, acc[kMlpIn];
 #pragma unroll
-        for (int k = 0; k < kMlpIn; ++k) acc[k] = 0.f;
-        for (int q = 0; q < kTrainChunk; ++q) {
-            const float d = H2[q][o];
-            bb += d;
+    for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int k = 0; k < kMlpIn; ++k) acc[k] += d * XS[q][k];
+        for (int r = 0; r < 4; ++r) {
+            const float h1 = H1[16 * mt + 4 * g + r][16 * w + j];
+            H2[16 * mt + 4 * g + r][16 * w + j] = dz1[mt][r] * (1.f - h1 * h1);
         }
-        g[kMlpB1 + o] = bb;
+    __syncthreads();
+    // [dW1 | db1][16w..16w+15][0..15] = dz1^T [X | 1], K = samples
+    {
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int k = 0; k < kMlpIn; ++k) g[kMlpW1 + o * kMlpIn + k] = acc[k];
+        for (int ks = 0; ks < 16; ++ks) {
+            const int q = 16 * g + ks;
+            acc = mfma4(H2[q][16 * w + j], XS[q][j], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int o = 16 * w + 4 * g + r;
+            if (j < kMlpIn) gr[kMlpW1 + o * kMlpIn + j] = acc[r];
+            else if (j == kMlpIn) gr[kMlpB1 + o] = acc[r];
+        }
     }
 }
 
-// Sum of the chunks' partials in chunk order; thread p < kMlpN: gradient
-// entry p (stored in G, and with adam the Adam step of parameter p);
-// thread kMlpN: the loss, mean of e^2 over the batch.
+// Sum of the chunks' partials, eight lanes per entry: lane q of entry p
+// sums chunks q, q + 8, q + 16, ... in order, and the eight sums combine
+// in a fixed butterfly (every lane gets the same bits: IEEE addition
+// commutes).  Entry p < kMlpN: gradient entry p (stored in G, and with
+// adam the Adam step of parameter p); entry kMlpN: the loss, mean of e^2
+// over the batch.
+constexpr int kReduceLanes = 8;
+constexpr int kReducePerBlock = 256 / kReduceLanes;
 __global__ __launch_bounds__(256) void k_mlp_reduce(const float *__restrict__ partial,
                                                     int nchunks, int B, float *__restrict__ G,
                                                     float *__restrict__ loss, int adam,
                                                     float *__restrict__ P, float *__restrict__ M,
                                                     float *__restrict__ V, float lr, float b1c,
                                                     float b2c) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > kMlpN) return;
+    const int p = blockIdx.x * kReducePerBlock + (threadIdx.x >> 3);
+    const int q = threadIdx.x & (kReduceLanes - 1);
     float g = 0.f;
-    for (int c = 0; c < nchunks; ++c) g += partial[(size_t)c * kPartialN + p];
+    if (p <= kMlpN)
+        for (int c = q; c < nchunks; c += kReduceLanes) g += partial[(size_t)c * kPartialN + p];
+    g += __shfl_xor(g, 1, 64);
+    g += __shfl_xor(g, 2, 64);
+    g += __shfl_xor(g, 4, 64);
+    if (p > kMlpN || q != 0) return;
     if (p == kMlpN) {
         *loss = g / (float)B;
         return;
@@ -374,7 +442,7 @@ constexpr int kMinTrainFaces = 256;
 
 // face slot k of the input: counts null -> every slot [0, nslots) holds a
 // face; else slot k is row r = k % scap of stripe s = k / scap, filled when
-// r < counts[s]
+// r < counts[s].  Launched with 256 threads per block.
 __global__ __launch_bounds__(256) void k_ds_prep(const double *__restrict__ F6,
                                                   const unsigned long long *__restrict__ counts,
                                                   int64_t scap, int64_t nslots, double gamma,
@@ -420,17 +488,23 @@ __global__ __launch_bounds__(256) void k_ds_prep(const double *__restrict__ F6,
         hlo[k] = (uint32_t)h;
         hhi[k] = (uint32_t)(h >> 32);
     }
-    // class counts: one atomic per wavefront and class
+    // the block's class counts: ncls[block] trivial, ncls[nblocks + block]
+    // non-trivial (every block stores both)
+    __shared__ unsigned long long wc[2][4];
     const unsigned long long n1 = __popcll(__ballot(cl == 1)), n2 = __popcll(__ballot(cl == 2));
     if ((threadIdx.x & 63) == 0) {
-        if (n1) atomicAdd(&ncls[0], n1);
-        if (n2) atomicAdd(&ncls[1], n2);
+        wc[0][threadIdx.x >> 6] = n1;
+        wc[1][threadIdx.x >> 6] = n2;
     }
+    __syncthreads();
+    if (threadIdx.x < 2)
+        ncls[threadIdx.x * gridDim.x + blockIdx.x] =
+            wc[threadIdx.x][0] + wc[threadIdx.x][1] + wc[threadIdx.x][2] + wc[threadIdx.x][3];
 }
 
 // selection (trivial faces kept when the top 53 bits of a second hash are
 // below thr53) and the high sort key: (h >> 33), with bit 31 set for the
-// faces that are not kept
+// faces that are not kept.  Launched with 256 threads per block.
 __global__ __launch_bounds__(256) void k_ds_select(const uint8_t *__restrict__ cls,
                                                     const uint32_t *__restrict__ hlo,
                                                     const uint32_t *__restrict__ hhi,
@@ -445,8 +519,12 @@ __global__ __launch_bounds__(256) void k_ds_select(const uint8_t *__restrict__ c
         sel = cl == 2 || (cl == 1 && (mix64(h ^ kSelectSalt) >> 11) < thr53);
         khi[k] = (hhi[k] >> 1) | (sel ? 0u : 0x80000000u);
     }
+    // the block's count of kept faces in nsel[block]
+    __shared__ unsigned long long wc[4];
     const unsigned long long n = __popcll(__ballot(sel));
-    if ((threadIdx.x & 63) == 0 && n) atomicAdd(nsel, n);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) nsel[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
 }
 
 __global__ __launch_bounds__(256) void k_iota_u32(uint32_t *__restrict__ v, int64_t n) {
@@ -607,7 +685,8 @@ static void enqueue_grad(mlh_ctx *c, TrainBufs &b, const int *idx, int B, bool a
     const int nch = train_chunks(B);
     hipLaunchKernelGGL(k_mlp_chunk_grad, dim3(nch), dim3(256), 0, s, m.params, b.X, b.T, idx, B,
                        b.partial);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartialN + 255) / 256), dim3(256), 0, s, b.partial,
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartialN + kReducePerBlock - 1) / kReducePerBlock),
+                       dim3(256), 0, s, b.partial,
                        nch, B, m.grad, b.loss, adam ? 1 : 0, m.params, m.adam_m, m.adam_v, lr,
                        b1c, b2c);
 }
@@ -658,18 +737,21 @@ static int build_set(mlh_ctx *c, const double *F6, const unsigned long long *cou
     DevBuf<float> Xall, Tall;
     DevBuf<uint32_t> hlo, hhi, khi, ord, k1, v1, scr;
     DevBuf<uint8_t> cls;
-    DevBuf<unsigned long long> cnt;  // trivial, non-trivial, kept, faces
+    DevBuf<unsigned long long> cnt;  // trivial, non-trivial, kept
+    DevBuf<unsigned long long> blk;  // per-block counts
     CHK(dmalloc(&Xall.p, sizeof(float) * kMlpIn * nslots));
     CHK(dmalloc(&Tall.p, sizeof(float) * nslots));
     uint32_t **u[] = {&hlo.p, &hhi.p, &khi.p, &ord.p, &k1.p, &v1.p};
     for (uint32_t **q : u) CHK(dmalloc(q, sizeof(uint32_t) * nslots));
     CHK(dmalloc(&scr.p, sizeof(uint32_t) * radix_sort_scratch_words((int)nslots)));
     CHK(dmalloc(&cls.p, nslots));
-    CHK(dmalloc(&cnt.p, sizeof(unsigned long long) * 3));
-    CHK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 3, s));
     const int64_t nb = (nslots + 255) / 256;
+    CHK(dmalloc(&cnt.p, sizeof(unsigned long long) * 3));
+    CHK(dmalloc(&blk.p, sizeof(unsigned long long) * 2 * nb));
     hipLaunchKernelGGL(k_ds_prep, dim3(nb), dim3(256), 0, s, F6, counts, scap, nslots, gamma,
-                       Xall.p, Tall.p, hlo.p, hhi.p, cls.p, cnt.p);
+                       Xall.p, Tall.p, hlo.p, hhi.p, cls.p, blk.p);
+    hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(256), 0, s, blk.p, (int)nb, cnt.p);
+    hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(256), 0, s, blk.p + nb, (int)nb, cnt.p + 1);
     unsigned long long h[3] = {};
     CHK(hipMemcpyAsync(h, cnt.p, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, s));
     CHK(hipStreamSynchronize(s));
@@ -695,7 +777,8 @@ static int build_set(mlh_ctx *c, const double *F6, const unsigned long long *cou
         thr53 = keep >= dc.trivial ? (1ull << 53)
                                    : (uint64_t)((double)keep / (double)dc.trivial * 9007199254740992.0);
     hipLaunchKernelGGL(k_ds_select, dim3(nb), dim3(256), 0, s, cls.p, hlo.p, hhi.p, nslots, thr53,
-                       khi.p, cnt.p + 2);
+                       khi.p, blk.p);
+    hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(256), 0, s, blk.p, (int)nb, cnt.p + 2);
     // sort by (khi, hlo): the low word first, then stably by the high one
     hipLaunchKernelGGL(k_iota_u32, dim3(nb), dim3(256), 0, s, ord.p, nslots);
     radix_sort_pairs(hlo.p, ord.p, k1.p, v1.p, (int)nslots, 32, scr.p, s);
