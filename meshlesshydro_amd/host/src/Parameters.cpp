@@ -44,7 +44,11 @@ Parameters default_parameters() {
 
 void apply_define(Parameters &p, const std::string &n, const std::string &v) {
     mlh_params &q = p.params;
-    if (n == "DIM") q.dim = to_int(n, v);
+    if (n == "DIM") {
+        q.dim = to_int(n, v);
+        if (q.dim != 2 && q.dim != 3)
+            throw std::runtime_error("parameter.h: DIM must be 2 or 3, not '" + v + "'");
+    }
     else if (n == "PERIODIC_BOUNDARIES") q.periodic = to_int(n, v);
     else if (n == "ADAPTIVE_TIMESTEP") p.adaptive_timestep = to_int(n, v) != 0;
     else if (n == "CFL") q.cfl = to_double(n, v);

@@ -532,12 +532,15 @@ int orc_get_neighbours(orc *o, int32_t *noi, int32_t *noi_ghost,
         noi[i] = o->noi[i];
         noi_ghost[i] = o->p.periodic ? o->noig[i] : 0;
         int32_t *row = &nbr[(size_t)i * stride];
+        /* packed as the C ABI's entries: index | image code << 27, formed
+         * in uint32 (codes up to 26 exceed INT32_MAX once shifted) */
         for (int n = 0; n < o->noi[i]; ++n)
-            row[n] = o->nnl[(size_t)i * o->maxr + n] | (13 << 27);
+            row[n] = (int32_t)((uint32_t)o->nnl[(size_t)i * o->maxr + n] | (13u << 27));
         if (o->p.periodic)
             for (int n = 0; n < o->noig[i]; ++n) {
                 int g = o->nnlg[(size_t)i * o->maxg + n];
-                row[o->noi[i] + n] = o->gpar[g] | (o->gcode[g] << 27);
+                row[o->noi[i] + n] =
+                    (int32_t)((uint32_t)o->gpar[g] | ((uint32_t)o->gcode[g] << 27));
             }
     }
     return 0;
