@@ -188,7 +188,6 @@ static HydroArgs hydro_args(mlh_ctx *c) {
     const mlh_params &p = c->p;
     a.g = c->g;
     a.N = c->N;
-    a.stride = c->N;
     a.gamma = p.gamma; a.cfl = p.cfl; a.beta = p.beta; a.psi1 = p.psi1; a.psi2 = p.psi2;
     a.riemann_tol = p.riemann_tol;
     a.riemann_max_iter = p.riemann_max_iter;
@@ -289,7 +288,7 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     }
     // four guard rows past the list: the search stores up to four hits past
     // MAX_NUM_INTERACTIONS before it detects the overflow
-    A_(&c->nbr, 4 * N * ((size_t)c->stride + 4));
+    A_(&c->nbr, 4 * nb_words((int)N, c->stride + 4));
     A_(&c->noi, 4 * N); A_(&c->nog, 4 * N);
     A_(&c->prim, c->tsz * kPrimN * N);
     A_(&c->frec, c->tsz * frn * N);
@@ -850,7 +849,7 @@ int mlh_get_neighbours(mlh_ctx *c, int32_t *noi, int32_t *noi_ghost, int32_t *nb
     std::vector<int32_t> cn(N), cg(N);
     HIPCHK(hipMemcpy(cn.data(), c->noi, 4 * N, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(cg.data(), c->nog, 4 * N, hipMemcpyDeviceToHost));
-    std::vector<uint32_t> t((size_t)c->stride * N);
+    std::vector<uint32_t> t(nb_words((int)N, c->stride));
     HIPCHK(hipMemcpy(t.data(), c->nbr, 4 * t.size(), hipMemcpyDeviceToHost));
     for (size_t q = 0; q < N; ++q) {
         size_t i = perm[q];
@@ -858,7 +857,7 @@ int mlh_get_neighbours(mlh_ctx *c, int32_t *noi, int32_t *noi_ghost, int32_t *nb
         noi_ghost[i] = cg[q];
         int cnt = cn[q] + cg[q];
         for (int n = 0; n < c->stride; ++n) {
-            uint32_t e = t[(size_t)n * N + q];
+            uint32_t e = t[nb_slot((int)N, (int)q, n)];
             nbr[i * c->stride + n] =
                 n < cnt ? (int32_t)(perm[nb_index(e)] | (nb_code(e) << kIdxBits)) : 0;
         }

@@ -32,6 +32,7 @@
 // cell order is ascending original index (stable sort of the identity), so
 // the reference's neighbour order is kept.
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <type_traits>
 #include "device_common.h"
 #include "kernels.h"
@@ -288,171 +289,191 @@ __device__ inline TW wdist(double d_search, const double *a, const double *b) {
     else return wsqrt(TW(d_search));
 }
 
-// kDens: 0, or the bytes of the primitive records (4, 8) when the step's
-// density is fused into the search (NbrArgs::dens)
-template <int D, bool kCaller, int kDens>
-__global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
+// Candidate sources of the search.  range() gives the cell-order range
+// [q0, q1) of cell c and a delta such that load(q + delta) is the position
+// of candidate q; the loads of a group of four are issued together.
+struct GlobalCand {
+    const char *pcb;  // pos_cell as bytes
+    const int *cstart, *cend;
+    __device__ inline void range(int, int, int, uint32_t c, int &q0, int &q1, int &delta) const {
+        q0 = cstart[c];
+        q1 = cend[c];
+        delta = 0;
+    }
+    // 32-bit byte offset from the array base (N < 2^27)
+    __device__ inline double4 load(int q) const {
+        return *(const double4 *)(pcb + ((uint32_t)q << 5));
+    }
+};
+
+// The search of work position t: real neighbours in the reference's stencil
+// order (i itself excluded: in cell order its candidate position is t), the
+// periodic images in ghost order, and with kDens the step's density.
+//  * A stencil cell whose box is farther than h from x_i (with a relative
+//    margin far above the rounding of positions and cell indices; edge
+//    cells are open outward, where clamped particles sit) holds no
+//    neighbour and is skipped.
+//  * Candidates are read four at a time (indices clamped to the cell, no
+//    branches around the loads); a hit is stored through a running pointer
+//    into the chunked row, so a position's eight consecutive entries fill
+//    one 32-byte sector.
+//  * The capacity is checked per group of four: the list has four guard
+//    entries past its last one for the overshoot.
+// Returns the real and image counts through cnt / gcnt.
+template <int D, bool kCaller, int kDens, typename Src>
+__device__ inline void search_one(const NbrArgs &a, int t, const Src &src, int &cnt_out,
+                                  int &gcnt_out) {
     using TW = typename std::conditional<kDens == 4, float, double>::type;
-    const int t = xcd_block() * blockDim.x + threadIdx.x;
-    const bool live = t < a.N;
     const Geom &g = a.g;
     const int N = a.N;
     int cnt = 0, gcnt = 0;
     const TW wh2 = TW(g.h / 2.), wsig = TW(g.sigma);
     TW omg = TW(0);
-    if (live) {
-        double4 pi = a.pos_w[t];
-        double xi[3] = {pi.x, pi.y, pi.z};
-        const uint32_t io = (uint32_t)pi.w;
-        uint32_t ck = a.ck_w[t];
-        int cc[3];
-        cell_coords<D>(g, ck, cc);
-        uint32_t *row = a.nbr + t;
-        int err = 0;
-        // real neighbours, reference stencil order (i itself excluded: in
-        // cell order its candidate position is t).  A stencil cell whose box
-        // is farther than h from x_i (with a relative margin far above the
-        // rounding of positions and cell indices; edge cells are open
-        // outward, where clamped particles sit) holds no neighbour and is
-        // skipped.  Candidates are read four at a time (indices clamped to
-        // the cell, no branches around the loads); a hit is stored through
-        // a running pointer.  The capacity is checked per group of four:
-        // the list has four guard rows past its last one for the overshoot.
-        uint32_t *wp = row;
-        const char *pcb = (const char *)a.pos_cell;
-        const double hlim = g.hsq * (1.0 + 1e-9);
-        for (int k = cc[0] - 1; k <= cc[0] + 1 && !err; ++k) {
-            if (k < 0 || k >= g.nc[0]) continue;
-            double bx = box_gap(xi[0], k, g.bmin[0], g.cs[0], g.nc[0]);
-            for (int l = cc[1] - 1; l <= cc[1] + 1 && !err; ++l) {
-                if (l < 0 || l >= g.nc[1]) continue;
-                double by = box_gap(xi[1], l, g.bmin[1], g.cs[1], g.nc[1]);
-                for (int m = (D == 3 ? cc[2] - 1 : 0); m <= (D == 3 ? cc[2] + 1 : 0) && !err; ++m) {
-                    if (D == 3 && (m < 0 || m >= g.nc[2])) continue;
-                    double bz = D == 3 ? box_gap(xi[2], m, g.bmin[2], g.cs[2], g.nc[2]) : 0.0;
-                    if (bx * bx + by * by + bz * bz > hlim) continue;
-                    uint32_t c = cell_key<D>(g, k, l, m);
-                    const int q1 = a.cend[c];
-                    for (int q = a.cstart[c]; q < q1; q += 4) {
-                        double4 pj[4];
+    double4 pi = a.pos_w[t];
+    double xi[3] = {pi.x, pi.y, pi.z};
+    const uint32_t io = (uint32_t)pi.w;
+    uint32_t ck = a.ck_w[t];
+    int cc[3];
+    cell_coords<D>(g, ck, cc);
+    int err = 0;
+    // running pointer through the chunked row: the next slot, or the next
+    // chunk row after every eighth entry
+    uint32_t *wp = a.nbr + nb_slot(N, t, 0);
+    const double hlim = g.hsq * (1.0 + 1e-9);
+    for (int k = cc[0] - 1; k <= cc[0] + 1 && !err; ++k) {
+        if (k < 0 || k >= g.nc[0]) continue;
+        double bx = box_gap(xi[0], k, g.bmin[0], g.cs[0], g.nc[0]);
+        for (int l = cc[1] - 1; l <= cc[1] + 1 && !err; ++l) {
+            if (l < 0 || l >= g.nc[1]) continue;
+            double by = box_gap(xi[1], l, g.bmin[1], g.cs[1], g.nc[1]);
+            for (int m = (D == 3 ? cc[2] - 1 : 0); m <= (D == 3 ? cc[2] + 1 : 0) && !err; ++m) {
+                if (D == 3 && (m < 0 || m >= g.nc[2])) continue;
+                double bz = D == 3 ? box_gap(xi[2], m, g.bmin[2], g.cs[2], g.nc[2]) : 0.0;
+                if (bx * bx + by * by + bz * bz > hlim) continue;
+                uint32_t c = cell_key<D>(g, k, l, m);
+                int q0, q1, delta;
+                src.range(k, l, m, c, q0, q1, delta);
+                for (int q = q0; q < q1; q += 4) {
+                    double4 pj[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            // 32-bit byte offset from the array base (N < 2^27)
-                            const uint32_t off = (uint32_t)(q + u < q1 ? q + u : q1 - 1) << 5;
-                            pj[u] = *(const double4 *)(pcb + off);
-                        }
+                    for (int u = 0; u < 4; ++u) pj[u] = src.load((q + u < q1 ? q + u : q1 - 1) + delta);
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const int qq = q + u;
-                            double xj[3] = {pj[u].x, pj[u].y, pj[u].z};
-                            double d = dist2<D>(xj, xi);
-                            bool other;
-                            uint32_t val;
-                            if constexpr (kCaller) {
-                                const uint32_t jo = (uint32_t)pj[u].w;
-                                other = jo != io;
-                                val = jo;
-                            } else {
-                                other = qq != t;
-                                val = (uint32_t)qq;
-                            }
-                            if (qq < q1 && other && d < g.hsq) {
-                                *wp = val | (kCodeReal << kIdxBits);
-                                wp += N;
-                                ++cnt;
-                                // density: the hits in row order, with the
-                                // distance of the test ((-a)^2 = a^2 exactly)
-                                if constexpr (kDens != 0)
-                                    omg += kernel_w<TW>(wdist<D, TW>(d, xj, xi), wh2, wsig);
-                            }
+                    for (int u = 0; u < 4; ++u) {
+                        const int qq = q + u;
+                        double xj[3] = {pj[u].x, pj[u].y, pj[u].z};
+                        double d = dist2<D>(xj, xi);
+                        bool other;
+                        uint32_t val;
+                        if constexpr (kCaller) {
+                            const uint32_t jo = (uint32_t)pj[u].w;
+                            other = jo != io;
+                            val = jo;
+                        } else {
+                            other = qq != t;
+                            val = (uint32_t)qq;
                         }
-                        if (cnt > a.maxr) {
-                            err = 1;
-                            break;
+                        if (qq < q1 && other && d < g.hsq) {
+                            *wp = val | (kCodeReal << kIdxBits);
+                            ++cnt;
+                            wp += (cnt & (kNbChunk - 1)) ? 1 : (size_t)kNbChunk * N - (kNbChunk - 1);
+                            // density: the hits in row order, with the
+                            // distance of the test ((-a)^2 = a^2 exactly)
+                            if constexpr (kDens != 0)
+                                omg += kernel_w<TW>(wdist<D, TW>(d, xj, xi), wh2, wsig);
                         }
+                    }
+                    if (cnt > a.maxr) {
+                        err = 1;
+                        break;
                     }
                 }
             }
-        }
-        if (err) {  // rows stay valid: the first maxr entries, no images
-            cnt = a.maxr;
-            atomicOr(a.status, 1 << 1);
-        }
-        a.noi[t] = cnt;
-        // images can only come from virtual cells within W of the particle's
-        // cell; a particle with no such cell wrapping skips the image search
-        const int W = a.img_window;
-        bool near_edge = false;
-        for (int ax = 0; ax < D; ++ax) near_edge |= cc[ax] - W < 0 || cc[ax] + W >= g.nc[ax];
-        if (!err && g.periodic && near_edge) {
-            uint32_t *grow = row + (size_t)cnt * N;
-            for (int vz = (D == 3 ? cc[2] - W : 0); vz <= (D == 3 ? cc[2] + W : 0) && !err; ++vz)
-                for (int vy = cc[1] - W; vy <= cc[1] + W && !err; ++vy)
-                    for (int vx = cc[0] - W; vx <= cc[0] + W && !err; ++vx) {
-                        int v[3] = {vx, vy, vz};
-                        int s[3] = {0, 0, 0}, r[3] = {0, 0, 0};
-                        bool wrapped = false;
-                        for (int ax = 0; ax < D; ++ax) {
-                            s[ax] = v[ax] < 0 ? -1 : (v[ax] >= g.nc[ax] ? 1 : 0);
-                            r[ax] = v[ax] - s[ax] * g.nc[ax];
-                            wrapped |= s[ax] != 0;
-                        }
-                        if (!wrapped) continue;
-                        uint32_t c = cell_key<D>(g, r[0], r[1], r[2]);
-                        uint32_t code = (uint32_t)((s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1));
-                        int q1 = a.cend[c];
-                        for (int q = a.cstart[c]; q < q1; ++q) {
-                            double4 pj = a.pos_cell[q];
-                            double xj[3] = {pj.x, pj.y, pj.z};
-                            bool exists = true;
-                            for (int ax = 0; ax < D; ++ax) {
-                                if (s[ax] == 0) continue;
-                                bool low = xj[ax] <= g.bmin[ax] + g.h;
-                                bool high = g.bmax[ax] - g.h < xj[ax];
-                                exists &= s[ax] > 0 ? low : (!low && high);
-                                xj[ax] = image_coord(xj[ax], s[ax], g.bmin[ax], g.bmax[ax]);
-                            }
-                            if (!exists) continue;
-                            double d = dist2<D>(xj, xi);
-                            if (!(d < g.hsq)) continue;
-                            if (gcnt >= a.maxg) { err = 1; break; }
-                            const uint32_t jo = (uint32_t)pj.w;
-                            uint64_t ke = ghost_key(jo, code);
-                            int p = gcnt;
-                            while (p > 0) {
-                                uint32_t prev = grow[(size_t)(p - 1) * N];
-                                uint32_t pp = (uint32_t)a.pos_w[nb_index(prev)].w;
-                                if (ghost_key(pp, nb_code(prev)) <= ke) break;
-                                grow[(size_t)p * N] = prev;
-                                --p;
-                            }
-                            grow[(size_t)p * N] = (kCaller ? jo : (uint32_t)q) | (code << kIdxBits);
-                            ++gcnt;
-                        }
-                    }
-            if (err) atomicOr(a.status, 1 << 3);
-        }
-        a.nog[t] = gcnt;
-        if constexpr (kDens != 0) {
-            // compDensity (Particles.cpp:1151-1184): the self term after the
-            // real neighbours, then the images in ghost order (:2262-2290)
-            TW om = omg + kernel_w<TW>(TW(0), wh2, wsig);
-            if (g.periodic) {
-                const uint32_t *grow = a.nbr + t + (size_t)cnt * N;
-                for (int p = 0; p < gcnt; ++p) {
-                    const uint32_t e = grow[(size_t)p * N];
-                    const double4 pj = kCaller ? a.pos_w[nb_index(e)] : a.pos_cell[nb_index(e)];
-                    double xj[3] = {pj.x, pj.y, pj.z};
-                    for (int ax = 0; ax < D; ++ax)
-                        xj[ax] = image_coord(xj[ax], code_shift(nb_code(e), ax), g.bmin[ax], g.bmax[ax]);
-                    om += kernel_w<TW>(wdist<D, TW>(dist2<D>(xi, xj), xi, xj), wh2, wsig);
-                }
-            }
-            write_prim<D, TW>((TW *)a.prim + (size_t)t * kPrimN, om, a.m[io], a.u[io],
-                              a.v[0][io], a.v[1][io], D == 3 ? a.v[2][io] : 0., a.gamma);
         }
     }
-    // largest counts: a wavefront maximum, one atomic per wavefront
+    if (err) {  // rows stay valid: the first maxr entries, no images
+        cnt = a.maxr;
+        atomicOr(a.status, 1 << 1);
+    }
+    a.noi[t] = cnt;
+    // images can only come from virtual cells within W of the particle's
+    // cell; a particle with no such cell wrapping skips the image search
+    const int W = a.img_window;
+    bool near_edge = false;
+    for (int ax = 0; ax < D; ++ax) near_edge |= cc[ax] - W < 0 || cc[ax] + W >= g.nc[ax];
+    if (!err && g.periodic && near_edge) {
+        uint32_t *nb = a.nbr;
+        const int g0 = cnt;  // images follow the real entries
+        for (int vz = (D == 3 ? cc[2] - W : 0); vz <= (D == 3 ? cc[2] + W : 0) && !err; ++vz)
+            for (int vy = cc[1] - W; vy <= cc[1] + W && !err; ++vy)
+                for (int vx = cc[0] - W; vx <= cc[0] + W && !err; ++vx) {
+                    int v[3] = {vx, vy, vz};
+                    int s[3] = {0, 0, 0}, r[3] = {0, 0, 0};
+                    bool wrapped = false;
+                    for (int ax = 0; ax < D; ++ax) {
+                        s[ax] = v[ax] < 0 ? -1 : (v[ax] >= g.nc[ax] ? 1 : 0);
+                        r[ax] = v[ax] - s[ax] * g.nc[ax];
+                        wrapped |= s[ax] != 0;
+                    }
+                    if (!wrapped) continue;
+                    uint32_t c = cell_key<D>(g, r[0], r[1], r[2]);
+                    uint32_t code = (uint32_t)((s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1));
+                    int q1 = a.cend[c];
+                    for (int q = a.cstart[c]; q < q1; ++q) {
+                        double4 pj = a.pos_cell[q];
+                        double xj[3] = {pj.x, pj.y, pj.z};
+                        bool exists = true;
+                        for (int ax = 0; ax < D; ++ax) {
+                            if (s[ax] == 0) continue;
+                            bool low = xj[ax] <= g.bmin[ax] + g.h;
+                            bool high = g.bmax[ax] - g.h < xj[ax];
+                            exists &= s[ax] > 0 ? low : (!low && high);
+                            xj[ax] = image_coord(xj[ax], s[ax], g.bmin[ax], g.bmax[ax]);
+                        }
+                        if (!exists) continue;
+                        double d = dist2<D>(xj, xi);
+                        if (!(d < g.hsq)) continue;
+                        if (gcnt >= a.maxg) { err = 1; break; }
+                        const uint32_t jo = (uint32_t)pj.w;
+                        uint64_t ke = ghost_key(jo, code);
+                        int p = gcnt;
+                        while (p > 0) {
+                            uint32_t prev = nb[nb_slot(N, t, g0 + p - 1)];
+                            uint32_t pp = (uint32_t)a.pos_w[nb_index(prev)].w;
+                            if (ghost_key(pp, nb_code(prev)) <= ke) break;
+                            nb[nb_slot(N, t, g0 + p)] = prev;
+                            --p;
+                        }
+                        nb[nb_slot(N, t, g0 + p)] = (kCaller ? jo : (uint32_t)q) | (code << kIdxBits);
+                        ++gcnt;
+                    }
+                }
+        if (err) atomicOr(a.status, 1 << 3);
+    }
+    a.nog[t] = gcnt;
+    if constexpr (kDens != 0) {
+        // compDensity (Particles.cpp:1151-1184): the self term after the
+        // real neighbours, then the images in ghost order (:2262-2290)
+        TW om = omg + kernel_w<TW>(TW(0), wh2, wsig);
+        if (g.periodic) {
+            for (int p = 0; p < gcnt; ++p) {
+                const uint32_t e = nb_get(a.nbr, N, t, cnt + p);
+                const double4 pj = kCaller ? a.pos_w[nb_index(e)] : a.pos_cell[nb_index(e)];
+                double xj[3] = {pj.x, pj.y, pj.z};
+                for (int ax = 0; ax < D; ++ax)
+                    xj[ax] = image_coord(xj[ax], code_shift(nb_code(e), ax), g.bmin[ax], g.bmax[ax]);
+                om += kernel_w<TW>(wdist<D, TW>(dist2<D>(xi, xj), xi, xj), wh2, wsig);
+            }
+        }
+        write_prim<D, TW>((TW *)a.prim + (size_t)t * kPrimN, om, a.m[io], a.u[io], a.v[0][io],
+                          a.v[1][io], D == 3 ? a.v[2][io] : 0., a.gamma);
+    }
+    cnt_out = cnt;
+    gcnt_out = gcnt;
+}
+
+// largest counts of the step: a wavefront maximum, one atomic per wavefront
+// (every lane of the wavefront calls this)
+__device__ inline void reduce_max_counts(const NbrArgs &a, int cnt, int gcnt) {
     for (int off = 32; off > 0; off >>= 1) {
         int o = __shfl_xor(cnt, off, 64);
         cnt = o > cnt ? o : cnt;
@@ -463,6 +484,20 @@ __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
         atomicMax(&a.max_counts[0], cnt);
         atomicMax(&a.max_counts[1], gcnt);
     }
+}
+
+// kDens: 0, or the bytes of the primitive records (4, 8) when the step's
+// density is fused into the search (NbrArgs::dens).  One thread per work
+// position, candidates from global memory: caller order and linear keys.
+template <int D, bool kCaller, int kDens>
+__global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    int cnt = 0, gcnt = 0;
+    if (t < a.N) {
+        const GlobalCand src{(const char *)a.pos_cell, a.cstart, a.cend};
+        search_one<D, kCaller, kDens>(a, t, src, cnt, gcnt);
+    }
+    reduce_max_counts(a, cnt, gcnt);
 }
 
 template __global__ void k_assign_cells<2>(Geom, const double *, const double *, const double *,

@@ -27,6 +27,35 @@ __host__ __device__ inline int code_mask(uint32_t code) {
         if (code_shift(code, a) != 0) m |= 1 << a;
     return m;
 }
+// Neighbour rows, chunked: entry n of work position t sits at word
+// ((n >> 3) * N + t) * 8 + (n & 7).  The eight entries 8c..8c+7 of one
+// position are 32 contiguous bytes (one sector), so the search fills a
+// position's sector before moving on, and a wavefront reads eight entries
+// per lane with two 16-byte loads at consecutive sectors.
+constexpr int kNbChunk = 8;
+__host__ __device__ inline size_t nb_slot(int N, int t, int n) {
+    return ((size_t)(n >> 3) * (size_t)N + (size_t)t) * kNbChunk + (size_t)(n & 7);
+}
+// words of a row array for S entries per position
+__host__ __device__ inline size_t nb_words(int N, int S) {
+    return (size_t)((S + kNbChunk - 1) / kNbChunk) * kNbChunk * (size_t)N;
+}
+__device__ inline uint32_t nb_get(const uint32_t *nbr, int N, int t, int n) {
+    return nbr[nb_slot(N, t, n)];
+}
+// entries n..n+7, n a multiple of 8
+__device__ inline void nb_get8(const uint32_t *nbr, int N, int t, int n, uint32_t *e) {
+    const uint4 *p = (const uint4 *)(nbr + nb_slot(N, t, n));
+    const uint4 a = p[0], b = p[1];
+    e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+    e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w;
+}
+// entries n..n+3, n a multiple of 4
+__device__ inline void nb_get4(const uint32_t *nbr, int N, int t, int n, uint32_t *e) {
+    const uint4 a = *(const uint4 *)(nbr + nb_slot(N, t, n));
+    e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+}
+
 // ghost-index order of the reference: parent (original index) ascending,
 // then the image subsets of one parent in descending axis-mask order
 __host__ __device__ inline uint64_t ghost_key(uint32_t parent, uint32_t code) {

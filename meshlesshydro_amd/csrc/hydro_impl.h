@@ -124,8 +124,7 @@ __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
     int n = 0;
     for (; n + 8 <= nr; n += 8) {
         uint32_t e[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) e[u] = a.nbr[(size_t)(n + u) * N + t];
+        nb_get8(a.nbr, N, t, n, e);
         double4 pj[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) pj[u] = a.pos[nb_index(e[u])];
@@ -137,7 +136,7 @@ __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
         }
     }
     for (; n < nr; ++n) {
-        const double4 pj = a.pos[nb_index(a.nbr[(size_t)n * N + t])];
+        const double4 pj = a.pos[nb_index(nb_get(a.nbr, N, t, n))];
         const double xj[3] = {pj.x, pj.y, pj.z};
         T r = tsqrt(T(dist2<D>(xi, xj)));
         omg += kernel_w<T>(r, h2, sig);
@@ -150,7 +149,7 @@ __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
         for (; n + 8 <= nr + ng; n += 8) {
             uint32_t e[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) e[u] = a.nbr[(size_t)(n + u) * N + t];
+            for (int u = 0; u < 8; ++u) e[u] = nb_get(a.nbr, N, t, n + u);
             double4 pj[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) pj[u] = a.pos[nb_index(e[u])];
@@ -165,7 +164,7 @@ __global__ __launch_bounds__(256) void k_density(HydroArgs a) {
         for (; n < nr + ng; ++n) {
             double xj[3];
             int s[3];
-            nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
+            nb_pos<D>(a, nb_get(a.nbr, N, t, n), xj, s);
             T r = tsqrt(T(dist2<D>(xi, xj)));
             og += kernel_w<T>(r, h2, sig);
         }
@@ -239,7 +238,7 @@ __device__ inline void slope_limit(const HydroArgs &a, int t, int cnt, const dou
     for (int n = 0; n < cnt; ++n) {
         double xj[3];
         int s[3];
-        nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
+        nb_pos<D>(a, nb_get(a.nbr, N, t, n), xj, s);
         double xijxi[D];
         for (int ax = 0; ax < D; ++ax) {
             double xij = a.foqp ? (xi[ax] + xj[ax]) / 2.
@@ -379,8 +378,12 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
             constexpr int BG = kGradBatch;
             for (; n + BG <= cnt; n += BG) {
                 uint32_t eb[BG];
+                if constexpr (BG == 4) {
+                    nb_get4(a.nbr, N, t, n, eb);
+                } else {
 #pragma unroll
-                for (int u = 0; u < BG; ++u) eb[u] = a.nbr[(size_t)(n + u) * N + t];
+                    for (int u = 0; u < BG; ++u) eb[u] = nb_get(a.nbr, N, t, n + u);
+                }
                 double4 pb[BG];
                 T rb[BG][kPrimN];
 #pragma unroll
@@ -400,7 +403,7 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
         for (; n < cnt; ++n) {
             double xj[3];
             int s[3];
-            int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
+            int j = nb_pos<D>(a, nb_get(a.nbr, N, t, n), xj, s);
             T prj[kPrimN];
             if (kFused) load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
             visit(n, j, xj, prj);
@@ -424,7 +427,7 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
             for (int n = 0; n < cnt; ++n) {
                 double xj[3];
                 int s[3];
-                int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
+                int j = nb_pos<D>(a, nb_get(a.nbr, N, t, n), xj, s);
                 T r = tsqrt(T(dist2<D>(xi, xj)));
                 T w = kernel_w<T>(r, h2, sig) / om_i;
                 T psi[D];
@@ -922,7 +925,7 @@ template <int D, typename T>
 __device__ inline void face_of_entry(const HydroArgs &a, int t, int i, int n, T dt,
                                      FacePrep<D, T> &fp, MixedProbe &mp, bool &self_lo,
                                      FaceStats &fs) {
-    const uint32_t e = a.nbr[(size_t)n * a.N + t];
+    const uint32_t e = nb_get(a.nbr, a.N, t, n);
     face_of_entry<D, T>(a, t, i, e, (int)a.pos[nb_index(e)].w, dt, fp, mp, self_lo, fs);
 }
 
@@ -1136,8 +1139,8 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     // waits only on its records
     uint32_t e0 = 0, e1 = 0;
     int jo0 = 0;
-    if (cnt > 0) e0 = a.nbr[t];
-    if (cnt > 1) e1 = a.nbr[(size_t)N + t];
+    if (cnt > 0) e0 = nb_get(a.nbr, N, t, 0);
+    if (cnt > 1) e1 = nb_get(a.nbr, N, t, 1);
     if (cnt > 0) jo0 = (int)a.pos[nb_index(e0)].w;
     for (int n = 0; n < lim; ++n) {
         const bool act = n < cnt;
@@ -1145,7 +1148,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
         const int jo = jo0;
         if (n + 1 < cnt) jo0 = (int)a.pos[nb_index(e1)].w;
         e0 = e1;
-        if (n + 2 < cnt) e1 = a.nbr[(size_t)(n + 2) * N + t];
+        if (n + 2 < cnt) e1 = nb_get(a.nbr, N, t, n + 2);
         FacePrep<D, T> fp;
         MixedProbe mp{};
         bool self_lo = true;

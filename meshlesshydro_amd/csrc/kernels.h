@@ -37,7 +37,7 @@ struct NbrArgs {
 // caller's particle order (sidx maps; null when the two orders coincide).
 struct HydroArgs {
     Geom g;
-    int N, stride;               // stride = rows of nbr (N)
+    int N;
     const uint32_t *sidx;        // work position -> original index, or null
     double gamma, cfl, beta, psi1, psi2, riemann_tol;
     int riemann_max_iter, learned_iters;

@@ -43,7 +43,7 @@ __device__ inline int mirror_entry(const HydroArgs &a, int t, int j, bool image)
     const int N = a.N, nr = a.noi[j], ng = a.nog[j];
     const int lo = image ? nr : 0, hi = image ? nr + ng : nr;
     for (int m = lo; m < hi; ++m)
-        if ((int)nb_index(a.nbr[(size_t)m * N + j]) == t) return m;
+        if ((int)nb_index(nb_get(a.nbr, N, j, m)) == t) return m;
     return -1;
 }
 
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_limit(HydroArgs a) {
     }
     const int cnt = a.noi[t] + a.nog[t];
     for (int n = 0; n < cnt; ++n) {
-        const int j = (int)nb_index(a.nbr[(size_t)n * N + t]);
+        const int j = (int)nb_index(nb_get(a.nbr, N, t, n));
         const T *prj = prim + (size_t)j * kPrimN;
         for (int f = 0; f < NF; ++f) {
             T fj = prj[field_slot<D>(f)];
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_gradient_field(HydroArgs a, const doubl
     for (int n = 0; n < cnt; ++n) {
         double xj[3];
         int s[3];
-        const int j = nb_pos<D>(a, a.nbr[(size_t)n * N + t], xj, s);
+        const int j = nb_pos<D>(a, nb_get(a.nbr, N, t, n), xj, s);
         T r = tsqrt(T(dist2<D>(xi, xj)));
         T w = kernel_w<T>(r, h2, sig) / om_i;
         const double df = f[orig_index(a, j)] - fi;
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void k_face_states(HydroArgs a, FaceArrays fa,
     const View<D, T> L = make_view<D, T>(a, t);
     const int cnt = a.noi[t] + a.nog[t];
     for (int n = 0; n < cnt; ++n) {
-        const uint32_t e = a.nbr[(size_t)n * N + t];
+        const uint32_t e = nb_get(a.nbr, N, t, n);
         const int j = (int)nb_index(e);
         const uint32_t code = nb_code(e);
         const bool image = code != kCodeReal;
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_face_solve(HydroArgs a, FaceArrays fa, 
         const int N = a.N, i = orig_index(a, t);
         const int cnt = a.noi[t] + a.nog[t];
         for (int n = 0; n < cnt; ++n) {
-            const uint32_t e = a.nbr[(size_t)n * N + t];
+            const uint32_t e = nb_get(a.nbr, N, t, n);
             const int j = (int)nb_index(e);
             const bool image = nb_code(e) != kCodeReal;
             const int jo = orig_index(a, j);
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_flux_check(HydroArgs a, FaceArrays fa, 
         const int N = a.N;
         const int cnt = a.noi[t] + a.nog[t];
         for (int n = 0; n < cnt; ++n) {
-            const uint32_t e = a.nbr[(size_t)n * N + t];
+            const uint32_t e = nb_get(a.nbr, N, t, n);
             const int j = (int)nb_index(e);
             const bool image = nb_code(e) != kCodeReal;
             const int m = mirror_entry(a, t, j, image);
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_mirror_check(HydroArgs a) {
         const int N = a.N;
         const int cnt = a.noi[t] + a.nog[t];
         for (int n = 0; n < cnt; ++n) {
-            const uint32_t e = a.nbr[(size_t)n * N + t];
+            const uint32_t e = nb_get(a.nbr, N, t, n);
             if (mirror_entry(a, t, (int)nb_index(e), nb_code(e) != kCodeReal) < 0) ++missing;
         }
     }
