@@ -455,7 +455,8 @@ static int enqueue_neighbours(mlh_ctx *c, bool fuse_density = false) {
         for (int k = 0; k < 3; ++k) a.v[k] = c->v[k];
         a.gamma = c->p.gamma;
     }
-    launch_neighbours(a, c->D, c->s);
+    const bool fused = launch_neighbours(a, c->D, c->s);
+    if (fuse_density && !fused) launch_density(hydro_args(c), c->D, c->prec, c->s);
     c->face_stage = 0;
     c->grad_state = 0;
     // DEBUG_LVL >= 1: every entry's partner must list the particle (the face

@@ -306,101 +306,21 @@ struct GlobalCand {
     }
 };
 
-// The search of work position t: real neighbours in the reference's stencil
-// order (i itself excluded: in cell order its candidate position is t), the
-// periodic images in ghost order, and with kDens the step's density.
-//  * A stencil cell whose box is farther than h from x_i (with a relative
-//    margin far above the rounding of positions and cell indices; edge
-//    cells are open outward, where clamped particles sit) holds no
-//    neighbour and is skipped.
-//  * Candidates are read four at a time (indices clamped to the cell, no
-//    branches around the loads); a hit is stored through a running pointer
-//    into the chunked row, so a position's eight consecutive entries fill
-//    one 32-byte sector.
-//  * The capacity is checked per group of four: the list has four guard
-//    entries past its last one for the overshoot.
-// Returns the real and image counts through cnt / gcnt.
-template <int D, bool kCaller, int kDens, typename Src>
-__device__ inline void search_one(const NbrArgs &a, int t, const Src &src, int &cnt_out,
-                                  int &gcnt_out) {
-    using TW = typename std::conditional<kDens == 4, float, double>::type;
+// Periodic images of work position t (position xi, cell cc) in the
+// reference's ghost order, stored after its cnt real entries: only
+// particles whose cell lies within the image window of a domain face
+// search.  Returns the image count (an overflow sets the status bit and
+// keeps the entries found so far).
+template <int D, bool kCaller>
+__device__ inline int image_search(const NbrArgs &a, int t, const double *xi, const int *cc,
+                                   int cnt) {
     const Geom &g = a.g;
     const int N = a.N;
-    int cnt = 0, gcnt = 0;
-    const TW wh2 = TW(g.h / 2.), wsig = TW(g.sigma);
-    TW omg = TW(0);
-    double4 pi = a.pos_w[t];
-    double xi[3] = {pi.x, pi.y, pi.z};
-    const uint32_t io = (uint32_t)pi.w;
-    uint32_t ck = a.ck_w[t];
-    int cc[3];
-    cell_coords<D>(g, ck, cc);
-    int err = 0;
-    // running pointer through the chunked row: the next slot, or the next
-    // chunk row after every eighth entry
-    uint32_t *wp = a.nbr + nb_slot(N, t, 0);
-    const double hlim = g.hsq * (1.0 + 1e-9);
-    for (int k = cc[0] - 1; k <= cc[0] + 1 && !err; ++k) {
-        if (k < 0 || k >= g.nc[0]) continue;
-        double bx = box_gap(xi[0], k, g.bmin[0], g.cs[0], g.nc[0]);
-        for (int l = cc[1] - 1; l <= cc[1] + 1 && !err; ++l) {
-            if (l < 0 || l >= g.nc[1]) continue;
-            double by = box_gap(xi[1], l, g.bmin[1], g.cs[1], g.nc[1]);
-            for (int m = (D == 3 ? cc[2] - 1 : 0); m <= (D == 3 ? cc[2] + 1 : 0) && !err; ++m) {
-                if (D == 3 && (m < 0 || m >= g.nc[2])) continue;
-                double bz = D == 3 ? box_gap(xi[2], m, g.bmin[2], g.cs[2], g.nc[2]) : 0.0;
-                if (bx * bx + by * by + bz * bz > hlim) continue;
-                uint32_t c = cell_key<D>(g, k, l, m);
-                int q0, q1, delta;
-                src.range(k, l, m, c, q0, q1, delta);
-                for (int q = q0; q < q1; q += 4) {
-                    double4 pj[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) pj[u] = src.load((q + u < q1 ? q + u : q1 - 1) + delta);
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int qq = q + u;
-                        double xj[3] = {pj[u].x, pj[u].y, pj[u].z};
-                        double d = dist2<D>(xj, xi);
-                        bool other;
-                        uint32_t val;
-                        if constexpr (kCaller) {
-                            const uint32_t jo = (uint32_t)pj[u].w;
-                            other = jo != io;
-                            val = jo;
-                        } else {
-                            other = qq != t;
-                            val = (uint32_t)qq;
-                        }
-                        if (qq < q1 && other && d < g.hsq) {
-                            *wp = val | (kCodeReal << kIdxBits);
-                            ++cnt;
-                            wp += (cnt & (kNbChunk - 1)) ? 1 : (size_t)kNbChunk * N - (kNbChunk - 1);
-                            // density: the hits in row order, with the
-                            // distance of the test ((-a)^2 = a^2 exactly)
-                            if constexpr (kDens != 0)
-                                omg += kernel_w<TW>(wdist<D, TW>(d, xj, xi), wh2, wsig);
-                        }
-                    }
-                    if (cnt > a.maxr) {
-                        err = 1;
-                        break;
-                    }
-                }
-            }
-        }
-    }
-    if (err) {  // rows stay valid: the first maxr entries, no images
-        cnt = a.maxr;
-        atomicOr(a.status, 1 << 1);
-    }
-    a.noi[t] = cnt;
-    // images can only come from virtual cells within W of the particle's
-    // cell; a particle with no such cell wrapping skips the image search
+    int gcnt = 0, err = 0;
     const int W = a.img_window;
     bool near_edge = false;
     for (int ax = 0; ax < D; ++ax) near_edge |= cc[ax] - W < 0 || cc[ax] + W >= g.nc[ax];
-    if (!err && g.periodic && near_edge) {
+    if (g.periodic && near_edge) {
         uint32_t *nb = a.nbr;
         const int g0 = cnt;  // images follow the real entries
         for (int vz = (D == 3 ? cc[2] - W : 0); vz <= (D == 3 ? cc[2] + W : 0) && !err; ++vz)
@@ -449,6 +369,110 @@ __device__ inline void search_one(const NbrArgs &a, int t, const Src &src, int &
                 }
         if (err) atomicOr(a.status, 1 << 3);
     }
+    return gcnt;
+}
+
+// The search of work position t: real neighbours in the reference's stencil
+// order (i itself excluded: in cell order its candidate position is t), the
+// periodic images in ghost order, and with kDens the step's density.
+//  * A stencil cell whose box is farther than h from x_i (with a relative
+//    margin far above the rounding of positions and cell indices; edge
+//    cells are open outward, where clamped particles sit) holds no
+//    neighbour and is skipped.
+//  * Candidates are read four at a time (indices clamped to the cell, no
+//    branches around the loads); a hit goes to the thread's LDS stage, and
+//    every eight hits are stored as one 32-byte sector of the chunked row.
+//  * The capacity is checked per group of four: the list has four guard
+//    entries past its last one for the overshoot.
+// Returns the real and image counts through cnt / gcnt.
+template <int D, bool kCaller, int kDens, typename Src>
+__device__ inline void search_one(const NbrArgs &a, int t, const Src &src, uint32_t *stage,
+                                  int &cnt_out, int &gcnt_out) {
+    using TW = typename std::conditional<kDens == 4, float, double>::type;
+    const Geom &g = a.g;
+    const int N = a.N;
+    int cnt = 0, gcnt = 0;
+    const TW wh2 = TW(g.h / 2.), wsig = TW(g.sigma);
+    TW omg = TW(0);
+    double4 pi = a.pos_w[t];
+    double xi[3] = {pi.x, pi.y, pi.z};
+    const uint32_t io = (uint32_t)pi.w;
+    uint32_t ck = a.ck_w[t];
+    int cc[3];
+    cell_coords<D>(g, ck, cc);
+    int err = 0;
+    // hits go to the thread's LDS stage (two chunks of eight entries, entry
+    // n at stage[n & 15]); each completed chunk is stored as one 32-byte
+    // sector of the row, so every sector is written once and whole
+    int flushed = 0;  // chunks stored
+    auto flush = [&]() {
+        const uint4 *sp = (const uint4 *)(stage + 8 * (flushed & 1));
+        uint4 *dp = (uint4 *)(a.nbr + nb_slot(N, t, 8 * flushed));
+        dp[0] = sp[0];
+        dp[1] = sp[1];
+        ++flushed;
+    };
+    const double hlim = g.hsq * (1.0 + 1e-9);
+    for (int k = cc[0] - 1; k <= cc[0] + 1 && !err; ++k) {
+        if (k < 0 || k >= g.nc[0]) continue;
+        double bx = box_gap(xi[0], k, g.bmin[0], g.cs[0], g.nc[0]);
+        for (int l = cc[1] - 1; l <= cc[1] + 1 && !err; ++l) {
+            if (l < 0 || l >= g.nc[1]) continue;
+            double by = box_gap(xi[1], l, g.bmin[1], g.cs[1], g.nc[1]);
+            for (int m = (D == 3 ? cc[2] - 1 : 0); m <= (D == 3 ? cc[2] + 1 : 0) && !err; ++m) {
+                if (D == 3 && (m < 0 || m >= g.nc[2])) continue;
+                double bz = D == 3 ? box_gap(xi[2], m, g.bmin[2], g.cs[2], g.nc[2]) : 0.0;
+                if (bx * bx + by * by + bz * bz > hlim) continue;
+                uint32_t c = cell_key<D>(g, k, l, m);
+                int q0, q1, delta;
+                src.range(k, l, m, c, q0, q1, delta);
+                for (int q = q0; q < q1; q += 4) {
+                    double4 pj[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) pj[u] = src.load((q + u < q1 ? q + u : q1 - 1) + delta);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int qq = q + u;
+                        double xj[3] = {pj[u].x, pj[u].y, pj[u].z};
+                        double d = dist2<D>(xj, xi);
+                        bool other;
+                        uint32_t val;
+                        if constexpr (kCaller) {
+                            const uint32_t jo = (uint32_t)pj[u].w;
+                            other = jo != io;
+                            val = jo;
+                        } else {
+                            other = qq != t;
+                            val = (uint32_t)qq;
+                        }
+                        if (qq < q1 && other && d < g.hsq) {
+                            stage[cnt & 15] = val | (kCodeReal << kIdxBits);
+                            ++cnt;
+                            // density: the hits in row order, with the
+                            // distance of the test ((-a)^2 = a^2 exactly)
+                            if constexpr (kDens != 0)
+                                omg += kernel_w<TW>(wdist<D, TW>(d, xj, xi), wh2, wsig);
+                        }
+                    }
+                    // at most one chunk completes per group of four
+                    if (cnt >= 8 * (flushed + 1)) flush();
+                    if (cnt > a.maxr) {
+                        err = 1;
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    // the last, partial chunk (and, after an overflow, up to one more);
+    // slots past the count are never read (images overwrite theirs)
+    while (8 * flushed < cnt) flush();
+    if (err) {  // rows stay valid: the first maxr entries, no images
+        cnt = a.maxr;
+        atomicOr(a.status, 1 << 1);
+    }
+    a.noi[t] = cnt;
+    if (!err) gcnt = image_search<D, kCaller>(a, t, xi, cc, cnt);
     a.nog[t] = gcnt;
     if constexpr (kDens != 0) {
         // compDensity (Particles.cpp:1151-1184): the self term after the
@@ -491,11 +515,12 @@ __device__ inline void reduce_max_counts(const NbrArgs &a, int cnt, int gcnt) {
 // position, candidates from global memory: caller order and linear keys.
 template <int D, bool kCaller, int kDens>
 __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
+    __shared__ __align__(16) uint32_t s_stage[256 * 16];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     int cnt = 0, gcnt = 0;
     if (t < a.N) {
         const GlobalCand src{(const char *)a.pos_cell, a.cstart, a.cend};
-        search_one<D, kCaller, kDens>(a, t, src, cnt, gcnt);
+        search_one<D, kCaller, kDens>(a, t, src, s_stage + 16 * threadIdx.x, cnt, gcnt);
     }
     reduce_max_counts(a, cnt, gcnt);
 }
@@ -690,7 +715,7 @@ static void launch_nbr(const NbrArgs &a, dim3 grid, hipStream_t s) {
     else hipLaunchKernelGGL((k_neighbours<D, kCaller, 0>), grid, dim3(256), 0, s, a);
 }
 
-void launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {
+bool launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {
     dim3 grid((a.N + 255) / 256);
     if (a.caller_order) {
         if (D == 2) launch_nbr<2, true>(a, grid, s);
@@ -699,6 +724,7 @@ void launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {
         if (D == 2) launch_nbr<2, false>(a, grid, s);
         else launch_nbr<3, false>(a, grid, s);
     }
+    return a.dens != 0;
 }
 
 }  // namespace mlh

@@ -210,6 +210,14 @@ struct FRec {
 
 __device__ inline int lane_id() { return __lane_id(); }
 
+// LDS written by some lanes of a wavefront, then read by others: a
+// wavefront-scope release / barrier / acquire
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // XCD-aware block index.  Workgroups are dispatched round-robin over the 8
 // XCDs of an MI355X (workgroup w runs on XCD w % 8); this bijection on
 // [0, gridDim.x) gives each XCD one contiguous range of logical blocks, so

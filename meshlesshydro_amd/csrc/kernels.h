@@ -114,7 +114,9 @@ enum CounterIdx {
 void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
                   const double *x2, int N, double4 *pos, uint32_t *key, uint32_t *idx,
                   uint32_t *skey, uint32_t *scratch, int *cstart, int *cend, hipStream_t s);
-void launch_neighbours(const NbrArgs &a, int D, hipStream_t s);
+// returns whether the search computed the density (NbrArgs::dens): the
+// cell-order search does not, and the caller then runs the density stage
+bool launch_neighbours(const NbrArgs &a, int D, hipStream_t s);
 // Particles::getDomainLimits with the reference's sequential semantics;
 // scratch: 3 * D * limits_blocks(N) doubles; out: [min, max] per axis
 int limits_blocks(int N);

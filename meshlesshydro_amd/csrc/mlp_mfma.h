@@ -10,6 +10,7 @@
 // operand after relabelling K; mfma_h1_neuron is that relabelling.
 #pragma once
 #include <hip/hip_runtime.h>
+#include "device_common.h"
 #include "mlp_api.h"
 #include "mlp_device.h"
 
@@ -47,12 +48,6 @@ __device__ inline void mlp_load_regs(const MlpFrags *__restrict__ F, MlpRegs &R)
 __device__ inline float tanh_fast(float x) {
     float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
     return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
-}
-
-__device__ inline void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // All 64 lanes must call this together.  x: the calling lane's 10 features.
