@@ -903,7 +903,7 @@ __global__ __launch_bounds__(256) void k_riemann_flux(HydroArgs a, long long n,
 template <int D, typename T>
 __device__ inline void face_of_entry(const HydroArgs &a, int t, int i, uint32_t e, int jo, T dt,
                                      FacePrep<D, T> &fp, MixedProbe &mp, bool &self_lo,
-                                     FaceStats &fs) {
+                                     FaceStats &fs, const View<D, T> *own = nullptr) {
     int j = (int)nb_index(e);
     uint32_t code = nb_code(e);
     bool image = code != kCodeReal;
@@ -911,8 +911,17 @@ __device__ inline void face_of_entry(const HydroArgs &a, int t, int i, uint32_t 
     if (image)
         for (int ax = 0; ax < D; ++ax) s[ax] = code_shift(code, ax);
     self_lo = !a.enforce_flux_sym || (image ? jo >= i : jo > i);
-    const View<D, T> L = make_view<D, T>(a, self_lo ? t : j);
-    const View<D, T> H = make_view<D, T>(a, self_lo ? j : t);
+    View<D, T> L, H;
+    if (own) {
+        // the particle's own view is held in registers for its whole row:
+        // only the neighbour's is gathered, and the sides are selected
+        const View<D, T> nb = make_view<D, T>(a, j);
+        L = self_lo ? *own : nb;
+        H = self_lo ? nb : *own;
+    } else {
+        L = make_view<D, T>(a, self_lo ? t : j);
+        H = make_view<D, T>(a, self_lo ? j : t);
+    }
     int sl[3];
     for (int ax = 0; ax < 3; ++ax) sl[ax] = self_lo ? s[ax] : -s[ax];
     face_prepare<D, T>(a, L, H, sl, image, dt, fp, fs);
@@ -1131,6 +1140,8 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
         }
     }
     const bool collect = a.collect && live && (i % a.collect_stride) == 0;
+    // the particle's own view, read once for its row
+    const View<D, T> own = make_view<D, T>(a, live ? t : 0);
     FluxSums acc{0., 0., {0., 0., 0.}};
     uint64_t defer[2] = {0ull, 0ull};
     bool defer_hi = false;
@@ -1153,7 +1164,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
         MixedProbe mp{};
         bool self_lo = true;
         const int it0 = fs.st.iters;
-        if (act) face_of_entry<D, T>(a, t, i, e, jo, dt, fp, mp, self_lo, fs);
+        if (act) face_of_entry<D, T>(a, t, i, e, jo, dt, fp, mp, self_lo, fs, &own);
         float seed = -1.f;
         if constexpr (kInLoopNet) {
             // FP64: the network runs for faces with a real jump
