@@ -1140,8 +1140,11 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
         }
     }
     const bool collect = a.collect && live && (i % a.collect_stride) == 0;
-    // the particle's own view, read once for its row
-    const View<D, T> own = make_view<D, T>(a, live ? t : 0);
+    // the particle's own view, read once for its row, where the registers
+    // it takes cost no occupancy (the 2D FP64 kernel would drop from 2 to
+    // 1 wavefront per SIMD and gathers both views per face instead)
+    constexpr bool kOwnView = kFp32 || D == 3;
+    const View<D, T> own = make_view<D, T>(a, live && kOwnView ? t : 0);
     FluxSums acc{0., 0., {0., 0., 0.}};
     uint64_t defer[2] = {0ull, 0ull};
     bool defer_hi = false;
@@ -1164,7 +1167,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
         MixedProbe mp{};
         bool self_lo = true;
         const int it0 = fs.st.iters;
-        if (act) face_of_entry<D, T>(a, t, i, e, jo, dt, fp, mp, self_lo, fs, &own);
+        if (act) face_of_entry<D, T>(a, t, i, e, jo, dt, fp, mp, self_lo, fs, kOwnView ? &own : nullptr);
         float seed = -1.f;
         if constexpr (kInLoopNet) {
             // FP64: the network runs for faces with a real jump
