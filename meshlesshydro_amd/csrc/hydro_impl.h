@@ -235,10 +235,10 @@ __device__ inline void slope_limit(const HydroArgs &a, int t, int cnt, const dou
         midmax[f] = tmin_pos<T>();
         midmin[f] = tmax_val<T>();
     }
-    for (int n = 0; n < cnt; ++n) {
-        double xj[3];
-        int s[3];
-        nb_pos<D>(a, nb_get(a.nbr, N, t, n), xj, s);
+    double gd[NF][D];
+    for (int f = 0; f < NF; ++f)
+        for (int ax = 0; ax < D; ++ax) gd[f][ax] = (double)grad[f][ax];
+    auto visit = [&](const double *xj) {
         double xijxi[D];
         for (int ax = 0; ax < D; ++ax) {
             double xij = a.foqp ? (xi[ax] + xj[ax]) / 2.
@@ -246,12 +246,33 @@ __device__ inline void slope_limit(const HydroArgs &a, int t, int cnt, const dou
             xijxi[ax] = xij - xi[ax];
         }
         for (int f = 0; f < NF; ++f) {
-            double gd[D];
-            for (int ax = 0; ax < D; ++ax) gd[ax] = (double)grad[f][ax];
-            double fij = (double)fi[f] + dotT<D, double>(gd, xijxi);
+            double fij = (double)fi[f] + dotT<D, double>(gd[f], xijxi);
             if (midmax[f] < fij) midmax[f] = fij;
             if (midmin[f] > fij) midmin[f] = fij;
         }
+    };
+    // neighbours in batches of four: the four row entries (one 16-byte
+    // load), then their positions, are in flight together; the batch is
+    // visited in entry order
+    int n = 0;
+    for (; n + 4 <= cnt; n += 4) {
+        uint32_t e[4];
+        nb_get4(a.nbr, N, t, n, e);
+        double4 pb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pb[u] = a.pos[nb_index(e[u])];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            double xj[3];
+            entry_pos<D>(a, e[u], pb[u], xj);
+            visit(xj);
+        }
+    }
+    for (; n < cnt; ++n) {
+        double xj[3];
+        int s[3];
+        nb_pos<D>(a, nb_get(a.nbr, N, t, n), xj, s);
+        visit(xj);
     }
     const double beta = a.beta;
     for (int f = 0; f < NF; ++f) {
