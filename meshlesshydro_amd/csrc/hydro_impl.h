@@ -391,11 +391,11 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
             }
         };
         int n = 0;
-        if constexpr (kFused) {
+        {
             // neighbours (real, then images) in batches: the batch's row
-            // entries, then its positions and records, are in flight
-            // together; the batch is visited in entry order, so E and the
-            // sums keep the reference order
+            // entries, then its positions (and in FP32 its records), are in
+            // flight together; the batch is visited in entry order, so E and
+            // the sums keep the reference order
             constexpr int BG = kGradBatch;
             for (; n + BG <= cnt; n += BG) {
                 uint32_t eb[BG];
@@ -406,18 +406,18 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                     for (int u = 0; u < BG; ++u) eb[u] = nb_get(a.nbr, N, t, n + u);
                 }
                 double4 pb[BG];
-                T rb[BG][kPrimN];
+                T rb[kFused ? BG : 1][kPrimN];
 #pragma unroll
                 for (int u = 0; u < BG; ++u) {
                     const int j = (int)nb_index(eb[u]);
                     pb[u] = a.pos[j];
-                    load_vec<T, kPrimN>(rb[u], prim + (size_t)j * kPrimN);
+                    if constexpr (kFused) load_vec<T, kPrimN>(rb[u], prim + (size_t)j * kPrimN);
                 }
 #pragma unroll
                 for (int u = 0; u < BG; ++u) {
                     double xj[3];
                     entry_pos<D>(a, eb[u], pb[u], xj);
-                    visit(n + u, (int)nb_index(eb[u]), xj, rb[u]);
+                    visit(n + u, (int)nb_index(eb[u]), xj, rb[kFused ? u : 0]);
                 }
             }
         }
@@ -444,11 +444,10 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                     grad[f][p] = T(g);
                 }
         } else {
-            // pass 2: gradients and neighbour extrema
-            for (int n = 0; n < cnt; ++n) {
-                double xj[3];
-                int s[3];
-                int j = nb_pos<D>(a, nb_get(a.nbr, N, t, n), xj, s);
+            // pass 2: gradients and neighbour extrema, the neighbours in
+            // batches of four (entries, positions and records in flight
+            // together), visited in entry order
+            auto visit2 = [&](const double *xj, const T *prj) {
                 T r = tsqrt(T(dist2<D>(xi, xj)));
                 T w = kernel_w<T>(r, h2, sig) / om_i;
                 T psi[D];
@@ -456,14 +455,39 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                     psi[p] = T(0);
                     for (int q = 0; q < D; ++q) psi[p] += B[p * D + q] * T(xj[q] - xi[q]) * w;
                 }
-                T prj[kPrimN];
-                load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
                 for (int f = 0; f < NF; ++f) {
                     T fj = prj[field_slot<D>(f)];
                     for (int p = 0; p < D; ++p) grad[f][p] += (fj - fi[f]) * psi[p];
                     if (ngbmax[f] < fj) ngbmax[f] = fj;
                     if (ngbmin[f] > fj) ngbmin[f] = fj;
                 }
+            };
+            int n2 = 0;
+            for (; n2 + 4 <= cnt; n2 += 4) {
+                uint32_t eb[4];
+                nb_get4(a.nbr, N, t, n2, eb);
+                double4 pb[4];
+                T rb[4][kPrimN];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = (int)nb_index(eb[u]);
+                    pb[u] = a.pos[j];
+                    load_vec<T, kPrimN>(rb[u], prim + (size_t)j * kPrimN);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    double xj[3];
+                    entry_pos<D>(a, eb[u], pb[u], xj);
+                    visit2(xj, rb[u]);
+                }
+            }
+            for (; n2 < cnt; ++n2) {
+                double xj[3];
+                int s[3];
+                int j = nb_pos<D>(a, nb_get(a.nbr, N, t, n2), xj, s);
+                T prj[kPrimN];
+                load_vec<T, kPrimN>(prj, prim + (size_t)j * kPrimN);
+                visit2(xj, prj);
             }
         }
         // pass 3: slope limiter
