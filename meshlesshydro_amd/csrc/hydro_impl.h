@@ -659,13 +659,14 @@ __device__ inline void face_prepare(const HydroArgs &a, const View<D, T> &L,
         T iL = T(1) / PD_OM(L), iH = T(1) / PD_OM(H);
         T w1 = wk * iL, w2 = wk * iH;
         // effective face A = psi_H(x_L)/omega_L - psi_L(x_H)/omega_H
+        // = (B_L d) w1 / omega_L + (B_H d) w2 / omega_H
         for (int p = 0; p < D; ++p) {
             T p1 = T(0), p2 = T(0);
             for (int q = 0; q < D; ++q) {
-                p1 += PD_B(L, p, q) * d[q] * w1;
-                p2 += PD_B(H, p, q) * (-d[q]) * w2;
+                p1 += PD_B(L, p, q) * d[q];
+                p2 += PD_B(H, p, q) * d[q];
             }
-            A[p] = iL * p1 - iH * p2;
+            A[p] = iL * (p1 * w1) + iH * (p2 * w2);
         }
         const T fl = a.foqp ? T(.5) : T(a.g.h / 4.);
         for (int ax = 0; ax < D; ++ax) {
@@ -876,10 +877,24 @@ __device__ inline int face_finish(const HydroArgs &a, const FacePrep<D, T> &fp, 
     }
     // fluxes projected on A (Riemann.cpp:164-181, 3D :204-227)
     T vLab[D];
+    for (int ax = 0; ax < D; ++ax) vLab[ax] = vS[ax] + vF[ax];
+    if constexpr (sizeof(T) == 4) {
+        // FP32: the same fluxes with the sums over A factored,
+        //   F_m = rho* (A.v*), F_p = vLab F_m + P* A,
+        //   F_E = (A.v*)(P*/(gamma-1) + rho* |vLab|^2 / 2) + P* (A.vLab),
+        // where MFM drops the advective A.v* of momentum and energy
+        const T Av = dotT<D, T>(A, vS);
+        F[0] = rs * Av;
+        const T Avm = a.mfm ? T(0) : Av;
+        const T mf = rs * Avm;
+        for (int c = 0; c < D; ++c) F[2 + c] = vLab[c] * mf + Ps * A[c];
+        const T ekin = dotT<D, T>(vLab, vLab);
+        F[1] = Avm * (Ps / (gam - T(1)) + rs * T(.5) * ekin) + Ps * dotT<D, T>(A, vLab);
+        return flag;
+    }
     T f0 = T(0);
     for (int ax = 0; ax < D; ++ax) f0 += A[ax] * rs * vS[ax];
     F[0] = f0;
-    for (int ax = 0; ax < D; ++ax) vLab[ax] = vS[ax] + vF[ax];
     if (a.mfm)
         for (int ax = 0; ax < D; ++ax) vS[ax] = T(0);
     for (int c = 0; c < D; ++c) {
