@@ -54,10 +54,6 @@ struct mlh_ctx {
     unsigned long long *counters = nullptr;
     int *status = nullptr, *max_counts = nullptr;
     double *sums_block = nullptr;
-    void *defer_rec = nullptr;  // FP32 learned seed: deferred particles
-    int *defer_n = nullptr;
-    unsigned long long *defer_hi = nullptr;  // their masks past entry 127
-    int defer_words = 0;
     MlpState mlp;
     int stage = 0;  // 0 state, 1 neighbours, 2 density, 3 gradients, 4 fluxes
     // per-method stages: 1 gradients unlimited (mlh_psi_gradients), 2 limited
@@ -222,10 +218,6 @@ static HydroArgs hydro_args(mlh_ctx *c) {
     a.counters = c->counters;
     a.status = c->status;
     a.mlp_frags = c->mlp.frags;
-    a.defer_rec = c->defer_rec;
-    a.defer_n = c->defer_n;
-    a.defer_hi = c->defer_hi;
-    a.defer_words = c->defer_words;
     a.collect = c->mlp.collect_stride > 0 ? c->mlp.collect : nullptr;
     a.collect_n = c->mlp.collect_n;
     a.collect_scap = c->mlp.collect_scap;
@@ -246,11 +238,6 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
         return fail(MLH_ERR_INVALID_ARGUMENT, "bad precision");
     if (p->work_order < MLH_ORDER_AUTO || p->work_order > MLH_ORDER_CALLER)
         return fail(MLH_ERR_INVALID_ARGUMENT, "bad work_order");
-    if (p->riemann_seed == MLH_SEED_LEARNED && p->precision == MLH_FP32 &&
-        (int64_t)p->max_interactions + p->max_ghost_interactions > kDeferMaxEntries)
-        return fail(MLH_ERR_INVALID_ARGUMENT,
-                    "FP32 with the learned seed: max_interactions + max_ghost_interactions "
-                    "must not exceed " + std::to_string(kDeferMaxEntries));
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(MLH_ERR_DEVICE, "no HIP device");
@@ -299,16 +286,6 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     A_(&c->status, sizeof(int) * 2);
     A_(&c->max_counts, sizeof(int) * 2);
     A_(&c->sums_block, sizeof(double) * 6 * (size_t)dt_blocks(n));
-    if (p->riemann_seed == MLH_SEED_LEARNED && c->prec == MLH_FP32) {
-        A_(&c->defer_rec, defer_record_bytes() * N);
-        A_(&c->defer_n, sizeof(int));
-        // mask words for entries 128 and up of rows longer than 128
-        c->defer_words = c->stride > 128 ? (c->stride + 63) / 64 - 2 : 0;
-        if (c->defer_words) {
-            A_(&c->defer_hi, sizeof(unsigned long long) * N * (size_t)c->defer_words);
-            hipMemsetAsync(c->defer_hi, 0, sizeof(unsigned long long) * N * c->defer_words, c->s);
-        }
-    }
 #undef A_
     hipMemsetAsync(c->counters, 0, sizeof(unsigned long long) * kNumCounters, c->s);
     hipMemsetAsync(c->status, 0, sizeof(int), c->s);

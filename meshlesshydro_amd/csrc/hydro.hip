@@ -6,7 +6,6 @@
 
 namespace mlh {
 
-size_t defer_record_bytes() { return sizeof(DeferRec); }
 
 // ------------------------------------------------------------------ launchers
 #define MLH_DISPATCH(KERNEL, D, P, GRID, ...)                                             \

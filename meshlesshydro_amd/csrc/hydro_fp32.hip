@@ -27,14 +27,10 @@ void launch_gradients_fp32(const HydroArgs &a, int D, hipStream_t s) {
 void launch_faces_fp32(const HydroArgs &a, int D, hipStream_t s) {
     dim3 grid((a.N + 255) / 256);
     if (a.learned) {
-        // the face loop defers the faces whose standard guess was far off to
-        // k_faces_deferred (a small grid-stride launch over the records)
-        hipLaunchKernelGGL(k_defer_reset, dim3(1), dim3(1), 0, s, a.defer_n);
+        // faces whose standard guess was far off are solved after the row
+        // from the network's seed, inside the same kernel
         if (D == 2) hipLaunchKernelGGL((k_faces<2, float, true>), grid, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_faces<3, float, true>), grid, dim3(256), 0, s, a);
-        dim3 dgrid(grid.x < 1024 ? grid.x : 1024);
-        if (D == 2) hipLaunchKernelGGL((k_faces_deferred<2>), dgrid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_faces_deferred<3>), dgrid, dim3(256), 0, s, a);
     } else {
         if (D == 2) hipLaunchKernelGGL((k_faces<2, float, false>), grid, dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_faces<3, float, false>), grid, dim3(256), 0, s, a);

@@ -1065,20 +1065,6 @@ __device__ inline void face_accumulate(const HydroArgs &a, const FacePrep<D, T> 
     acc.e += sgn * (double)F[1];
 }
 
-// FP32 learned: a particle with faces whose probe failed leaves the face
-// loop with the sums of its other faces in one of these records; bits of
-// d0 / d1 are its deferred neighbour entries 0..127, and hi = 1 when entries
-// from 128 on are deferred too (their bits are in HydroArgs::defer_hi).
-// k_faces_deferred solves the recorded faces with the network's seed, adds
-// them in entry order and finishes the particle.  Whether a face is
-// deferred depends on the face alone (the probe runs on the solving side's
-// operands), so both sides of a face defer it, whatever its entry number.
-struct DeferRec {
-    int t, hi;
-    unsigned long long d0, d1;
-    double m, e, v[3];
-};
-static_assert(sizeof(DeferRec) == 64, "DeferRec is one 64-byte record");
 
 
 // the state of particle i carried over unchanged (a step whose neighbour
@@ -1173,11 +1159,12 @@ __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_faces(HydroArgs a) {
     constexpr int NF = D + 2;
     constexpr bool kFp32 = sizeof(T) == 4;
-    // FP32 learned: faces whose probe fails are deferred to k_faces_deferred,
-    // which seeds them with the network; the FP64 learned kernel runs the
+    // FP32 learned: faces whose probe fails are deferred to the end of the
+    // row and seeded there by the network; the FP64 learned kernel runs the
     // network inside the face loop, over the wavefront's longest row
     constexpr bool kInLoopNet = LEARNED && !kFp32;
-    __shared__ uint16_t lds[kInLoopNet ? 4 : 1][kInLoopNet ? 64 * 16 : 1];
+    constexpr bool kDefer = LEARNED && kFp32;
+    __shared__ uint16_t lds[LEARNED ? 4 : 1][LEARNED ? 64 * 16 : 1];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     const bool live = t < a.N;
     FaceStats fs{};
@@ -1207,7 +1194,7 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
     const View<D, T> own = make_view<D, T>(a, live && kOwnView ? t : 0);
     FluxSums acc{0., 0., {0., 0., 0.}};
     uint64_t defer[2] = {0ull, 0ull};
-    bool defer_hi = false;
+    bool defer_far = false;
     // software pipeline of the face's first two loads: row entries two
     // faces ahead, the neighbour's original index one face ahead, so a face
     // waits only on its records
@@ -1246,187 +1233,76 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
             if (__ballot(strong)) seed = face_net_seed<D, T>(a, strong, fp, lds[threadIdx.x >> 6], fs);
         }
         if (act) {
-            if (LEARNED && kFp32 && mp.valid && !mp.accepted && mp.chg > kSeedGate) {
+            if (kDefer && mp.valid && !mp.accepted && mp.chg > kSeedGate) {
                 // the standard guess was far off: Newton from the network's
-                // seed.  Entries 0..127 are marked in registers, later ones
-                // (rows longer than 128) in this particle's defer_hi words
-                if (n < 128) {
-                    defer[n >> 6] |= 1ull << (n & 63);
-                } else {
-                    a.defer_hi[(size_t)((n >> 6) - 2) * N + t] |= 1ull << (n & 63);
-                    defer_hi = true;
-                }
+                // seed, after the row.  Entries 0..127 are marked in
+                // registers; for later ones (rows longer than 128) a flag
+                // has the row's tail searched again after the row (no
+                // stores in the face loop)
+                if (n < 128) defer[n >> 6] |= 1ull << (n & 63);
+                else defer_far = true;
                 continue;
             }
             face_accumulate<D, T>(a, fp, mp, seed, self_lo, it0, collect, acc, fs);
         }
     }
-    bool finish = live;
-    if constexpr (LEARNED && kFp32) {
-        // a record per particle with deferred faces: one atomic per
-        // wavefront for the slots; the particle is finished by
-        // k_faces_deferred (record order does not matter: each record is
-        // completed on its own)
-        const bool dfr = live && ((defer[0] | defer[1]) != 0ull || defer_hi);
-        const unsigned long long bal = __ballot(dfr);
-        if (bal) {
-            const int lane = lane_id();
-            const int first = __ffsll((long long)bal) - 1;
-            int base = 0;
-            if (lane == first) base = atomicAdd(a.defer_n, __popcll(bal));
-            base = __shfl(base, first, 64);
-            if (dfr) {
-                const int slot = base + __popcll(bal & ((1ull << lane) - 1ull));
-                DeferRec *r = (DeferRec *)a.defer_rec + slot;
-                r->t = t;
-                r->hi = defer_hi ? 1 : 0;
-                r->d0 = defer[0];
-                r->d1 = defer[1];
-                r->m = acc.m;
-                r->e = acc.e;
-                for (int ax = 0; ax < 3; ++ax) r->v[ax] = acc.v[ax];
-                finish = false;
+    if constexpr (kDefer) {
+        // the deferred faces, each lane's in ascending entry order after the
+        // sums of its other faces, one face per lane and round: every lane
+        // takes part in the round's BF16 MFMA inference of the seeds.
+        // Entries 0..127 come from the masks; entries from 128 on (rows
+        // longer than 128) are found again by the same probe.  Whether a
+        // face is deferred depends on the face alone (the probe runs on the
+        // solving side's operands), so both sides of a face defer it and
+        // solve it identically, whatever its entry number.
+        const bool dfr = live && ((defer[0] | defer[1]) != 0ull || defer_far);
+        if (__ballot(dfr)) {
+            int wd = 0, far_n = 128;
+            unsigned long long m = dfr ? defer[0] : 0ull;
+            for (;;) {
+                FacePrep<D, float> fp;
+                MixedProbe mp{};
+                bool self_lo = true, act = false;
+                if (dfr) {
+                    // the probes were counted in the row loop
+                    const FaceStats keep = fs;
+                    for (;;) {
+                        int n;
+                        bool scan = false;
+                        if (m != 0ull) {
+                            n = wd * 64 + __builtin_ctzll(m);
+                            m &= m - 1ull;
+                        } else if (wd == 0) {
+                            wd = 1;
+                            m = defer[1];
+                            continue;
+                        } else if (defer_far && far_n < cnt) {
+                            n = far_n++;
+                            scan = true;
+                        } else {
+                            break;
+                        }
+                        face_of_entry<D, float>(a, t, i, n, dt, fp, mp, self_lo, fs);
+                        if (!scan || (mp.valid && !mp.accepted && mp.chg > kSeedGate)) {
+                            act = true;
+                            break;
+                        }
+                    }
+                    fs = keep;
+                }
+                if (!__ballot(act)) break;
+                const float seed = face_net_seed<D, float>(a, act && mp.valid && !mp.accepted, fp,
+                                                           lds[threadIdx.x >> 6], fs);
+                if (act)
+                    face_accumulate<D, float>(a, fp, mp, seed, self_lo, fs.st.iters, collect, acc,
+                                              fs);
             }
         }
     }
-    if (finish) fs.nonfinite += particle_finish<D>(a, i, acc);
+    if (live) fs.nonfinite += particle_finish<D>(a, i, acc);
     add_counters(a, fs, cnt);
 }
 
-// FP32 learned seed: the deferred faces of the particles recorded by
-// k_faces.  A wavefront takes kDeferGroup records (one per owner lane),
-// lays their deferral masks out in LDS, numbers their deferred faces in
-// owner order and ascending entry (a prefix sum of the owners' counts), and
-// solves them 64 per round with every lane busy: lane e of a round decodes
-// queue position e to (owner, entry) -- the owner from the prefix sums, the
-// entry as the k-th set bit of the owner's mask -- recomputes the face from
-// the solving side, and solves it from the network's seed (one BF16 MFMA
-// inference for the round's faces; FP32 Newton, FP64 acceptance, FP64
-// fallback).  Each owner then adds its faces' fluxes in queue order -- its
-// entries in ascending order, after the sums of its other faces -- and
-// finishes the particle, so the result does not depend on how records are
-// grouped.  Grid-stride over the records, the count read from the device.
-constexpr int kDeferGroup = 16;                     // records per wavefront
-constexpr int kDeferWords = kDeferMaxEntries / 64;  // mask words per owner
-
-template <int D>
-__global__ __launch_bounds__(256) void k_faces_deferred(HydroArgs a) {
-    constexpr int NF = D + 2;
-    __shared__ uint16_t lds[4][64 * 16];
-    __shared__ unsigned long long msk[4][kDeferGroup][kDeferWords];
-    __shared__ int qoff[4][kDeferGroup];
-    __shared__ float res[4][64 * NF];
-    __shared__ int own_t[4][kDeferGroup], own_i[4][kDeferGroup];
-    const int w = threadIdx.x >> 6;
-    const int count = *a.defer_n;
-    const int lane = lane_id();
-    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
-    const float dt = (float)a.dt[0];
-    const int nwords = 2 + a.defer_words;  // <= kDeferWords (checked at create)
-    const DeferRec *recs = (const DeferRec *)a.defer_rec;
-    for (int base = wave * kDeferGroup; base < count; base += nwaves * kDeferGroup) {
-        const int r = base + lane;
-        const bool owner = lane < kDeferGroup && r < count;
-        int i = 0, c = 0;
-        FluxSums acc{0., 0., {0., 0., 0.}};
-        if (owner) {
-            const DeferRec q = recs[r];
-            const int t = q.t;
-            i = a.sidx ? (int)a.sidx[t] : t;
-            acc.m = q.m;
-            acc.e = q.e;
-            for (int ax = 0; ax < 3; ++ax) acc.v[ax] = q.v[ax];
-            msk[w][lane][0] = q.d0;
-            msk[w][lane][1] = q.d1;
-            c = __popcll(q.d0) + __popcll(q.d1);
-            for (int k = 2; k < nwords; ++k) {
-                unsigned long long b = 0ull;
-                if (q.hi) {
-                    // read and cleared: the words are zero again for the
-                    // next step's face loop
-                    unsigned long long *p = a.defer_hi + (size_t)(k - 2) * a.N + t;
-                    b = *p;
-                    *p = 0ull;
-                }
-                msk[w][lane][k] = b;
-                c += __popcll(b);
-            }
-            own_t[w][lane] = t;
-            own_i[w][lane] = i;
-        }
-        // queue positions: exclusive prefix sum of the owners' face counts
-        // (lanes without a record count 0, so their offset is the total)
-        int incl = c;
-        for (int o = 1; o < 64; o <<= 1) {
-            int v = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += v;
-        }
-        const int excl = incl - c;
-        const int total = __shfl(incl, 63, 64);
-        if (lane < kDeferGroup) qoff[w][lane] = excl;
-        wave_lds_sync();
-        FaceStats fs{};
-        for (int b = 0; b < total; b += 64) {
-            const int e = b + lane;
-            const bool act = e < total;
-            FacePrep<D, float> fp;
-            MixedProbe mp{};
-            bool self_lo = true;
-            int o = 0;
-            if (act) {
-                // owner: the last one whose offset is <= e (it has a face
-                // at position e, since the next offset is > e)
-                while (o + 1 < kDeferGroup && qoff[w][o + 1] <= e) ++o;
-                // entry: the (e - offset)-th set bit of the owner's mask
-                int k = e - qoff[w][o];
-                int wd = 0;
-                unsigned long long m = msk[w][o][0];
-                for (int pc = __popcll(m); k >= pc; pc = __popcll(m)) {
-                    k -= pc;
-                    m = msk[w][o][++wd];
-                }
-                for (; k > 0; --k) m &= m - 1ull;
-                const int n = wd * 64 + __builtin_ctzll(m);
-                // the probe was counted by k_faces
-                const FaceStats keep = fs;
-                face_of_entry<D, float>(a, own_t[w][o], own_i[w][o], n, dt, fp, mp, self_lo, fs);
-                fs = keep;
-            }
-            float seed = face_net_seed<D, float>(a, act && mp.valid && !mp.accepted, fp, lds[w], fs);
-            if (act) {
-                float F[NF];
-                const int it0 = fs.st.iters;
-                face_finish<D, float>(a, fp, seed, F, fs, &mp);
-                // training data of the owner's particle (every collect_stride-th)
-                if (a.collect)
-                    collect_faces<D, float>(a, self_lo && fs.st.iters - it0 >= 2 &&
-                                                   own_i[w][o] % a.collect_stride == 0,
-                                            fp);
-                const float sg = self_lo ? 1.f : -1.f;
-                for (int c2 = 0; c2 < NF; ++c2) res[w][lane * NF + c2] = sg * F[c2];
-            }
-            wave_lds_sync();
-            const int q0 = excl > b ? excl : b;
-            const int q1 = excl + c < b + 64 ? excl + c : b + 64;
-            for (int q = q0; q < q1; ++q) {
-                const float *rr = &res[w][(q - b) * NF];
-                acc.m += (double)rr[0];
-                for (int ax = 0; ax < D; ++ax) acc.v[ax] += (double)rr[2 + ax];
-                acc.e += (double)rr[1];
-            }
-            wave_lds_sync();
-        }
-        if (owner) fs.nonfinite += particle_finish<D>(a, i, acc);
-        add_counters(a, fs, 0);
-        // the next group's masks and offsets overwrite this group's
-        wave_lds_sync();
-    }
-}
-
-#ifdef MLH_FP32_TU
-__global__ void k_defer_reset(int *n) { *n = 0; }
-#endif
 
 // conserved sums (Particles.cpp:2830-2886): V, M, E, px, py, pz per block
 template <int D, typename T>

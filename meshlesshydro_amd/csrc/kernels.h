@@ -62,16 +62,6 @@ struct HydroArgs {
     // is, so a run stops where the reference exits (Particles.cpp:351,2252).
     int *status;
     const struct MlpFrags *mlp_frags;  // learned seed network (device)
-    // FP32 learned seed: particles whose faces wait for the network
-    // (DeferRec[N], written by k_faces, consumed by k_faces_deferred) and
-    // their count
-    void *defer_rec;
-    int *defer_n;
-    // entries 128.. of the deferred faces' bit masks: [defer_words][N] words
-    // by work position (entries 0..127 travel in the record); zero between
-    // steps (k_faces_deferred clears what it reads)
-    unsigned long long *defer_hi;
-    int defer_words;
     double *collect;             // face collection [stripes][scap][6] (or null)
     unsigned long long *collect_n;  // [stripes] fill counters
     long long collect_scap;      // rows per stripe
@@ -101,9 +91,6 @@ enum StageOp {
 
 // error bits of HydroArgs::status[0] (the reference's exit codes 1 and 3)
 constexpr int kStatusErrMask = (1 << 1) | (1 << 3);
-// longest neighbour row (maxr + maxg) the FP32 learned seed's deferral masks
-// cover
-constexpr int kDeferMaxEntries = 2048;
 
 enum CounterIdx {
     kCntNegReal = 0, kCntNegGhost, kCntVacuum, kCntNonconv, kCntMissing,
@@ -145,7 +132,6 @@ void launch_density_fp32(const HydroArgs &a, int D, hipStream_t s);
 void launch_gradients_fp32(const HydroArgs &a, int D, hipStream_t s);
 void launch_faces_fp32(const HydroArgs &a, int D, hipStream_t s);
 // bytes of one deferred-particle record (FP32 learned seed)
-size_t defer_record_bytes();
 void launch_sums(const HydroArgs &a, int D, int precision, double *out, hipStream_t s);
 void launch_stage(const HydroArgs &a, int D, int precision, int op, int what,
                   const FaceArrays &fa, const double *f, double *grad, hipStream_t s);
