@@ -1238,10 +1238,12 @@ __attribute__((amdgpu_waves_per_eu(MLH_FACES_WAVES(D, T, LEARNED)))) void k_face
                 // seed, after the row.  Entries 0..127 are marked in
                 // registers; for later ones (rows longer than 128) a flag
                 // has the row's tail searched again after the row (no
-                // stores in the face loop)
+                // stores in the face loop).  Here the face goes on with
+                // A = 0, for which face_finish returns zero flux at once:
+                // the loop keeps one path for every face
                 if (n < 128) defer[n >> 6] |= 1ull << (n & 63);
                 else defer_far = true;
-                continue;
+                for (int ax = 0; ax < D; ++ax) fp.A[ax] = T(0);
             }
             face_accumulate<D, T>(a, fp, mp, seed, self_lo, it0, collect, acc, fs);
         }
