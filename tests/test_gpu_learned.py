@@ -8,12 +8,13 @@ which solves every face from the standard guess:
 
 * FP64 with the seed (k_faces<D, double, true>: the network runs inside the
   face loop for faces with a real jump) within the FP64 bars;
-* FP32 with the seed (k_faces<D, float, true> + k_faces_deferred: faces whose
-  probe at the standard guess fails are deferred and solved from the
-  network's seed) within the FP32 bars;
+* FP32 with the seed (k_faces<D, float, true>: faces whose probe at the
+  standard guess fails are deferred to the end of the row and solved from
+  the network's seed) within the FP32 bars;
 * with rows longer than 128 entries, where deferred faces sit at entries
-  past the register masks, conservation to round-off (both sides of a face
-  must defer it and solve it identically).
+  past the register masks (found again by the probe after the row),
+  conservation to round-off (both sides of a face must defer it and solve
+  it identically).
 
 The network is trained on the blast's own FP64 faces, and every test asserts
 that faces were actually seeded.
@@ -79,9 +80,9 @@ def test_fp64_learned_steps_vs_oracle(blast, hip, oracle_mod):
 
 
 def test_fp32_learned_fluxes_vs_oracle(blast, hip, oracle_mod):
-    """k_faces<3, float, true> + k_faces_deferred through the stage entry
-    points: one step's flux sums against the oracle's within the FP32 bars,
-    with deferred faces seeded by the network.  The step is the first one of
+    """k_faces<3, float, true> through the stage entry points: one step's
+    flux sums against the oracle's within the FP32 bars, with deferred faces
+    seeded by the network.  The step is the first one of
     the blast whose faces get seeded (at the first step every face's
     standard guess passes the probe); oracle and GPU start from the same
     state."""
