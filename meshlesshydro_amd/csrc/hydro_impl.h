@@ -378,7 +378,7 @@ __global__ __launch_bounds__(256) void k_gradients(HydroArgs a) {
                 double vs = ci + cj;
                 if constexpr (kFused) {
                     // FP32 mode: 1/|x_ij| as a refined reciprocal square
-                    // root (FP64 rounding, no divide) for every pair, then
+                    // root (error ~1e-14, no divide) for every pair, then
                     // the same selection, without a divergent branch
                     const double vx = dv * rsqrt_nodiv(d2);
                     vs = dv < 0. ? vs - (vx < 0. ? vx : 0.) : vs;

@@ -103,38 +103,37 @@ __device__ inline void prefun(const GasConst<T> &c, T p, T rk, T pk, T ck,
     }
 }
 
-// x^(1/5) in FP64 without a division: the FP32 estimate of r = x^(-1/5)
-// refined by two Newton steps r <- r (6 - x r^5) / 5 (error ~1e-7 ->
-// ~1e-14 -> round-off), then x^(1/5) = x r^4.
+// The FP32 mode's FP64 helpers below start from the FP32 hardware estimate
+// (relative error e ~ 1e-7) and take one FP64 Newton step, which squares
+// the error: ~1e-14 for the reciprocal and its square root, ~1e-12 for the
+// fifth root.  The FP64 residual needs to be far more accurate than the
+// solver's relative-change test (1e-6) and than the FP32 p* it yields
+// (6e-8), not rounding-exact.  Outside the FP32 normal range the exact
+// operations.
+
+// x^(1/5) without a division: the estimate of r = x^(-1/5) refined by
+// r <- r (6 - x r^5) / 5 (error 3 e^2), then x^(1/5) = x r^4.
 __device__ inline double root5_nodiv(double x) {
     if (!(x > 1e-30) || !(x < 1e30)) return pow(x, 0.2);
     double r = (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)x));
-    for (int k = 0; k < 2; ++k) {
-        double r2 = r * r;
-        double r5 = r2 * r2 * r;
-        r = r * (1.2 - 0.2 * (x * r5));
-    }
     double r2 = r * r;
+    r = r * (1.2 - 0.2 * (x * (r2 * r2 * r)));
+    r2 = r * r;
     return x * (r2 * r2);
 }
 
-// FP64 reciprocal and reciprocal square root without the divide / square
-// root expansions: the FP32 hardware estimate (~1 ulp of FP32) and two FP64
-// Newton steps (error ~(1e-7)^4, i.e. rounding-limited).  Outside the FP32
-// normal range the exact operations.
+// 1 / x without the divide expansion: r <- r (2 - x r) (error e^2)
 __device__ inline double rcp_nodiv(double x) {
     if (!(fabs(x) > 1e-30) || !(fabs(x) < 1e30)) return 1.0 / x;
     double r = (double)__builtin_amdgcn_rcpf((float)x);
-    r = r * (2.0 - x * r);
-    r = r * (2.0 - x * r);
-    return r;
+    return r * (2.0 - x * r);
 }
+// 1 / sqrt(x) without the square root: r <- r (3 - x r^2) / 2 (error
+// 3 e^2 / 2)
 __device__ inline double rsqrt_nodiv(double x) {
     if (!(x > 1e-30) || !(x < 1e30)) return 1.0 / sqrt(x);
     double r = (double)__builtin_amdgcn_rsqf((float)x);
-    r = r * (1.5 - 0.5 * x * (r * r));
-    r = r * (1.5 - 0.5 * x * (r * r));
-    return r;
+    return r * (1.5 - 0.5 * x * (r * r));
 }
 
 // Residual of the exact solver for the FP32 mode's acceptance steps: f_K(p)
