@@ -128,6 +128,10 @@ typedef struct mlh_diag {
                                        * reference, Helper.cpp:13-14)         */
     int64_t nonfinite_particles;      /* particles whose new state is not     *
                                        * finite after an update               */
+    int64_t acoustic_faces;           /* FP32 mode: near-uniform faces given  *
+                                       * the acoustic (linearised) solution,  *
+                                       * no Newton step (relative jumps below *
+                                       * 1e-5, |u*| <= c/2; error ~4e-11)     */
 } mlh_diag;
 
 typedef struct mlh_ctx mlh_ctx;

@@ -98,6 +98,7 @@ class Diag(C.Structure):
         ("max_ghost_neighbours", C.c_int32),
         ("singular_matrices", C.c_int64),
         ("nonfinite_particles", C.c_int64),
+        ("acoustic_faces", C.c_int64),
     ]
 
     def as_dict(self):

@@ -968,6 +968,7 @@ int mlh_get_diagnostics(mlh_ctx *c, mlh_diag *d) {
     d->learned_seeded = (int64_t)k[kCntSeeded];
     d->singular_matrices = (int64_t)k[kCntSingular];
     d->nonfinite_particles = (int64_t)k[kCntNonfinite];
+    d->acoustic_faces = (int64_t)k[kCntAcoustic];
     d->max_neighbours = mc[0];
     d->max_ghost_neighbours = mc[1];
     return MLH_OK;

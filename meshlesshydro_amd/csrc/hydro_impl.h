@@ -1137,7 +1137,7 @@ __device__ inline void add_counters(const HydroArgs &a, const FaceStats fs, int 
         (unsigned long long)fs.vacuum, (unsigned long long)fs.st.nonconv, 0ull,
         (unsigned long long)fs.st.fallback, (unsigned long long)fs.st.iters,
         (unsigned long long)faces, (unsigned long long)fs.seeded, 0ull,
-        (unsigned long long)fs.nonfinite};
+        (unsigned long long)fs.nonfinite, (unsigned long long)fs.st.acoustic};
     for (int k = 0; k < kNumCounters; ++k) {
         unsigned long long v = wave_sum(c[k]);
         if (lane_id() == 0 && v) atomicAdd(&a.counters[k], v);

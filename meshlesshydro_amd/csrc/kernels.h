@@ -95,6 +95,7 @@ constexpr int kStatusErrMask = (1 << 1) | (1 << 3);
 enum CounterIdx {
     kCntNegReal = 0, kCntNegGhost, kCntVacuum, kCntNonconv, kCntMissing,
     kCntFallback, kCntIters, kCntFaces, kCntSeeded, kCntSingular, kCntNonfinite,
+    kCntAcoustic,
     kNumCounters
 };
 

@@ -11,7 +11,8 @@
 //
 // Precision variants:
 //   solve_fp64   every operation in FP64 (the plain exact solver);
-//   solve_mixed  mixed_probe then mixed_finish (below): an FP64 Newton step
+//   solve_mixed  mixed_probe then mixed_finish (below): the acoustic
+//                solution for near-uniform faces, else an FP64 Newton step
 //                at the seed or the standard guess, FP32 Newton only when
 //                that step fails the exact solver's relative-change test,
 //                an FP64 acceptance step, solve_fp64 as the fallback.
@@ -299,6 +300,7 @@ struct SolveStats {
     int iters;      // Newton iterations (FP32 + FP64)
     int fallback;   // learned / FP32 start rejected by the FP64 test
     int nonconv;    // FP64 iteration cap reached
+    int acoustic;   // FP32: near-uniform faces given the acoustic solution
 };
 
 // Plain FP64 exact solver from the standard guess (or from p0 > 0).
@@ -390,7 +392,21 @@ struct MixedProbe {
     double pn, um;    // FP64 root and contact velocity once accepted
     bool valid;       // positive densities and no vacuum generated
     bool accepted;    // an FP64 step passed the change test
+    bool acoustic;    // near-uniform face: pn, um from the acoustic solution
 };
+
+// Largest relative jump |drho|/rho, |dP|/P, |du|/c of a face that the FP32
+// mode solves by the acoustic (linearised) solution, and the largest
+// |u*| / c there.  The acoustic solution
+//   P* = P + (u_L - u_R) Z / 2,  u* = u + (P_L - P_R) / (2 Z),
+//   rho* = rho_K (1 + (P* - P_K) / (gamma P_K))  on the contact's upwind
+// side K (L when u* >= 0), with the averages P, u, rho, c and Z = rho c
+// (Z is the PVRS guess's impedance), differs from the exact solution by
+// about 0.4 delta^2 relative (4e-11 at delta = 1e-5: far below FP32
+// rounding).  |u*| <= c / 2 keeps x/t = 0 between the two acoustic waves
+// (a transonic rarefaction fan is not linear there).
+constexpr float kAcousticJump = 1e-5f;
+constexpr float kAcousticMach = 0.5f;
 
 __device__ inline MixedProbe mixed_probe(const GasConst<float> &cf, const GasConst<double> &cd,
                                          float dl, float ul, float pl, float dr, float ur,
@@ -401,6 +417,26 @@ __device__ inline MixedProbe mixed_probe(const GasConst<float> &cf, const GasCon
     m.cr = ok ? __builtin_sqrtf(cf.g * pr / dr) : 0.f;
     m.valid = ok && cf.g4 * (m.cl + m.cr) > ur - ul;
     if (!m.valid) return m;
+    if (!(p_start > 0.f) && pl > 0.f && pr > 0.f) {
+        // near-uniform face (a smooth or quiescent region): the acoustic
+        // solution, no Newton step
+        const float db = 0.5f * (dl + dr), pb = 0.5f * (pl + pr), cb = 0.5f * (m.cl + m.cr);
+        if (__builtin_fabsf(dl - dr) <= kAcousticJump * db &&
+            __builtin_fabsf(pl - pr) <= kAcousticJump * pb &&
+            __builtin_fabsf(ul - ur) <= kAcousticJump * cb) {
+            const float z = db * cb;
+            const float ps = pb + 0.5f * (ul - ur) * z;
+            const float us = 0.5f * (ul + ur) + 0.5f * (pl - pr) / z;
+            if (__builtin_fabsf(us) <= kAcousticMach * cb) {
+                m.accepted = m.acoustic = true;
+                m.p = ps;
+                m.pn = ps;
+                m.um = us;
+                st.acoustic += 1;
+                return m;
+            }
+        }
+    }
     float tiny = 1e-6f * 0.5f * (pl + pr);
     float p = p_start > 0.f ? p_start : guess_p(cf, dl, ul, pl, m.cl, dr, ur, pr, m.cr, tiny);
     const double dld = dl, pld = pl, drd = dr, prd = pr, dud = (double)ur - (double)ul;
@@ -423,6 +459,13 @@ __device__ inline RSol<float> mixed_finish(const GasConst<float> &cf, const GasC
                                            float dl, float ul, float pl, float dr, float ur,
                                            float pr, double tol, int max_iter, int iters32,
                                            float restart, const MixedProbe &m, SolveStats &st) {
+    if (m.acoustic) {
+        // the star state on the contact's upwind side (x/t = 0 lies between
+        // the acoustic waves)
+        const bool left = 0. <= m.um;
+        const float dk = left ? dl : dr, pk = left ? pl : pr, ps = (float)m.pn;
+        return RSol<float>{dk + dk * ((ps - pk) / (cf.g * pk)), (float)m.um, ps, left ? -1 : 1};
+    }
     bool ok = m.valid && m.accepted;
     double pn = m.pn, um = m.um;
     if (m.valid && !m.accepted) {

@@ -88,6 +88,52 @@ def test_random_states_vs_oracle(hip, oracle_mod):
     assert np.quantile(rel, 0.999) < 1e-5, np.quantile(rel, 0.999)
 
 
+def test_acoustic_faces_vs_oracle(hip, oracle_mod):
+    """FP32 mode: near-uniform faces (relative jumps of rho, P and u / c at
+    most 1e-5, |u*| <= c / 2) take the acoustic solution without a Newton
+    step (0 iterations).  It matches the exact solution of the same FP32
+    inputs (the oracle at tol 1e-12) to FP32 rounding; the linearisation's
+    own error, about 0.4 delta^2, is below 4e-11 there.  Larger jumps and
+    transonic faces keep the Newton path."""
+    g = 5.0 / 3.0
+    rng = np.random.default_rng(7)
+    n = 50_000
+    rl = 10 ** rng.uniform(-2, 2, n)
+    pl = 10 ** rng.uniform(-4, 2, n)
+    cl = np.sqrt(g * pl / rl)
+    ul = rng.uniform(-0.4, 0.4, n) * cl
+    delta = 10 ** rng.uniform(-12, -5.2, n)
+    e = rng.uniform(-1, 1, (3, n)) * delta
+    st = [rl, ul, pl, rl * (1 + e[0]), ul + e[2] * cl, pl * (1 + e[1])]
+    st = [a.astype(np.float32).astype(np.float64) for a in st]   # what FP32 solves
+    o = oracle_mod.riemann_batch(g, *st, tol=1e-12, max_iter=100)
+    gm = hip.riemann_batch(g, *st, precision=MLH_FP32, tol=1e-6)
+    assert np.all(gm[4] == 0), np.unique(gm[4])
+    cs = 0.5 * (np.sqrt(g * st[2] / st[0]) + np.sqrt(g * st[5] / st[3]))
+    # the sampled side follows the sign of u*, which may differ only where
+    # u* is at rounding level
+    assert np.all((gm[3] == o[3]) | (np.abs(o[1]) < 1e-6 * cs))
+    for a, b, scale in ((gm[0], o[0], o[0]), (gm[1], o[1], np.abs(o[1]) + cs), (gm[2], o[2], o[2])):
+        rel = np.abs(a - b) / scale
+        assert rel.max() < 2.5e-7, rel.max()
+    # not near-uniform, or transonic: Newton as before
+    big = [rl, ul, pl, rl * (1 + 1e-3), ul, pl]
+    fast = [rl, 0.9 * cl, pl, rl, 0.9 * cl, pl * (1 + 1e-7)]
+    for case in (big, fast):
+        gb = hip.riemann_batch(g, *case, precision=MLH_FP32, tol=1e-6)
+        assert np.all(gb[4] >= 1)
+    # the FP32 step counts its acoustic faces (the blast's background)
+    from cases import blast3d
+    from meshlesshydro_amd._abi import MLH_FP32 as P32
+    p32, s = blast3d(16, precision=P32)
+    sim = hip.Simulation(p32, s)
+    sim.step()
+    d = sim.diag()
+    sim.close()
+    print(f"\n[acoustic] {d['acoustic_faces']} of {d['faces_solved']} faces", flush=True)
+    assert 0.5 * d["faces_solved"] < d["acoustic_faces"] < d["faces_solved"]
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_face_flux_batch_vs_oracle(hip, oracle_mod, dim):
     """mlh_riemann_flux_batch (the device face solve + flux projection of the
