@@ -199,3 +199,39 @@ def test_near_vacuum_newton_fallback(oracle_mod):
             lo, hi = (mid, hi) if F(mid) < 0.0 else (lo, mid)
         p_ref = 0.5 * (lo + hi)
         assert abs(pstar[k] - p_ref) <= 2e-6 * p_ref, (k, pstar[k], p_ref, its[k])
+
+
+def test_acoustic_solution_error_is_second_order(oracle_mod):
+    """The FP32 mode's acoustic solution for near-uniform faces (riemann.h:
+    P* = P + (u_L - u_R) Z / 2, u* = u + (P_L - P_R) / (2 Z), Z = rho c,
+    rho* = rho_K (1 + (P* - P_K) / (gamma P_K)) on the contact's upwind
+    side, for |u*| <= c / 2) against the oracle's exact solver: the error
+    shrinks as delta^2 (a wrong coefficient or sign in any term would leave
+    it first order), and stays below 0.5 delta^2 in P*, u*/c and rho*."""
+    g = 5.0 / 3.0
+    rng = np.random.default_rng(3)
+    n = 20_000
+    worst = []
+    for delta in (1e-3, 1e-4, 1e-5):
+        rl = 10 ** rng.uniform(-2, 2, n)
+        pl = 10 ** rng.uniform(-4, 2, n)
+        cl = np.sqrt(g * pl / rl)
+        ul = rng.uniform(-0.4, 0.4, n) * cl
+        e = rng.uniform(-1, 1, (3, n)) * delta
+        rr, pr, ur = rl * (1 + e[0]), pl * (1 + e[1]), ul + e[2] * cl
+        rs, us, ps, fl, _ = oracle_mod.riemann_batch(g, rl, ul, pl, rr, ur, pr, tol=1e-14, max_iter=100)
+        cr = np.sqrt(g * pr / rr)
+        db, pb, cb = 0.5 * (rl + rr), 0.5 * (pl + pr), 0.5 * (cl + cr)
+        z = db * cb
+        pa = pb + 0.5 * (ul - ur) * z
+        ua = 0.5 * (ul + ur) + 0.5 * (pl - pr) / z
+        left = ua >= 0
+        dk, pk = np.where(left, rl, rr), np.where(left, pl, pr)
+        ra = dk * (1 + (pa - pk) / (g * pk))
+        assert np.all(np.abs(ua) <= 0.5 * cb)
+        assert np.array_equal(np.where(left, -1, 1), fl)
+        err = max(np.max(np.abs(pa - ps) / ps), np.max(np.abs(ua - us) / cb), np.max(np.abs(ra - rs) / rs))
+        assert err < 0.5 * delta ** 2, (delta, err)
+        worst.append(err)
+    # second order: each tenth of delta takes the error down a hundredfold
+    assert worst[1] < 0.03 * worst[0] and worst[2] < 0.03 * worst[1], worst
