@@ -524,7 +524,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this MFV step has no multi-GPU path "
-              "(sharding is described in DESIGN.md); use --gpus 1.", file=sys.stderr)
+              "(it would shard by space, one slab of cells per GPU); use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.calib_copy:
