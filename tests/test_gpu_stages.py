@@ -159,6 +159,23 @@ def test_flux_symmetry_check(hip, oracle_mod, case):
         sim.close()
 
 
+def test_flux_symmetry_fp32_acoustic(hip):
+    """FP32 mode on the blast, where most faces take the acoustic solution
+    and the centre's faces take Newton: every face's flux is the exact
+    negative of its mirror's (F_ji = -F_ij bitwise), none missing."""
+    p, s = CASES["blast16"](precision=MLH_FP32)
+    sim = hip.Simulation(p, s)
+    sim.neighbours(); sim.density(); dt = sim.timestep(); sim.gradients()
+    sim.effective_faces(); sim.riemann_states(dt); sim.solve_faces()
+    got = sim.check_flux_symmetry(0.0)
+    d = sim.diag()
+    sim.close()
+    print(f"\n[stages] FP32 flux symmetry: {got}, acoustic {d['acoustic_faces']} of "
+          f"{d['faces_solved']} faces, {d['newton_iterations']} Newton iterations", flush=True)
+    assert got == (0, 0)
+    assert 0 < d["acoustic_faces"] < d["faces_solved"] and d["newton_iterations"] > 0
+
+
 def test_mirror_check_and_debug_counters(hip):
     """At DEBUG_LVL 1 the fused step checks that every entry's partner lists
     the particle (none missing here, as in the oracle) and counts singular
