@@ -230,10 +230,14 @@ __device__ inline void slope_limit(const HydroArgs &a, int t, int cnt, const dou
 #pragma clang fp contract(off)
     constexpr int NF = D + 2;
     const int N = a.N;
-    double midmax[NF], midmin[NF];
+    // The extrema of f_ij = f_i + grad f . (x_ij - x_i) over the neighbours
+    // are those of the dot products, f_i added once after the loop: rounding
+    // is monotone, so max_j fl(f_i + d_j) = fl(f_i + max_j d_j) exactly (and
+    // NaN products are skipped either way)
+    double dmax[NF], dmin[NF];
     for (int f = 0; f < NF; ++f) {
-        midmax[f] = tmin_pos<T>();
-        midmin[f] = tmax_val<T>();
+        dmax[f] = -HUGE_VAL;
+        dmin[f] = HUGE_VAL;
     }
     double gd[NF][D];
     for (int f = 0; f < NF; ++f)
@@ -246,9 +250,9 @@ __device__ inline void slope_limit(const HydroArgs &a, int t, int cnt, const dou
             xijxi[ax] = xij - xi[ax];
         }
         for (int f = 0; f < NF; ++f) {
-            double fij = (double)fi[f] + dotT<D, double>(gd[f], xijxi);
-            if (midmax[f] < fij) midmax[f] = fij;
-            if (midmin[f] > fij) midmin[f] = fij;
+            const double dj = dotT<D, double>(gd[f], xijxi);
+            if (dmax[f] < dj) dmax[f] = dj;
+            if (dmin[f] > dj) dmin[f] = dj;
         }
     };
     // neighbours in batches of four: the four row entries (one 16-byte
@@ -273,6 +277,16 @@ __device__ inline void slope_limit(const HydroArgs &a, int t, int cnt, const dou
         int s[3];
         nb_pos<D>(a, nb_get(a.nbr, N, t, n), xj, s);
         visit(xj);
+    }
+    // midmax / midmin start from the smallest positive and the largest
+    // value of T, as in the reference
+    double midmax[NF], midmin[NF];
+    for (int f = 0; f < NF; ++f) {
+        midmax[f] = tmin_pos<T>();
+        midmin[f] = tmax_val<T>();
+        const double hi = (double)fi[f] + dmax[f], lo = (double)fi[f] + dmin[f];
+        if (midmax[f] < hi) midmax[f] = hi;
+        if (midmin[f] > lo) midmin[f] = lo;
     }
     const double beta = a.beta;
     for (int f = 0; f < NF; ++f) {
