@@ -41,6 +41,7 @@ struct mlh_ctx {
     int order = 0;             // resolved work order (MLH_ORDER_CELL / _CALLER), 0 = not yet
     uint32_t *cell_key = nullptr, *sorted_idx = nullptr, *sorted_key = nullptr,
              *sort_scratch = nullptr;
+    uint32_t *inv_idx = nullptr;  // original index -> cell-order position
     int *cstart = nullptr, *cend = nullptr;
     int cell_cap = 0;                // cells cstart / cend can hold
     double *lim_scratch = nullptr;   // domain limits (non-periodic)
@@ -265,6 +266,7 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
     A_(&c->pos, sizeof(double4) * N);
     A_(&c->pos_w, sizeof(double4) * N);
     A_(&c->cell_key, 4 * N); A_(&c->sorted_idx, 4 * N); A_(&c->sorted_key, 4 * N);
+    A_(&c->inv_idx, 4 * N);
     A_(&c->sort_scratch, 4 * (2 * N + radix_sort_scratch_words(n)));
     if (p->periodic) {
         A_(&c->cstart, 4 * (size_t)c->g.nkeys); A_(&c->cend, 4 * (size_t)c->g.nkeys);
@@ -401,7 +403,7 @@ static int enqueue_neighbours(mlh_ctx *c, bool fuse_density = false) {
     }
     launch_cells(c->g, c->D, c->x[0], c->x[1], c->D == 3 ? c->x[2] : c->x[0], c->N, c->pos,
                  c->cell_key, c->sorted_idx, c->sorted_key, c->sort_scratch, c->cstart, c->cend,
-                 c->s);
+                 c->inv_idx, c->s);
     if (c->order == 0) resolve_order(c);
     const bool co = c->order == MLH_ORDER_CALLER;
     if (co)
@@ -418,6 +420,7 @@ static int enqueue_neighbours(mlh_ctx *c, bool fuse_density = false) {
     a.caller_order = co ? 1 : 0;
     a.cstart = c->cstart;
     a.cend = c->cend;
+    a.inv = c->inv_idx;
     a.nbr = c->nbr;
     a.noi = c->noi;
     a.nog = c->nog;

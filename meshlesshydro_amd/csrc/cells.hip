@@ -65,14 +65,17 @@ __global__ void k_assign_cells(Geom g, const double *__restrict__ x0,
     idx[i] = (uint32_t)i;
 }
 
-// positions in cell order; w = original index (exact in a double)
+// positions in cell order; w = original index (exact in a double); inv:
+// original index -> cell-order position
 __global__ void k_gather_pos(const double *__restrict__ x0, const double *__restrict__ x1,
                              const double *__restrict__ x2, int D, int N,
-                             const uint32_t *__restrict__ sidx, double4 *__restrict__ pos) {
+                             const uint32_t *__restrict__ sidx, double4 *__restrict__ pos,
+                             uint32_t *__restrict__ inv) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     uint32_t i = sidx[t];
     pos[t] = make_double4(x0[i], x1[i], D == 3 ? x2[i] : 0.0, (double)i);
+    inv[i] = (uint32_t)t;
 }
 
 // ---------------------------------------------------------------- radix sort
@@ -306,14 +309,27 @@ struct GlobalCand {
     }
 };
 
+// Entries a thread's image search sorts in its LDS stage before they go to
+// the row (more images than this continue with the row's own insertion)
+constexpr int kImgStage = 24;
+// stage slots per thread: the real hits' two 8-entry chunks, or the image
+// entries being sorted
+constexpr int kStageSlots = kImgStage > 16 ? kImgStage : 16;
+
 // Periodic images of work position t (position xi, cell cc) in the
 // reference's ghost order, stored after its cnt real entries: only
 // particles whose cell lies within the image window of a domain face
 // search.  Returns the image count (an overflow sets the status bit and
 // keeps the entries found so far).
+//  * The first kImgStage hits are insertion-sorted in the thread's LDS
+//    stage (free once the real hits are stored), holding the parent's
+//    original index, so the ghost-order key of an entry needs no load; they
+//    go to the row as work positions (a.inv) when the search ends.
+//  * Further hits (more images than the stage holds) are insertion-sorted
+//    in the row itself, each key read back through the entry's position.
 template <int D, bool kCaller>
 __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, const int *cc,
-                                   int cnt) {
+                                   int cnt, uint32_t *stage) {
     const Geom &g = a.g;
     const int N = a.N;
     int gcnt = 0, err = 0;
@@ -323,6 +339,11 @@ __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, co
     if (g.periodic && near_edge) {
         uint32_t *nb = a.nbr;
         const int g0 = cnt;  // images follow the real entries
+        // staged entry (original index | code) -> row entry
+        auto to_row = [&](uint32_t e) {
+            return kCaller ? e : (a.inv[nb_index(e)] | (nb_code(e) << kIdxBits));
+        };
+        bool in_row = false;  // the stage overflowed: entries are in the row
         for (int vz = (D == 3 ? cc[2] - W : 0); vz <= (D == 3 ? cc[2] + W : 0) && !err; ++vz)
             for (int vy = cc[1] - W; vy <= cc[1] + W && !err; ++vy)
                 for (int vx = cc[0] - W; vx <= cc[0] + W && !err; ++vx) {
@@ -355,6 +376,22 @@ __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, co
                         if (gcnt >= a.maxg) { err = 1; break; }
                         const uint32_t jo = (uint32_t)pj.w;
                         uint64_t ke = ghost_key(jo, code);
+                        if (!in_row && gcnt < kImgStage) {
+                            int p = gcnt;
+                            while (p > 0) {
+                                const uint32_t prev = stage[p - 1];
+                                if (ghost_key(nb_index(prev), nb_code(prev)) <= ke) break;
+                                stage[p] = prev;
+                                --p;
+                            }
+                            stage[p] = jo | (code << kIdxBits);
+                            ++gcnt;
+                            continue;
+                        }
+                        if (!in_row) {
+                            for (int k = 0; k < gcnt; ++k) nb[nb_slot(N, t, g0 + k)] = to_row(stage[k]);
+                            in_row = true;
+                        }
                         int p = gcnt;
                         while (p > 0) {
                             uint32_t prev = nb[nb_slot(N, t, g0 + p - 1)];
@@ -367,6 +404,8 @@ __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, co
                         ++gcnt;
                     }
                 }
+        if (!in_row)
+            for (int k = 0; k < gcnt; ++k) nb[nb_slot(N, t, g0 + k)] = to_row(stage[k]);
         if (err) atomicOr(a.status, 1 << 3);
     }
     return gcnt;
@@ -472,7 +511,7 @@ __device__ inline void search_one(const NbrArgs &a, int t, const Src &src, uint3
         atomicOr(a.status, 1 << 1);
     }
     a.noi[t] = cnt;
-    if (!err) gcnt = image_search<D, kCaller>(a, t, xi, cc, cnt);
+    if (!err) gcnt = image_search<D, kCaller>(a, t, xi, cc, cnt, stage);
     a.nog[t] = gcnt;
     if constexpr (kDens != 0) {
         // compDensity (Particles.cpp:1151-1184): the self term after the
@@ -515,12 +554,12 @@ __device__ inline void reduce_max_counts(const NbrArgs &a, int cnt, int gcnt) {
 // position, candidates from global memory: caller order and linear keys.
 template <int D, bool kCaller, int kDens>
 __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
-    __shared__ __align__(16) uint32_t s_stage[256 * 16];
+    __shared__ __align__(16) uint32_t s_stage[256 * kStageSlots];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     int cnt = 0, gcnt = 0;
     if (t < a.N) {
         const GlobalCand src{(const char *)a.pos_cell, a.cstart, a.cend};
-        search_one<D, kCaller, kDens>(a, t, src, s_stage + 16 * threadIdx.x, cnt, gcnt);
+        search_one<D, kCaller, kDens>(a, t, src, s_stage + kStageSlots * threadIdx.x, cnt, gcnt);
     }
     reduce_max_counts(a, cnt, gcnt);
 }
@@ -684,7 +723,8 @@ __global__ void k_pack_pos(const double *__restrict__ x0, const double *__restri
 
 void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
                   const double *x2, int N, double4 *pos, uint32_t *key, uint32_t *idx,
-                  uint32_t *skey, uint32_t *scratch, int *cstart, int *cend, hipStream_t s) {
+                  uint32_t *skey, uint32_t *scratch, int *cstart, int *cend, uint32_t *inv,
+                  hipStream_t s) {
     int nb = (N + 255) / 256;
     if (D == 2)
         hipLaunchKernelGGL(k_assign_cells<2>, dim3(nb), dim3(256), 0, s, g, x0, x1, x2, N, key, idx);
@@ -700,7 +740,7 @@ void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
     hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, s, cstart, 0, nc);
     hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, s, cend, 0, nc);
     hipLaunchKernelGGL(k_cell_ranges, dim3(nb), dim3(256), 0, s, skey, N, cstart, cend);
-    hipLaunchKernelGGL(k_gather_pos, dim3(nb), dim3(256), 0, s, x0, x1, x2, D, N, idx, pos);
+    hipLaunchKernelGGL(k_gather_pos, dim3(nb), dim3(256), 0, s, x0, x1, x2, D, N, idx, pos, inv);
 }
 
 void launch_pack_pos(const double *x0, const double *x1, const double *x2, int D, int N,

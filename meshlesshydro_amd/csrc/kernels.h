@@ -15,6 +15,7 @@ struct NbrArgs {
     const uint32_t *ck_w;        // cell of the particle at each work position
     int caller_order;            // work order = caller order (else cell order)
     const int *cstart, *cend;    // cell -> [start, end) of the cell order
+    const uint32_t *inv;         // original index -> cell-order position
     uint32_t *nbr;               // [maxr+maxg][N], work-order rows and entries
     int *noi, *nog;              // per work position
     int *status;                 // [0] bit 1: real overflow, bit 3: image overflow
@@ -101,7 +102,8 @@ enum CounterIdx {
 
 void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
                   const double *x2, int N, double4 *pos, uint32_t *key, uint32_t *idx,
-                  uint32_t *skey, uint32_t *scratch, int *cstart, int *cend, hipStream_t s);
+                  uint32_t *skey, uint32_t *scratch, int *cstart, int *cend, uint32_t *inv,
+                  hipStream_t s);
 // returns whether the search computed the density (NbrArgs::dens): the
 // cell-order search does not, and the caller then runs the density stage
 bool launch_neighbours(const NbrArgs &a, int D, hipStream_t s);
