@@ -309,12 +309,40 @@ struct GlobalCand {
     }
 };
 
+// whether the particle in cell cc has periodic images to search: its cell
+// lies within the image window of a domain face
+template <int D>
+__device__ inline bool has_images(const NbrArgs &a, const int *cc) {
+    if (!a.g.periodic) return false;
+    const int W = a.img_window;
+    bool near_edge = false;
+    for (int ax = 0; ax < D; ++ax) near_edge |= cc[ax] - W < 0 || cc[ax] + W >= a.g.nc[ax];
+    return near_edge;
+}
+
 // Entries a thread's image search sorts in its LDS stage before they go to
-// the row (more images than this continue with the row's own insertion)
-constexpr int kImgStage = 24;
-// stage slots per thread: the real hits' two 8-entry chunks, or the image
-// entries being sorted
-constexpr int kStageSlots = kImgStage > 16 ? kImgStage : 16;
+// the row (more images than this continue with the row's own insertion).
+// 40 holds every image of the 3D lattice with h = 2.4 dx (5 to 40 per
+// particle, mean 12) and keeps the image kernel at four wavefronts per SIMD
+// (10 KB of LDS per wavefront).
+constexpr int kImgStage = 40;
+// parents of staged images lie below 2^kStageIdxBits
+constexpr int kStageIdxBits = 26;
+
+// Sort word of an image of parent jo (below 2^kStageIdxBits) with the shift
+// s: jo << 6 | (7 - axis mask) << 3 | signs (bit a: +1 along a).  The words
+// of one particle's images ascend in its ghost order (parent, then the
+// descending mask) and never tie: the creation test fixes a parent's shift
+// along an axis by the side of the domain it lies on (+1 wants it within h
+// of the low face, -1 not there), so the parent and the mask fix the signs.
+__device__ inline uint32_t stage_word(uint32_t jo, const int *s) {
+    uint32_t m = 0, sg = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+        m |= (uint32_t)(s[ax] != 0) << ax;
+        sg |= (uint32_t)(s[ax] > 0) << ax;
+    }
+    return (jo << 6) | ((7u - m) << 3) | sg;
+}
 
 // Periodic images of work position t (position xi, cell cc) in the
 // reference's ghost order, stored after its cnt real entries: only
@@ -322,14 +350,15 @@ constexpr int kStageSlots = kImgStage > 16 ? kImgStage : 16;
 // search.  Returns the image count (an overflow sets the status bit and
 // keeps the entries found so far).
 //  * The first kImgStage hits are insertion-sorted in the thread's LDS
-//    stage (free once the real hits are stored), holding the parent's
-//    original index, so the ghost-order key of an entry needs no load; they
-//    go to the row as work positions (a.inv) when the search ends.
+//    stage as sort words (parent << 6 | (7 - axis mask) << 3 | shift signs:
+//    ascending words are the ghost order, see stage_word), each placed by a
+//    binary search; they go to the row as work positions (a.inv) when the
+//    search ends.  Parents from 2^26 on skip the stage.
 //  * Further hits (more images than the stage holds) are insertion-sorted
 //    in the row itself, each key read back through the entry's position.
 template <int D, bool kCaller>
 __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, const int *cc,
-                                   int cnt, uint32_t *stage) {
+                                   int cnt, uint32_t *stage, int ss) {
     const Geom &g = a.g;
     const int N = a.N;
     int gcnt = 0, err = 0;
@@ -339,11 +368,17 @@ __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, co
     if (g.periodic && near_edge) {
         uint32_t *nb = a.nbr;
         const int g0 = cnt;  // images follow the real entries
-        // staged entry (original index | code) -> row entry
-        auto to_row = [&](uint32_t e) {
-            return kCaller ? e : (a.inv[nb_index(e)] | (nb_code(e) << kIdxBits));
+        // staged sort word -> row entry (index | code)
+        auto to_row = [&](uint32_t w) {
+            const uint32_t jo = w >> 6, m = 7u - ((w >> 3) & 7u);
+            uint32_t code = 0, mul = 1;
+            for (int ax = 0; ax < 3; ++ax, mul *= 3)
+                code += mul * (((m >> ax) & 1u) ? (((w >> ax) & 1u) ? 2u : 0u) : 1u);
+            return (kCaller ? jo : a.inv[jo]) | (code << kIdxBits);
         };
-        bool in_row = false;  // the stage overflowed: entries are in the row
+        // entries are in the row: the stage overflowed, or parents are too
+        // large for sort words
+        bool in_row = (uint32_t)N > (1u << kStageIdxBits);
         for (int vz = (D == 3 ? cc[2] - W : 0); vz <= (D == 3 ? cc[2] + W : 0) && !err; ++vz)
             for (int vy = cc[1] - W; vy <= cc[1] + W && !err; ++vy)
                 for (int vx = cc[0] - W; vx <= cc[0] + W && !err; ++vx) {
@@ -358,9 +393,10 @@ __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, co
                     if (!wrapped) continue;
                     uint32_t c = cell_key<D>(g, r[0], r[1], r[2]);
                     uint32_t code = (uint32_t)((s[0] + 1) + 3 * (s[1] + 1) + 9 * (s[2] + 1));
-                    int q1 = a.cend[c];
-                    for (int q = a.cstart[c]; q < q1; ++q) {
-                        double4 pj = a.pos_cell[q];
+                    const uint32_t cw = stage_word(0, s);
+                    // one candidate: the reference's ghost creation test and
+                    // image, the distance test, the entry in ghost order
+                    auto visit = [&](int q, const double4 &pj) {
                         double xj[3] = {pj.x, pj.y, pj.z};
                         bool exists = true;
                         for (int ax = 0; ax < D; ++ax) {
@@ -370,26 +406,41 @@ __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, co
                             exists &= s[ax] > 0 ? low : (!low && high);
                             xj[ax] = image_coord(xj[ax], s[ax], g.bmin[ax], g.bmax[ax]);
                         }
-                        if (!exists) continue;
+                        if (!exists) return;
                         double d = dist2<D>(xj, xi);
-                        if (!(d < g.hsq)) continue;
-                        if (gcnt >= a.maxg) { err = 1; break; }
-                        const uint32_t jo = (uint32_t)pj.w;
-                        uint64_t ke = ghost_key(jo, code);
-                        if (!in_row && gcnt < kImgStage) {
-                            int p = gcnt;
-                            while (p > 0) {
-                                const uint32_t prev = stage[p - 1];
-                                if (ghost_key(nb_index(prev), nb_code(prev)) <= ke) break;
-                                stage[p] = prev;
-                                --p;
-                            }
-                            stage[p] = jo | (code << kIdxBits);
-                            ++gcnt;
-                            continue;
+                        if (!(d < g.hsq)) return;
+                        if (gcnt >= a.maxg) {
+                            err = 1;
+                            return;
                         }
+                        const uint32_t jo = (uint32_t)pj.w;
+                        if (!in_row && gcnt < kImgStage) {
+                            // after the last word not above this one, the
+                            // words behind it moved up by one (four reads,
+                            // then four writes)
+                            const uint32_t w = (jo << 6) | cw;
+                            int lo = 0, hi = gcnt;
+                            while (lo < hi) {
+                                const int mid = (lo + hi) >> 1;
+                                if (stage[mid * ss] <= w) lo = mid + 1;
+                                else hi = mid;
+                            }
+                            int p = gcnt;
+                            for (; p - 4 >= lo; p -= 4) {
+                                uint32_t b[4];
+#pragma unroll
+                                for (int u = 0; u < 4; ++u) b[u] = stage[(p - 1 - u) * ss];
+#pragma unroll
+                                for (int u = 0; u < 4; ++u) stage[(p - u) * ss] = b[u];
+                            }
+                            for (; p > lo; --p) stage[p * ss] = stage[(p - 1) * ss];
+                            stage[p * ss] = w;
+                            ++gcnt;
+                            return;
+                        }
+                        const uint64_t ke = ghost_key(jo, code);
                         if (!in_row) {
-                            for (int k = 0; k < gcnt; ++k) nb[nb_slot(N, t, g0 + k)] = to_row(stage[k]);
+                            for (int k = 0; k < gcnt; ++k) nb[nb_slot(N, t, g0 + k)] = to_row(stage[k * ss]);
                             in_row = true;
                         }
                         int p = gcnt;
@@ -402,18 +453,30 @@ __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, co
                         }
                         nb[nb_slot(N, t, g0 + p)] = (kCaller ? jo : (uint32_t)q) | (code << kIdxBits);
                         ++gcnt;
+                    };
+                    // candidates four at a time (indices clamped to the cell,
+                    // the loads in flight together), visited in cell order
+                    const int q0 = a.cstart[c], q1 = a.cend[c];
+                    for (int q = q0; q < q1 && !err; q += 4) {
+                        double4 pb[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) pb[u] = a.pos_cell[q + u < q1 ? q + u : q1 - 1];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (q + u < q1 && !err) visit(q + u, pb[u]);
                     }
                 }
         if (!in_row)
-            for (int k = 0; k < gcnt; ++k) nb[nb_slot(N, t, g0 + k)] = to_row(stage[k]);
+            for (int k = 0; k < gcnt; ++k) nb[nb_slot(N, t, g0 + k)] = to_row(stage[k * ss]);
         if (err) atomicOr(a.status, 1 << 3);
     }
     return gcnt;
 }
 
 // The search of work position t: real neighbours in the reference's stencil
-// order (i itself excluded: in cell order its candidate position is t), the
-// periodic images in ghost order, and with kDens the step's density.
+// order (i itself excluded: in cell order its candidate position is t), and
+// with kDens the step's density (whole, or up to the images' terms for a
+// particle whose images k_images searches).
 //  * A stencil cell whose box is farther than h from x_i (with a relative
 //    margin far above the rounding of positions and cell indices; edge
 //    cells are open outward, where clamped particles sit) holds no
@@ -423,14 +486,14 @@ __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, co
 //    every eight hits are stored as one 32-byte sector of the chunked row.
 //  * The capacity is checked per group of four: the list has four guard
 //    entries past its last one for the overshoot.
-// Returns the real and image counts through cnt / gcnt.
+// Returns the real count through cnt_out.
 template <int D, bool kCaller, int kDens, typename Src>
 __device__ inline void search_one(const NbrArgs &a, int t, const Src &src, uint32_t *stage,
-                                  int &cnt_out, int &gcnt_out) {
+                                  int &cnt_out) {
     using TW = typename std::conditional<kDens == 4, float, double>::type;
     const Geom &g = a.g;
     const int N = a.N;
-    int cnt = 0, gcnt = 0;
+    int cnt = 0;
     const TW wh2 = TW(g.h / 2.), wsig = TW(g.sigma);
     TW omg = TW(0);
     double4 pi = a.pos_w[t];
@@ -511,31 +574,28 @@ __device__ inline void search_one(const NbrArgs &a, int t, const Src &src, uint3
         atomicOr(a.status, 1 << 1);
     }
     a.noi[t] = cnt;
-    if (!err) gcnt = image_search<D, kCaller>(a, t, xi, cc, cnt, stage);
-    a.nog[t] = gcnt;
+    // a particle within the image window of a periodic domain face leaves
+    // its images to k_images (nog = -1 marks it); the others have none
+    const bool images = !err && has_images<D>(a, cc);
+    a.nog[t] = images ? -1 : 0;
     if constexpr (kDens != 0) {
         // compDensity (Particles.cpp:1151-1184): the self term after the
-        // real neighbours, then the images in ghost order (:2262-2290)
-        TW om = omg + kernel_w<TW>(TW(0), wh2, wsig);
-        if (g.periodic) {
-            for (int p = 0; p < gcnt; ++p) {
-                const uint32_t e = nb_get(a.nbr, N, t, cnt + p);
-                const double4 pj = kCaller ? a.pos_w[nb_index(e)] : a.pos_cell[nb_index(e)];
-                double xj[3] = {pj.x, pj.y, pj.z};
-                for (int ax = 0; ax < D; ++ax)
-                    xj[ax] = image_coord(xj[ax], code_shift(nb_code(e), ax), g.bmin[ax], g.bmax[ax]);
-                om += kernel_w<TW>(wdist<D, TW>(dist2<D>(xi, xj), xi, xj), wh2, wsig);
-            }
-        }
-        write_prim<D, TW>((TW *)a.prim + (size_t)t * kPrimN, om, a.m[io], a.u[io], a.v[0][io],
-                          a.v[1][io], D == 3 ? a.v[2][io] : 0., a.gamma);
+        // real neighbours, then the images in ghost order (:2262-2290), which
+        // k_images adds to the partial sum left in the record's omega slot
+        const TW om = omg + kernel_w<TW>(TW(0), wh2, wsig);
+        TW *pr = (TW *)a.prim + (size_t)t * kPrimN;
+        if (images)
+            pr[kPrimOmega] = om;
+        else
+            write_prim<D, TW>(pr, om, a.m[io], a.u[io], a.v[0][io], a.v[1][io],
+                              D == 3 ? a.v[2][io] : 0., a.gamma);
     }
     cnt_out = cnt;
-    gcnt_out = gcnt;
 }
 
 // largest counts of the step: a wavefront maximum, one atomic per wavefront
-// (every lane of the wavefront calls this)
+// and count (every lane of the wavefront calls this); the maxima start at 0,
+// so a zero maximum needs no atomic
 __device__ inline void reduce_max_counts(const NbrArgs &a, int cnt, int gcnt) {
     for (int off = 32; off > 0; off >>= 1) {
         int o = __shfl_xor(cnt, off, 64);
@@ -544,9 +604,72 @@ __device__ inline void reduce_max_counts(const NbrArgs &a, int cnt, int gcnt) {
         gcnt = o > gcnt ? o : gcnt;
     }
     if (__lane_id() == 0) {
-        atomicMax(&a.max_counts[0], cnt);
-        atomicMax(&a.max_counts[1], gcnt);
+        if (cnt > 0) atomicMax(&a.max_counts[0], cnt);
+        if (gcnt > 0) atomicMax(&a.max_counts[1], gcnt);
     }
+}
+
+// The periodic images of the particles the search left them to (nog = -1):
+// the image search, then with kDens the images' terms of the density in row
+// order after the partial sum the search left, and the primitive record.  A
+// kernel of its own, so that the near-boundary lanes' image scans neither
+// stall the search's wavefronts nor raise its register count.  One thread
+// per work position (cell order: a wavefront's particles lie near the same
+// domain faces and share their image windows), one wavefront per block; a
+// lane's kImgStage LDS slots are interleaved with its neighbours' (slot k of
+// lane l at k * 64 + l: no bank conflicts).
+constexpr int kImgThreads = 64;
+template <int D, bool kCaller, int kDens>
+__global__ __launch_bounds__(kImgThreads) void k_images(NbrArgs a) {
+    __shared__ uint32_t s_stage[kImgThreads * kImgStage];
+    using TW = typename std::conditional<kDens == 4, float, double>::type;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    int gcnt = 0;
+    if (t < a.N && a.nog[t] < 0) {
+        const Geom &g = a.g;
+        const int N = a.N;
+        const double4 pi = a.pos_w[t];
+        const double xi[3] = {pi.x, pi.y, pi.z};
+        int cc[3];
+        cell_coords<D>(g, a.ck_w[t], cc);
+        const int cnt = a.noi[t];
+        gcnt = image_search<D, kCaller>(a, t, xi, cc, cnt, s_stage + threadIdx.x, kImgThreads);
+        a.nog[t] = gcnt;
+        if constexpr (kDens != 0) {
+            const TW wh2 = TW(g.h / 2.), wsig = TW(g.sigma);
+            TW *pr = (TW *)a.prim + (size_t)t * kPrimN;
+            TW om = pr[kPrimOmega];
+            // the images in row order, four at a time: their entries, then
+            // their positions, are in flight together
+            auto add = [&](uint32_t e, const double4 &pj) {
+                double xj[3] = {pj.x, pj.y, pj.z};
+                for (int ax = 0; ax < D; ++ax)
+                    xj[ax] = image_coord(xj[ax], code_shift(nb_code(e), ax), g.bmin[ax],
+                                         g.bmax[ax]);
+                om += kernel_w<TW>(wdist<D, TW>(dist2<D>(xi, xj), xi, xj), wh2, wsig);
+            };
+            const double4 *pw = kCaller ? a.pos_w : a.pos_cell;
+            int p = 0;
+            for (; p + 4 <= gcnt; p += 4) {
+                uint32_t eb[4];
+                double4 pb[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) eb[u] = nb_get(a.nbr, N, t, cnt + p + u);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pb[u] = pw[nb_index(eb[u])];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) add(eb[u], pb[u]);
+            }
+            for (; p < gcnt; ++p) {
+                const uint32_t e = nb_get(a.nbr, N, t, cnt + p);
+                add(e, pw[nb_index(e)]);
+            }
+            const uint32_t io = (uint32_t)pi.w;
+            write_prim<D, TW>(pr, om, a.m[io], a.u[io], a.v[0][io], a.v[1][io],
+                              D == 3 ? a.v[2][io] : 0., a.gamma);
+        }
+    }
+    reduce_max_counts(a, 0, gcnt);
 }
 
 // kDens: 0, or the bytes of the primitive records (4, 8) when the step's
@@ -554,14 +677,14 @@ __device__ inline void reduce_max_counts(const NbrArgs &a, int cnt, int gcnt) {
 // position, candidates from global memory: caller order and linear keys.
 template <int D, bool kCaller, int kDens>
 __global__ __launch_bounds__(256) void k_neighbours(NbrArgs a) {
-    __shared__ __align__(16) uint32_t s_stage[256 * kStageSlots];
+    __shared__ __align__(16) uint32_t s_stage[256 * 16];
     const int t = xcd_block() * blockDim.x + threadIdx.x;
-    int cnt = 0, gcnt = 0;
+    int cnt = 0;
     if (t < a.N) {
         const GlobalCand src{(const char *)a.pos_cell, a.cstart, a.cend};
-        search_one<D, kCaller, kDens>(a, t, src, s_stage + kStageSlots * threadIdx.x, cnt, gcnt);
+        search_one<D, kCaller, kDens>(a, t, src, s_stage + 16 * threadIdx.x, cnt);
     }
-    reduce_max_counts(a, cnt, gcnt);
+    reduce_max_counts(a, cnt, 0);
 }
 
 template __global__ void k_assign_cells<2>(Geom, const double *, const double *, const double *,
@@ -748,11 +871,19 @@ void launch_pack_pos(const double *x0, const double *x1, const double *x2, int D
     hipLaunchKernelGGL(k_pack_pos, dim3((N + 255) / 256), dim3(256), 0, s, x0, x1, x2, D, N, pos);
 }
 
+template <int D, bool kCaller, int kDens>
+static void launch_nbr_dens(const NbrArgs &a, dim3 grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_neighbours<D, kCaller, kDens>), grid, dim3(256), 0, s, a);
+    if (a.g.periodic)
+        hipLaunchKernelGGL((k_images<D, kCaller, kDens>), dim3((a.N + kImgThreads - 1) / kImgThreads),
+                           dim3(kImgThreads), 0, s, a);
+}
+
 template <int D, bool kCaller>
 static void launch_nbr(const NbrArgs &a, dim3 grid, hipStream_t s) {
-    if (a.dens == 4) hipLaunchKernelGGL((k_neighbours<D, kCaller, 4>), grid, dim3(256), 0, s, a);
-    else if (a.dens == 8) hipLaunchKernelGGL((k_neighbours<D, kCaller, 8>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_neighbours<D, kCaller, 0>), grid, dim3(256), 0, s, a);
+    if (a.dens == 4) launch_nbr_dens<D, kCaller, 4>(a, grid, s);
+    else if (a.dens == 8) launch_nbr_dens<D, kCaller, 8>(a, grid, s);
+    else launch_nbr_dens<D, kCaller, 0>(a, grid, s);
 }
 
 bool launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {
