@@ -140,8 +140,12 @@ __device__ inline double image_coord(double x, int s, double mn, double mx) {
     return s == 0 ? x : (s > 0 ? mx + (x - mn) : mn - (mx - x));
 }
 
-// glibc pow(t, 3.) is correctly rounded; a double-double cube rounded once
-// gives the same bits (error of the unrounded value ~2^-106 relative).
+// The cube of the kernel's outer branch, correctly rounded: a double-double
+// cube rounded once (error of the unrounded value ~2^-106 relative).  glibc's
+// pow(t, 3.) agrees except where its own result (within 0.52 ulp) is not the
+// correctly rounded one: 2 of the 117 distinct arguments of a 2D lattice with
+// h = 8 dx, none of a lattice with h = 4 dx; there the densities differ by
+// one ulp.
 __device__ inline double cube_cr(double t) {
     double hi = t * t;
     double lo = __fma_rn(t, t, -hi);

@@ -9,7 +9,8 @@ from meshlesshydro_amd._abi import MLH_ABS_FABS, default_params
 def kh_lattice(n=100, **kw):
     """The reference's only compilable MFV setup (parameter_long_run.h,
     config_long_run.info with box [0,1]^2): KH lattice, h = 4 dx."""
-    p = default_params(kernel_size=4.0 / n, **kw)
+    kw.setdefault("kernel_size", 4.0 / n)
+    p = default_params(**kw)
     return p, ics.kelvin_helmholtz(n)
 
 
@@ -72,7 +73,8 @@ def jitter3d(n=12, seed=11, **kw):
     u = P / ((5.0 / 3.0 - 1.0) * rho)
     kw.setdefault("max_interactions", 128)
     kw.setdefault("max_ghost_interactions", 128)
-    p = default_params(dim=3, kernel_size=2.4 / n, **kw)
+    kw.setdefault("kernel_size", 2.4 / n)
+    p = default_params(dim=3, **kw)
     return p, dict(x=X, y=Y, z=Z, vx=vx, vy=vy, vz=vz, m=m, u=u)
 
 

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from cases import CASES, assert_fp32_fluxes, kh_lattice, relerr
+from meshlesshydro_amd._abi import MLH_ABS_FABS
 
 pytestmark = pytest.mark.gpu
 
@@ -102,6 +103,48 @@ def test_multistep_and_conservation(hip, oracle_mod):
     assert abs(c1[2] - np.sum(m * (gs["u"] + 0.5 * v2))) <= 1e-12 * abs(c1[2])
     assert abs(c1[3] - np.sum(m * gs["vx"])) <= 1e-13 * np.sum(m)
     assert abs(c1[4] - np.sum(m * gs["vy"])) <= 1e-13 * np.sum(m)
+    sim.close()
+
+
+@pytest.mark.parametrize("case", ["jitter12_h3.6", "kh40_h8"])
+def test_image_rows_past_the_lds_stage(case, hip, oracle_mod):
+    """Particles with more periodic images than the image search's LDS stage
+    holds (40): h = 3.6 dx in 3D, 8 dx in 2D, up to 158 images per particle.
+    The first 40 images go to the row in ghost order, the rest continue with
+    the insertion into the row itself: lists bit-exact against the oracle,
+    and the fused step (whose density adds the images' terms in row order)
+    within 1e-10 of the oracle's step.
+
+    The 2D case runs the limiter with fabs: its pressure is uniform up to
+    round-off, the reference's int abs() turns the limiter's 0/0 ratio into
+    INT_MIN and scales the gradient's round-off noise by -2^31, and glibc's
+    pow(t, 3.) misses the correctly rounded cube for 2 of the case's 117
+    distinct kernel arguments (the GPU's cube is correctly rounded), so that
+    one-ulp density differences would reach the state at ~1e-8."""
+    from cases import jitter3d
+    if case == "jitter12_h3.6":
+        p, s = jitter3d(12, kernel_size=3.6 / 12, max_interactions=256, max_ghost_interactions=256)
+    else:
+        p, s = kh_lattice(40, kernel_size=8.0 / 40, max_interactions=256, max_ghost_interactions=256,
+                          limiter_abs=MLH_ABS_FABS)
+    print(f"\n[images past the stage {case}] N={len(s['x'])} dim={p.dim}", flush=True)
+    sim = hip.Simulation(p, s)
+    orc = oracle_mod.Oracle(p, s)
+    sim.neighbours()
+    orc.neighbours()
+    gn, gg, gl = sim.get_neighbours()
+    on, og, ol = orc.get_neighbours()
+    assert gg.max() > 40, "the case must overflow the LDS stage"
+    assert np.array_equal(gn, on) and np.array_equal(gg, og)
+    mask = np.arange(gl.shape[1])[None, :] < (gn + gg)[:, None]
+    assert np.array_equal(np.where(mask, gl, 0), np.where(mask, ol, 0))
+    sim.close()
+    sim = hip.Simulation(p, s)
+    dt_g, dt_o = sim.step(), orc.step()
+    assert abs(dt_g - dt_o) <= 1e-11 * dt_o
+    gs, os_ = sim.state(), orc.state()
+    for k in gs:
+        assert relerr(gs[k], os_[k]) < 1e-10, k
     sim.close()
 
 
