@@ -33,6 +33,11 @@ struct mlh_ctx {
     size_t tsz = 8;  // bytes of the derived-data type
     int stride = 0;  // maxr + maxg
     hipStream_t s = nullptr;
+    // periodic domains: the image search's stream and its fork / join events
+    hipStream_t s_img = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int *img_cnt = nullptr;
+    int img_base = 0;
     Geom g{};
     double *x[3] = {}, *v[3] = {}, *m = nullptr, *u = nullptr;
     double *xn[3] = {}, *vn[3] = {}, *mn = nullptr, *un = nullptr;
@@ -255,6 +260,14 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
         delete c;
         return fail(MLH_ERR_DEVICE, "hipStreamCreate failed");
     }
+    if (p->periodic &&
+        (hipStreamCreateWithFlags(&c->s_img, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)) {
+        mlh_destroy(c);
+        return fail(MLH_ERR_DEVICE, "hipStreamCreate / hipEventCreate failed");
+    }
+    c->img_base = image_base(p->max_interactions);
     size_t N = (size_t)n, dsz = sizeof(double) * N;
     int D = c->D;
     int frn = D == 2 ? FRec<2>::N : FRec<3>::N;
@@ -276,8 +289,15 @@ int mlh_create(const mlh_params *p, int32_t n, mlh_ctx **out) {
         A_(&c->lim_out, sizeof(double) * 2 * D);
     }
     // four guard rows past the list: the search stores up to four hits past
-    // MAX_NUM_INTERACTIONS before it detects the overflow
-    A_(&c->nbr, 4 * nb_words((int)N, c->stride + 4));
+    // MAX_NUM_INTERACTIONS before it detects the overflow; periodic domains
+    // also hold the image search's entries from img_base on
+    {
+        int rows = c->stride + 4;
+        if (p->periodic && c->img_base + p->max_ghost_interactions > rows)
+            rows = c->img_base + p->max_ghost_interactions;
+        A_(&c->nbr, 4 * nb_words((int)N, rows));
+    }
+    if (p->periodic) A_(&c->img_cnt, 4 * N);
     A_(&c->noi, 4 * N); A_(&c->nog, 4 * N);
     A_(&c->prim, c->tsz * kPrimN * N);
     A_(&c->frec, c->tsz * frn * N);
@@ -319,6 +339,10 @@ int mlh_destroy(mlh_ctx *c) {
     mlp_free(c->mlp);
     for (hipEvent_t e : c->evpool) hipEventDestroy(e);
     if (c->dt_hist) hipFree(c->dt_hist);
+    if (c->s_img) hipStreamSynchronize(c->s_img);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->s_img) hipStreamDestroy(c->s_img);
     if (c->s) hipStreamDestroy(c->s);
     delete c;
     return MLH_OK;
@@ -427,6 +451,8 @@ static int enqueue_neighbours(mlh_ctx *c, bool fuse_density = false) {
     a.status = c->status;
     a.max_counts = c->max_counts;
     a.img_window = c->img_window;
+    a.img_cnt = c->img_cnt;
+    a.img_base = c->img_base;
     a.dens = fuse_density ? (c->prec == MLH_FP32 ? 4 : 8) : 0;
     if (fuse_density) {
         a.prim = c->prim;
@@ -435,7 +461,7 @@ static int enqueue_neighbours(mlh_ctx *c, bool fuse_density = false) {
         for (int k = 0; k < 3; ++k) a.v[k] = c->v[k];
         a.gamma = c->p.gamma;
     }
-    const bool fused = launch_neighbours(a, c->D, c->s);
+    const bool fused = launch_neighbours(a, c->D, c->s, c->s_img, c->ev_fork, c->ev_join);
     if (fuse_density && !fused) launch_density(hydro_args(c), c->D, c->prec, c->s);
     c->face_stage = 0;
     c->grad_state = 0;

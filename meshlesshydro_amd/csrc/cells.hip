@@ -345,7 +345,7 @@ __device__ inline uint32_t stage_word(uint32_t jo, const int *s) {
 }
 
 // Periodic images of work position t (position xi, cell cc) in the
-// reference's ghost order, stored after its cnt real entries: only
+// reference's ghost order, stored in its row from entry g0 on: only
 // particles whose cell lies within the image window of a domain face
 // search.  Returns the image count (an overflow sets the status bit and
 // keeps the entries found so far).
@@ -358,7 +358,7 @@ __device__ inline uint32_t stage_word(uint32_t jo, const int *s) {
 //    in the row itself, each key read back through the entry's position.
 template <int D, bool kCaller>
 __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, const int *cc,
-                                   int cnt, uint32_t *stage, int ss) {
+                                   int g0, uint32_t *stage, int ss) {
     const Geom &g = a.g;
     const int N = a.N;
     int gcnt = 0, err = 0;
@@ -367,7 +367,6 @@ __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, co
     for (int ax = 0; ax < D; ++ax) near_edge |= cc[ax] - W < 0 || cc[ax] + W >= g.nc[ax];
     if (g.periodic && near_edge) {
         uint32_t *nb = a.nbr;
-        const int g0 = cnt;  // images follow the real entries
         // staged sort word -> row entry (index | code)
         auto to_row = [&](uint32_t w) {
             const uint32_t jo = w >> 6, m = 7u - ((w >> 3) & 7u);
@@ -476,7 +475,7 @@ __device__ inline int image_search(const NbrArgs &a, int t, const double *xi, co
 // The search of work position t: real neighbours in the reference's stencil
 // order (i itself excluded: in cell order its candidate position is t), and
 // with kDens the step's density (whole, or up to the images' terms for a
-// particle whose images k_images searches).
+// particle whose images k_images searches and k_images_finish adds).
 //  * A stencil cell whose box is farther than h from x_i (with a relative
 //    margin far above the rounding of positions and cell indices; edge
 //    cells are open outward, where clamped particles sit) holds no
@@ -574,14 +573,15 @@ __device__ inline void search_one(const NbrArgs &a, int t, const Src &src, uint3
         atomicOr(a.status, 1 << 1);
     }
     a.noi[t] = cnt;
-    // a particle within the image window of a periodic domain face leaves
-    // its images to k_images (nog = -1 marks it); the others have none
+    // a particle within the image window of a periodic domain face has its
+    // images from k_images, moved into place by k_images_finish (nog = -1
+    // marks it); the others have none
     const bool images = !err && has_images<D>(a, cc);
     a.nog[t] = images ? -1 : 0;
     if constexpr (kDens != 0) {
         // compDensity (Particles.cpp:1151-1184): the self term after the
         // real neighbours, then the images in ghost order (:2262-2290), which
-        // k_images adds to the partial sum left in the record's omega slot
+        // k_images_finish adds to the partial sum left in the omega slot
         const TW om = omg + kernel_w<TW>(TW(0), wh2, wsig);
         TW *pr = (TW *)a.prim + (size_t)t * kPrimN;
         if (images)
@@ -609,61 +609,79 @@ __device__ inline void reduce_max_counts(const NbrArgs &a, int cnt, int gcnt) {
     }
 }
 
-// The periodic images of the particles the search left them to (nog = -1):
-// the image search, then with kDens the images' terms of the density in row
-// order after the partial sum the search left, and the primitive record.  A
-// kernel of its own, so that the near-boundary lanes' image scans neither
-// stall the search's wavefronts nor raise its register count.  One thread
-// per work position (cell order: a wavefront's particles lie near the same
-// domain faces and share their image windows), one wavefront per block; a
-// lane's kImgStage LDS slots are interleaved with its neighbours' (slot k of
-// lane l at k * 64 + l: no bank conflicts).
+// The periodic image search, on a stream of its own beside k_neighbours: a
+// particle whose cell lies within the image window of a domain face stores
+// its images from row entry img_base on (behind every sector the real
+// search stores) and their count in img_cnt.  One thread per work position
+// (cell order: a wavefront's particles lie near the same domain faces), one
+// wavefront per block; a lane's kImgStage LDS slots are interleaved with
+// its neighbours' (slot k of lane l at k * 64 + l: no bank conflicts).
 constexpr int kImgThreads = 64;
-template <int D, bool kCaller, int kDens>
+template <int D, bool kCaller>
 __global__ __launch_bounds__(kImgThreads) void k_images(NbrArgs a) {
     __shared__ uint32_t s_stage[kImgThreads * kImgStage];
+    const int t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= a.N) return;
+    int cc[3];
+    cell_coords<D>(a.g, a.ck_w[t], cc);
+    if (!has_images<D>(a, cc)) return;
+    const double4 pi = a.pos_w[t];
+    const double xi[3] = {pi.x, pi.y, pi.z};
+    a.img_cnt[t] = image_search<D, kCaller>(a, t, xi, cc, a.img_base, s_stage + threadIdx.x,
+                                            kImgThreads);
+}
+
+// After k_neighbours and k_images: for the particles k_neighbours left their
+// images to (nog = -1; not after a real overflow), the images move behind
+// the real entries (an ascending copy: the target starts below the source),
+// and with kDens their density terms are added in row order to the partial
+// sum in the record's omega slot, then the primitive record is written.
+template <int D, bool kCaller, int kDens>
+__global__ __launch_bounds__(256) void k_images_finish(NbrArgs a) {
     using TW = typename std::conditional<kDens == 4, float, double>::type;
     const int t = xcd_block() * blockDim.x + threadIdx.x;
     int gcnt = 0;
     if (t < a.N && a.nog[t] < 0) {
         const Geom &g = a.g;
-        const int N = a.N;
+        const int N = a.N, cnt = a.noi[t], src = a.img_base;
+        gcnt = a.img_cnt[t];
+        a.nog[t] = gcnt;
         const double4 pi = a.pos_w[t];
         const double xi[3] = {pi.x, pi.y, pi.z};
-        int cc[3];
-        cell_coords<D>(g, a.ck_w[t], cc);
-        const int cnt = a.noi[t];
-        gcnt = image_search<D, kCaller>(a, t, xi, cc, cnt, s_stage + threadIdx.x, kImgThreads);
-        a.nog[t] = gcnt;
-        if constexpr (kDens != 0) {
-            const TW wh2 = TW(g.h / 2.), wsig = TW(g.sigma);
-            TW *pr = (TW *)a.prim + (size_t)t * kPrimN;
-            TW om = pr[kPrimOmega];
-            // the images in row order, four at a time: their entries, then
-            // their positions, are in flight together
-            auto add = [&](uint32_t e, const double4 &pj) {
-                double xj[3] = {pj.x, pj.y, pj.z};
-                for (int ax = 0; ax < D; ++ax)
-                    xj[ax] = image_coord(xj[ax], code_shift(nb_code(e), ax), g.bmin[ax],
-                                         g.bmax[ax]);
-                om += kernel_w<TW>(wdist<D, TW>(dist2<D>(xi, xj), xi, xj), wh2, wsig);
-            };
-            const double4 *pw = kCaller ? a.pos_w : a.pos_cell;
-            int p = 0;
-            for (; p + 4 <= gcnt; p += 4) {
-                uint32_t eb[4];
-                double4 pb[4];
+        const TW wh2 = TW(g.h / 2.), wsig = TW(g.sigma);
+        TW *pr = (TW *)a.prim + (size_t)t * kPrimN;
+        TW om = TW(0);
+        if constexpr (kDens != 0) om = pr[kPrimOmega];
+        auto add = [&](uint32_t e, const double4 &pj) {
+            double xj[3] = {pj.x, pj.y, pj.z};
+            for (int ax = 0; ax < D; ++ax)
+                xj[ax] = image_coord(xj[ax], code_shift(nb_code(e), ax), g.bmin[ax], g.bmax[ax]);
+            om += kernel_w<TW>(wdist<D, TW>(dist2<D>(xi, xj), xi, xj), wh2, wsig);
+        };
+        const double4 *pw = kCaller ? a.pos_w : a.pos_cell;
+        // four entries at a time: read, stored, and (with kDens) their
+        // positions in flight together
+        int p = 0;
+        for (; p + 4 <= gcnt; p += 4) {
+            uint32_t eb[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) eb[u] = nb_get(a.nbr, N, t, cnt + p + u);
+            for (int u = 0; u < 4; ++u) eb[u] = nb_get(a.nbr, N, t, src + p + u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a.nbr[nb_slot(N, t, cnt + p + u)] = eb[u];
+            if constexpr (kDens != 0) {
+                double4 pb[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) pb[u] = pw[nb_index(eb[u])];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) add(eb[u], pb[u]);
             }
-            for (; p < gcnt; ++p) {
-                const uint32_t e = nb_get(a.nbr, N, t, cnt + p);
-                add(e, pw[nb_index(e)]);
-            }
+        }
+        for (; p < gcnt; ++p) {
+            const uint32_t e = nb_get(a.nbr, N, t, src + p);
+            a.nbr[nb_slot(N, t, cnt + p)] = e;
+            if constexpr (kDens != 0) add(e, pw[nb_index(e)]);
+        }
+        if constexpr (kDens != 0) {
             const uint32_t io = (uint32_t)pi.w;
             write_prim<D, TW>(pr, om, a.m[io], a.u[io], a.v[0][io], a.v[1][io],
                               D == 3 ? a.v[2][io] : 0., a.gamma);
@@ -871,29 +889,46 @@ void launch_pack_pos(const double *x0, const double *x1, const double *x2, int D
     hipLaunchKernelGGL(k_pack_pos, dim3((N + 255) / 256), dim3(256), 0, s, x0, x1, x2, D, N, pos);
 }
 
+struct NbrStreams {
+    hipStream_t s, s_img;
+    hipEvent_t fork, join;
+};
+
+// periodic: the image search forks to s_img after the cell list and joins
+// before k_images_finish
 template <int D, bool kCaller, int kDens>
-static void launch_nbr_dens(const NbrArgs &a, dim3 grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_neighbours<D, kCaller, kDens>), grid, dim3(256), 0, s, a);
-    if (a.g.periodic)
-        hipLaunchKernelGGL((k_images<D, kCaller, kDens>), dim3((a.N + kImgThreads - 1) / kImgThreads),
-                           dim3(kImgThreads), 0, s, a);
+static void launch_nbr_dens(const NbrArgs &a, dim3 grid, const NbrStreams &st) {
+    if (a.g.periodic) {
+        hipEventRecord(st.fork, st.s);
+        hipStreamWaitEvent(st.s_img, st.fork, 0);
+        hipLaunchKernelGGL((k_images<D, kCaller>), dim3((a.N + kImgThreads - 1) / kImgThreads),
+                           dim3(kImgThreads), 0, st.s_img, a);
+        hipEventRecord(st.join, st.s_img);
+    }
+    hipLaunchKernelGGL((k_neighbours<D, kCaller, kDens>), grid, dim3(256), 0, st.s, a);
+    if (a.g.periodic) {
+        hipStreamWaitEvent(st.s, st.join, 0);
+        hipLaunchKernelGGL((k_images_finish<D, kCaller, kDens>), grid, dim3(256), 0, st.s, a);
+    }
 }
 
 template <int D, bool kCaller>
-static void launch_nbr(const NbrArgs &a, dim3 grid, hipStream_t s) {
-    if (a.dens == 4) launch_nbr_dens<D, kCaller, 4>(a, grid, s);
-    else if (a.dens == 8) launch_nbr_dens<D, kCaller, 8>(a, grid, s);
-    else launch_nbr_dens<D, kCaller, 0>(a, grid, s);
+static void launch_nbr(const NbrArgs &a, dim3 grid, const NbrStreams &st) {
+    if (a.dens == 4) launch_nbr_dens<D, kCaller, 4>(a, grid, st);
+    else if (a.dens == 8) launch_nbr_dens<D, kCaller, 8>(a, grid, st);
+    else launch_nbr_dens<D, kCaller, 0>(a, grid, st);
 }
 
-bool launch_neighbours(const NbrArgs &a, int D, hipStream_t s) {
+bool launch_neighbours(const NbrArgs &a, int D, hipStream_t s, hipStream_t s_img,
+                       hipEvent_t fork, hipEvent_t join) {
     dim3 grid((a.N + 255) / 256);
+    const NbrStreams st{s, s_img, fork, join};
     if (a.caller_order) {
-        if (D == 2) launch_nbr<2, true>(a, grid, s);
-        else launch_nbr<3, true>(a, grid, s);
+        if (D == 2) launch_nbr<2, true>(a, grid, st);
+        else launch_nbr<3, true>(a, grid, st);
     } else {
-        if (D == 2) launch_nbr<2, false>(a, grid, s);
-        else launch_nbr<3, false>(a, grid, s);
+        if (D == 2) launch_nbr<2, false>(a, grid, st);
+        else launch_nbr<3, false>(a, grid, st);
     }
     return a.dens != 0;
 }

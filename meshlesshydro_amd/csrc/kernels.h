@@ -21,6 +21,12 @@ struct NbrArgs {
     int *status;                 // [0] bit 1: real overflow, bit 3: image overflow
     int *max_counts;             // [2]
     int img_window;              // virtual-cell reach of the image search
+    // The image search runs beside the real search: a particle's images go
+    // to its row from entry img_base on (behind every sector the real search
+    // may store, guards included), their count to img_cnt; k_images_finish
+    // moves them behind the real entries.
+    int *img_cnt;
+    int img_base;
     // The step's compDensity + compPressure fused into the search (dens =
     // 4: FP32 records, 8: FP64 records, 0: off): omega summed over the hits
     // in row order, the self term, the images in row order, then the
@@ -106,7 +112,13 @@ void launch_cells(const Geom &g, int D, const double *x0, const double *x1,
                   hipStream_t s);
 // returns whether the search computed the density (NbrArgs::dens): the
 // cell-order search does not, and the caller then runs the density stage
-bool launch_neighbours(const NbrArgs &a, int D, hipStream_t s);
+// Row entry where the image search stores: past the sector holding the real
+// search's last guard entry (maxr + 3).
+inline int image_base(int maxr) { return (maxr + 4 + kNbChunk - 1) / kNbChunk * kNbChunk; }
+// The search on stream s; for a periodic domain the image search runs on
+// s_img between the events fork (recorded on s) and join (waited on by s).
+bool launch_neighbours(const NbrArgs &a, int D, hipStream_t s, hipStream_t s_img,
+                       hipEvent_t fork, hipEvent_t join);
 // Particles::getDomainLimits with the reference's sequential semantics;
 // scratch: 3 * D * limits_blocks(N) doubles; out: [min, max] per axis
 int limits_blocks(int N);
