@@ -5,7 +5,9 @@
  * Python/ctypes or any FFI binds these symbols from libmlh_hip.so.
  * Every entry point returns an mlh_status (0 = OK).  All calls on one
  * context are synchronous with respect to the caller and must come from one
- * host thread; device work of a context runs on one HIP stream.
+ * host thread; device work of a context runs on its HIP stream (for a
+ * periodic domain the image search also runs on a second stream of the
+ * context, forked from and joined back into the first within each search).
  *
  * Reference interface replaced (jammartin/meshlessHydro, demonstrator/):
  *   Particles      include/Particles.h:28-240   (SoA state, per-stage methods)
