@@ -79,36 +79,47 @@ __global__ void k_gather_pos(const double *__restrict__ x0, const double *__rest
 }
 
 // ---------------------------------------------------------------- radix sort
+// LSD passes of up to kRsMaxBits bits: digit width ceil(bits / passes) with
+// passes = ceil(bits / kRsMaxBits) (18-bit Morton keys: two 9-bit passes).
 constexpr int kRsThreads = 256;
 constexpr int kRsItems = 8;
 constexpr int kRsTile = kRsThreads * kRsItems;
+constexpr int kRsMaxBits = 9;
+constexpr int kRsMaxBins = 1 << kRsMaxBits;
 
+// per-tile digit counts, hist[digit * ntiles + tile]
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(
-    const uint32_t *__restrict__ keys, int N, int shift, uint32_t *__restrict__ hist,
-    int ntiles) {
-    __shared__ uint32_t h[256];
-    int tid = threadIdx.x;
-    h[tid] = 0;
+    const uint32_t *__restrict__ keys, int N, int shift, int dbits,
+    uint32_t *__restrict__ hist, int ntiles) {
+    __shared__ uint32_t h[kRsMaxBins];
+    const int tid = threadIdx.x, nb = 1 << dbits;
+    const uint32_t mask = (uint32_t)nb - 1u;
+    for (int b = tid; b < nb; b += kRsThreads) h[b] = 0;
     __syncthreads();
     int base = blockIdx.x * kRsTile;
     for (int r = 0; r < kRsItems; ++r) {
         int e = base + r * kRsThreads + tid;
-        if (e < N) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+        if (e < N) atomicAdd(&h[(keys[e] >> shift) & mask], 1u);
     }
     __syncthreads();
-    hist[tid * ntiles + blockIdx.x] = h[tid];
+    for (int b = tid; b < nb; b += kRsThreads) hist[b * ntiles + blockIdx.x] = h[b];
 }
 
+// stable scatter of one tile: a key's rank among the equal digits of its
+// wavefront from dbits ballots, the wavefronts before it from LDS counts
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, int N, int shift,
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, int N, int shift, int dbits,
     const uint32_t *__restrict__ offs, int ntiles) {
     constexpr int W = kRsThreads / 64;
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[W][256];
-    int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    run[tid] = offs[tid * ntiles + blockIdx.x];
-    for (int w = 0; w < W; ++w) wcnt[w][tid] = 0;
+    __shared__ uint32_t run[kRsMaxBins];
+    __shared__ uint32_t wcnt[W][kRsMaxBins];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, nb = 1 << dbits;
+    const uint32_t mask = (uint32_t)nb - 1u;
+    for (int b = tid; b < nb; b += kRsThreads) {
+        run[b] = offs[b * ntiles + blockIdx.x];
+        for (int w = 0; w < W; ++w) wcnt[w][b] = 0;
+    }
     __syncthreads();
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int base = blockIdx.x * kRsTile;
@@ -117,9 +128,9 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
         bool valid = e < N;
         uint32_t k = valid ? kin[e] : 0u;
         uint32_t v = valid ? vin[e] : 0u;
-        uint32_t d = (k >> shift) & 255u;
+        uint32_t d = (k >> shift) & mask;
         uint64_t m = __ballot(valid);
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < dbits; ++b) {
             bool bit = (d >> b) & 1u;
             uint64_t bb = __ballot(valid && bit);
             m &= bit ? bb : ~bb;
@@ -134,12 +145,14 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
             vout[pos] = v;
         }
         __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < W; ++w) {
-            add += wcnt[w][tid];
-            wcnt[w][tid] = 0;
+        for (int b = tid; b < nb; b += kRsThreads) {
+            uint32_t add = 0;
+            for (int w = 0; w < W; ++w) {
+                add += wcnt[w][b];
+                wcnt[w][b] = 0;
+            }
+            run[b] += add;
         }
-        run[tid] += add;
         __syncthreads();
     }
 }
@@ -213,7 +226,7 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, int n, uint32_t *tmp,
 
 size_t radix_sort_scratch_words(int N) {
     int ntiles = (N + kRsTile - 1) / kRsTile;
-    size_t h = (size_t)256 * ntiles;
+    size_t h = (size_t)kRsMaxBins * ntiles;
     return 2 * h + (h + kScanBlock - 1) / kScanBlock + 16;
 }
 
@@ -222,15 +235,18 @@ size_t radix_sort_scratch_words(int N) {
 void radix_sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int N,
                       int bits, uint32_t *scratch, hipStream_t s) {
     int ntiles = (N + kRsTile - 1) / kRsTile;
-    int h = 256 * ntiles;
-    uint32_t *hist = scratch, *offs = scratch + h, *tmp = scratch + 2 * h;
-    int passes = (bits + 7) / 8;
+    const int passes = (bits + kRsMaxBits - 1) / kRsMaxBits;
+    const int dbits = passes ? (bits + passes - 1) / passes : 0;
+    const int h = (1 << dbits) * ntiles;
+    uint32_t *hist = scratch, *offs = scratch + (size_t)kRsMaxBins * ntiles,
+             *tmp = scratch + 2 * (size_t)kRsMaxBins * ntiles;
     for (int p = 0; p < passes; ++p) {
-        int shift = 8 * p;
-        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kRsThreads), 0, s, k0, N, shift, hist, ntiles);
+        int shift = dbits * p;
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(kRsThreads), 0, s, k0, N, shift, dbits,
+                           hist, ntiles);
         scan_exclusive_u32(hist, offs, h, tmp, s);
         hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(kRsThreads), 0, s, k0, v0, k1, v1, N,
-                           shift, offs, ntiles);
+                           shift, dbits, offs, ntiles);
         uint32_t *t = k0; k0 = k1; k1 = t;
         t = v0; v0 = v1; v1 = t;
     }
