@@ -541,10 +541,16 @@ __global__ __launch_bounds__(256) void k_dt_finalize(const double *__restrict__ 
                                                      double *__restrict__ dt_hist, int step,
                                                      int *__restrict__ status) {
     __shared__ double sh[256];
+    // eight block minima per thread in flight together; the minimum does not
+    // depend on the order (a NaN is skipped either way)
+    constexpr int U = 8;
     double v = DBL_MAX;
-    for (int k = threadIdx.x; k < nblocks; k += 256) {
-        double o = dt_block[k];
-        v = o < v ? o : v;
+    for (int k0 = threadIdx.x; k0 < nblocks; k0 += 256 * U) {
+        double o[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) o[u] = k0 + 256 * u < nblocks ? dt_block[k0 + 256 * u] : DBL_MAX;
+#pragma unroll
+        for (int u = 0; u < U; ++u) v = o[u] < v ? o[u] : v;
     }
     sh[threadIdx.x] = v;
     __syncthreads();
