@@ -212,17 +212,20 @@ def test_stages_fp32(name, hip, oracle_mod):
     sim.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("name", ["khrand48", "blast16"])
-def test_work_order_invariance(name, hip):
+def test_work_order_invariance(name, prec, hip):
     """Cell order and the caller's order visit particles differently but sum
     in the same (reference) order: neighbour lists, derived fields and three
-    steps agree bitwise."""
-    from meshlesshydro_amd._abi import MLH_ORDER_CALLER, MLH_ORDER_CELL
+    steps agree bitwise, in FP64 and in FP32 (whose step fuses the density,
+    with the images' terms, into the search in either order)."""
+    from meshlesshydro_amd._abi import MLH_FP32, MLH_FP64, MLH_ORDER_CALLER, MLH_ORDER_CELL
     out = []
     for order in (MLH_ORDER_CELL, MLH_ORDER_CALLER):
         # debug_level 0: the random KH set has a few negative reconstructed
         # face pressures, which abort a run at the reference's level 1
-        p, s = CASES[name](work_order=order, debug_level=0)
+        p, s = CASES[name](work_order=order, debug_level=0,
+                           precision=MLH_FP32 if prec == "fp32" else MLH_FP64)
         sim = hip.Simulation(p, s)
         sim.neighbours()
         sim.density()
